@@ -1,0 +1,13 @@
+"""MI355X-native bispectrum-multipole estimator (hot path of BARBIE/BEAM).
+
+The estimator runs in libbarbie_amd.so (hand-written HIP kernels for gfx950
+behind the C ABI of include/barbie.h); this package holds the Python side of
+that boundary and the seeded input generator.
+"""
+from .native import (Geometry, Plan, Estimator, BarbieError, normalise, lib, LIB_PATH, BIN_PATH,
+                     NPOLES, PRECISIONS, RINGS_REFERENCE, RINGS_UNIFORM, RINGS_EXPLICIT)
+from .field import make_field, write_field, read_field, seed_for
+
+__all__ = ["Geometry", "Plan", "Estimator", "BarbieError", "normalise", "lib", "LIB_PATH",
+           "BIN_PATH", "NPOLES", "PRECISIONS", "RINGS_REFERENCE", "RINGS_UNIFORM",
+           "RINGS_EXPLICIT", "make_field", "write_field", "read_field", "seed_for"]

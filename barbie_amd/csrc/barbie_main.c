@@ -1,0 +1,181 @@
+/*
+ * barbie -- drop-in replacement of the reference estimator executable
+ * (barbie.c:26-399) running on one MI355X through libbarbie_amd.
+ *
+ *   barbie NN NRi UL_k1 LL_k1 BW_k1
+ *
+ * Same arguments and usage message (barbie.c:58-65), same input file
+ * field_N<NN>_R<NRi>.txt in the working directory (barbie.c:178), same 16
+ * output files ResFiles_N<NN>/BS_test_ii<ii>_N<NN>_R<NRi>.txt
+ * (barbie.c:383-397) and timing log ResFiles_N<NN>/time_<NN>_R<NRi>.txt.
+ * Additionally writes ResFiles_N<NN>/BS_bins_N<NN>_R<NRi>.f64, the three
+ * un-normalised FP64 bin arrays (bin_BS_real, bin_normalization, bin_Biana),
+ * for comparisons beyond the 7 digits of the text files.
+ *
+ * Values the reference hard-codes can be overridden from the environment
+ * (defaults are the reference's):
+ *   BARBIE_NMAX (100), BARBIE_NR (40), BARBIE_NMTBIN (5),
+ *   BARBIE_RING_RULE (reference | uniform), BARBIE_RING_WIDTH (1.0),
+ *   BARBIE_LAST_RING_TO_NMAX (1), BARBIE_PRECISION (fp64 | fp32 | bf16x3).
+ * Invalid arguments exit with status 2 and a message (the reference does not
+ * validate them, docs/REVIEW.md:10); missing input exits with status 3;
+ * device errors with status 4.
+ */
+#include <errno.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+#include <sys/time.h>
+
+#include "barbie.h"
+
+static double now(void)
+{
+    struct timeval tv;
+    gettimeofday(&tv, NULL);
+    return tv.tv_sec + 1e-6 * tv.tv_usec;
+}
+
+static int env_int(const char *name, int def)
+{
+    const char *v = getenv(name);
+    return (v && *v) ? atoi(v) : def;
+}
+
+static double env_double(const char *name, double def)
+{
+    const char *v = getenv(name);
+    return (v && *v) ? atof(v) : def;
+}
+
+int main(int argc, char *argv[])
+{
+    long NN;
+    int NRi, UL_k1, LL_k1, BW_k1;
+    if (argc != 6) {
+        printf("Usage: %s NN(384) NRi(1) UL_k1(Nmax) LL_k1(Nmax-10) BW_k1(1)\n", argv[0]);
+        return 1;
+    }
+    if (sscanf(argv[1], "%ld", &NN) != 1 || sscanf(argv[2], "%d", &NRi) != 1 ||
+        sscanf(argv[3], "%d", &UL_k1) != 1 || sscanf(argv[4], "%d", &LL_k1) != 1 ||
+        sscanf(argv[5], "%d", &BW_k1) != 1) {
+        fprintf(stderr, "barbie: arguments must be integers\n");
+        return 2;
+    }
+    printf("\n Calculating BS Multipoles for Realization(NRi) = %d and Grids(NN) = %ld\n", NRi, NN);
+    if (NN <= 0 || NN > 4096) {
+        fprintf(stderr, "barbie: NN out of range\n");
+        return 2;
+    }
+
+    bb_geometry g;
+    bb_geometry_reference((int)NN, UL_k1, LL_k1, BW_k1, &g);
+    g.Nmax = env_int("BARBIE_NMAX", g.Nmax);
+    g.Nr = env_int("BARBIE_NR", g.Nr);
+    g.n_mtbin = env_int("BARBIE_NMTBIN", g.n_mtbin);
+    const char *rule = getenv("BARBIE_RING_RULE");
+    if (rule && strcmp(rule, "uniform") == 0) g.ring_rule = BB_RINGS_UNIFORM;
+    g.ring_width = env_double("BARBIE_RING_WIDTH", g.ring_width);
+    g.last_ring_to_nmax = env_int("BARBIE_LAST_RING_TO_NMAX", g.last_ring_to_nmax);
+    bb_precision prec = BB_FP64;
+    const char *ps = getenv("BARBIE_PRECISION");
+    if (ps && strcmp(ps, "fp32") == 0) prec = BB_FP32;
+    if (ps && strcmp(ps, "bf16x3") == 0) prec = BB_BF16X3;
+
+    double t0 = now();
+    bb_plan *plan = NULL;
+    int rc = bb_plan_create(&g, &plan);
+    if (rc) {
+        fprintf(stderr, "barbie: %s\n", bb_last_error());
+        return 2;
+    }
+    bb_plan_stats st;
+    bb_plan_stats_get(plan, &st);
+    printf("\n All parameters are with me including mu-t bins=%d, k1 bins=%d  !!\n", st.n_mtbin,
+           st.n_k1bin);
+    double *lo = malloc(sizeof(double) * st.Nr), *hi = malloc(sizeof(double) * st.Nr),
+           *av = malloc(sizeof(double) * st.Nr);
+    bb_plan_ring_edges(plan, lo, hi, av);
+    for (int i = 0; i < st.Nr; ++i) printf("\t%f\t%f\t%f\n", lo[i], hi[i], av[i]);
+    printf("\n Total combinations of rings on a plane forming closed triangle = %d\n", st.rcount);
+    printf(" Contractions = %lld over %d plane groups, %d plane-ring fields\n", st.ncontr,
+           st.ngroups, st.nfields_used);
+
+    /* Input, barbie.c:176-185 (read in one block). */
+    char fname[1024];
+    snprintf(fname, sizeof fname, "field_N%ld_R%d.txt", NN, NRi);
+    FILE *fp = fopen(fname, "rb");
+    if (!fp) {
+        fprintf(stderr, "barbie: cannot open %s: %s\n", fname, strerror(errno));
+        return 3;
+    }
+    size_t nd = (size_t)NN * NN * (NN + 2);
+    double *field = malloc(sizeof(double) * nd);
+    if (!field) {
+        fprintf(stderr, "barbie: out of memory\n");
+        return 3;
+    }
+    size_t got = fread(field, sizeof(double), nd, fp);
+    fclose(fp);
+    if (got != nd) {
+        fprintf(stderr, "barbie: %s holds %zu doubles, expected %zu\n", fname, got, nd);
+        return 3;
+    }
+
+    double t_plan = now();
+    bb_estimator *est = NULL;
+    rc = bb_estimator_create(plan, prec, &est);
+    if (rc) {
+        fprintf(stderr, "barbie: %s\n", bb_last_error());
+        return 4;
+    }
+    size_t nb = (size_t)st.nbins * BB_NPOLES;
+    double *bins = calloc(3 * nb, sizeof(double));
+    double t_est = now();
+    rc = bb_estimate(est, field, bins, bins + nb, bins + 2 * nb);
+    if (rc) {
+        fprintf(stderr, "barbie: %s\n", bb_last_error());
+        return 4;
+    }
+    double t_done = now();
+    double ms[4];
+    bb_estimator_timings(est, ms);
+
+    char dir[512];
+    snprintf(dir, sizeof dir, "ResFiles_N%ld", NN);
+    mkdir(dir, 0777);
+    snprintf(fname, sizeof fname, "%s/time_%ld_R%d.txt", dir, NN, NRi);
+    fp = fopen(fname, "w");
+    if (fp) {
+        fprintf(fp, "plan (ring triples, contraction list) in  time %f\n", t_plan - t0);
+        fprintf(fp, "device setup and triangle counts in  time %f\n", t_est - t_plan);
+        fprintf(fp, "plane gather in  time %f\n", ms[0] * 1e-3);
+        fprintf(fp, "plane-ring IFFTs in  time %f\n", ms[1] * 1e-3);
+        fprintf(fp, "contraction in  time %f\n", ms[2] * 1e-3);
+        fprintf(fp, "binning in  time %f\n", ms[3] * 1e-3);
+        fprintf(fp, "Total time %f\n", t_done - t0);
+        fclose(fp);
+    }
+    printf("Total time %f\n", t_done - t0);
+
+    rc = bb_write_outputs(plan, bins, bins + nb, bins + 2 * nb, dir, NRi);
+    if (rc) {
+        fprintf(stderr, "barbie: %s\n", bb_last_error());
+        return 3;
+    }
+    snprintf(fname, sizeof fname, "%s/BS_bins_N%ld_R%d.f64", dir, NN, NRi);
+    fp = fopen(fname, "wb");
+    if (fp) {
+        fwrite(bins, sizeof(double), 3 * nb, fp);
+        fclose(fp);
+    }
+    bb_estimator_destroy(est);
+    bb_plan_destroy(plan);
+    free(bins);
+    free(field);
+    free(lo);
+    free(hi);
+    free(av);
+    return 0;
+}

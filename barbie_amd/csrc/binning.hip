@@ -1,0 +1,153 @@
+// Shape/size binning, Y_lm weights and accumulation (FP64).
+// Replaces barbie.c:287-298 (triangle parameters), :301-305 (analytic
+// comparison), :343-363 (bins, zero-signal rule, accumulation) and cal_coeff
+// (funcs.c:708-745).
+//
+// Contractions arrive sorted by bin; one wavefront owns one bin, each lane
+// accumulates a strided subset of the bin's contractions in registers and
+// the wave reduces its 48 sums in a fixed tree order (deterministic).
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+
+namespace bb {
+
+// 16 orientation weights bs[] and norms norm[] of (px, pz), funcs.c:708-745.
+__device__ __forceinline__ void cal_coeff(double px, double pz, double *bs, double *norm)
+{
+    const double x2 = px * px, x4 = x2 * x2, x6 = x4 * x2;
+    const double z2 = pz * pz, z4 = z2 * z2, z6 = z4 * z2;
+    const double om = -1. + z2;  // (-1 + pz^2)
+    const double u = 1. - z2;    // (1 - pz^2)
+    bs[0] = 1.;
+    bs[1] = 2. * (3. * z2 - 1.);
+    bs[2] = -sqrt(2. / 3.) * px * pz;
+    bs[3] = 2. * sqrt(0.6666666666666666) * (-1. + 2. * x2 + z2);
+    bs[4] = 8. * (3. - 30. * z2 + 35. * z4);
+    bs[5] = (-4. * px * pz * (-3. + 7. * z2)) / sqrt(5.);
+    bs[6] = 4. * sqrt(0.4) * (-1 + 2. * x2 + z2) * (-1. + 7. * z2);
+    bs[7] = (-4. * px * pz * (-3. + 4. * x2 + 3. * z2)) / sqrt(35.);
+    bs[8] = 8. * sqrt(0.05714285714285714) * (8. * x4 + 8. * x2 * om + om * om);
+    bs[9] = 16. * (-5. + 105. * z2 - 315. * z4 + 231. * z6);
+    bs[10] = -8. * sqrt(0.09523809523809523) * px * pz * (5. - 30. * z2 + 33. * z4);
+    bs[11] = (32. * (-1. + 2. * x2 + z2) * (1. - 18. * z2 + 33. * z4)) / sqrt(105.0);
+    bs[12] = (-16. * px * pz * (-3. + 4. * x2 + 3. * z2) * (-3. + 11. * z2)) / sqrt(105.0);
+    bs[13] = (16. * sqrt(0.2857142857142857) * (-1. + 11. * z2) * (8. * x4 + 8. * x2 * om + om * om)) / 3.;
+    bs[14] = (-16 * pz * (16 * x4 * px + 20 * x2 * px * om + 5 * px * om * om)) / (3. * sqrt(77.0));
+    bs[15] = (32. * (32. * x6 + 48. * x4 * om + 18. * x2 * om * om + om * om * om)) / sqrt(231.);
+
+    const double a1 = 3. * z2 - 1.;
+    const double a4 = 3. - 30. * z2 + 35. * z4;
+    const double a5 = -3. * pz + 7. * z2 * pz;
+    const double a6 = -1. + 7. * z2;
+    const double a9 = -5. + 105. * z2 - 315. * z4 + 231. * z6;
+    const double a10 = 5. * pz - 30. * z2 * pz + 33. * z4 * pz;
+    const double a11 = 1. - 18. * z2 + 33. * z4;
+    const double a12 = -3. * pz + 11. * z2 * pz;
+    const double a13 = -1 + 11 * z2;
+    const double om2 = om * om, u2 = u * u, u3 = u2 * u;
+    norm[0] = 1.;
+    norm[1] = a1 * a1;
+    norm[2] = z2 * u;
+    norm[3] = om2;
+    norm[4] = a4 * a4;
+    norm[5] = u * a5 * a5;
+    norm[6] = om2 * a6 * a6;
+    norm[7] = z2 * u3;
+    norm[8] = om2 * om2;
+    norm[9] = a9 * a9;
+    norm[10] = u * a10 * a10;
+    norm[11] = om2 * a11 * a11;
+    norm[12] = u3 * a12 * a12;
+    norm[13] = om2 * om2 * a13 * a13;
+    norm[14] = z2 * u3 * u2;
+    norm[15] = om2 * om2 * om2;
+}
+
+__global__ __launch_bounds__(64) void k_binning(const int32_t *__restrict__ seg,
+                                                const BinRec *__restrict__ recs,
+                                                const double *__restrict__ S,
+                                                const double *__restrict__ num,
+                                                const int *__restrict__ nzflag,
+                                                const double *__restrict__ avg, BinParams bp,
+                                                double *__restrict__ out_bs,
+                                                double *__restrict__ out_norm,
+                                                double *__restrict__ out_biana)
+{
+    const int b = blockIdx.x;
+    const int lane = threadIdx.x;
+    double abs_[16], anm[16], abi[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) abs_[k] = anm[k] = abi[k] = 0.0;
+
+    for (int c = seg[b] + lane; c < seg[b + 1]; c += 64) {
+        const BinRec R = recs[c];
+        const double a1 = avg[R.r1], a2 = avg[R.r2], a3 = avg[R.r3];
+        const double p1 = R.p1, p2 = R.p2, p3 = R.p3;
+        double K[3][2] = {{sqrt(a1 * a1 + p1 * p1), p1},
+                          {sqrt(a2 * a2 + p2 * p2), p2},
+                          {sqrt(a3 * a3 + p3 * p3), p3}};
+        // funcs.c:113-126: descending, strict '<'
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2 - i; j++)
+                if (K[j][0] < K[j + 1][0]) {
+                    double s1 = K[j][0], s2 = K[j][1];
+                    K[j][0] = K[j + 1][0]; K[j][1] = K[j + 1][1];
+                    K[j + 1][0] = s1; K[j + 1][1] = s2;
+                }
+        const double T = K[1][0] / K[0][0];
+        const double MU = 0.5 * (T + 1. / T - (K[2][0] / K[0][0]) * (K[2][0] / K[1][0]));
+        const double pz = K[0][1] / K[0][0];
+        const double px = ((K[1][1] / K[1][0]) + MU * pz) / (sqrt(1. - (MU * MU)));
+        double Num = num[R.rp];
+        const double m1 = K[0][1] / K[0][0], m2 = K[1][1] / K[1][0], m3 = K[2][1] / K[2][0];
+        const double k0 = bp.Cunits * K[0][0], k1 = bp.Cunits * K[1][0], k2 = bp.Cunits * K[2][0];
+        const double P0 = 1.0 / (k0 * k0), P1 = 1.0 / (k1 * k1), P2 = 1.0 / (k2 * k2);
+        double Biana = 2. * bp.ff * (P0 * P1 + P1 * P2 + P2 * P0);
+        Biana = Biana * (1 + bp.beta * m1 * m1) * (1 + bp.beta * m2 * m2) * (1 + bp.beta * m3 * m3);
+        const double Sv = S[R.cidx];
+        // barbie.c:355-357.  The sum is exactly zero iff one of its three
+        // fields is identically zero; the fields' zero flags decide it.
+        if (!nzflag[R.f1] || !nzflag[R.f2] || !nzflag[R.f3]) {
+            Biana = 0.;
+            Num = 0.;
+        }
+        double bs[16], nm[16];
+        cal_coeff(px, pz, bs, nm);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            abs_[k] += Sv * bs[k];
+            abi[k] += Biana * Num * bs[k];
+            anm[k] += Num * nm[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        double v0 = abs_[k], v1 = anm[k], v2 = abi[k];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            v0 += __shfl_xor(v0, off, 64);
+            v1 += __shfl_xor(v1, off, 64);
+            v2 += __shfl_xor(v2, off, 64);
+        }
+        if (lane == 0) {
+            if (out_bs) out_bs[(long long)b * 16 + k] = v0;
+            if (out_norm) out_norm[(long long)b * 16 + k] = v1;
+            if (out_biana) out_biana[(long long)b * 16 + k] = v2;
+        }
+    }
+}
+
+hipError_t launch_binning(const int32_t *seg, const BinRec *recs, const double *S,
+                          const double *num, const int *nzflag, const double *avg,
+                          const BinParams &bp, double *bins_bs, double *bins_norm,
+                          double *bins_biana, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_binning, dim3(bp.nbins), dim3(64), 0, s, seg, recs, S, num, nzflag, avg,
+                       bp, bins_bs, bins_norm, bins_biana);
+    return hipGetLastError();
+}
+
+}  // namespace bb

@@ -1,0 +1,263 @@
+"""ctypes bindings of libbarbie_amd.so (the C ABI declared in include/barbie.h).
+
+The product path is the HIP library; this module only marshals arguments.
+Loading fails loudly when the library has not been built -- there is no
+fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field as dc_field
+from typing import Optional
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libbarbie_amd.so")
+BIN_PATH = os.path.join(PKG_DIR, "bin", "barbie")
+
+NPOLES = 16
+BB_FP64, BB_FP32, BB_BF16X3 = 0, 1, 2
+PRECISIONS = {"fp64": BB_FP64, "fp32": BB_FP32, "bf16x3": BB_BF16X3}
+RINGS_REFERENCE, RINGS_UNIFORM, RINGS_EXPLICIT = 0, 1, 2
+
+# C-ABI symbols declared in include/barbie.h (checked by the CPU tests).
+EXPORTED = [
+    "bb_last_error", "bb_version", "bb_geometry_reference", "bb_plan_create",
+    "bb_plan_destroy", "bb_plan_stats_get", "bb_plan_ring_edges", "bb_plan_triples",
+    "bb_plan_contractions", "bb_estimator_create", "bb_estimator_destroy",
+    "bb_estimator_footprint", "bb_estimate", "bb_estimate_device",
+    "bb_estimator_contraction_sums", "bb_estimator_triangle_counts", "bb_estimator_field",
+    "bb_estimator_timings", "bb_write_outputs",
+]
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class BBGeometry(ctypes.Structure):
+    _fields_ = [("NN", ctypes.c_int), ("Nmax", ctypes.c_int), ("Nr", ctypes.c_int),
+                ("ring_rule", ctypes.c_int), ("ring_width", ctypes.c_double),
+                ("last_ring_to_nmax", ctypes.c_int), ("ring_low", _dp), ("ring_high", _dp),
+                ("n_mtbin", ctypes.c_int), ("UL_k1", ctypes.c_int), ("LL_k1", ctypes.c_int),
+                ("BW_k1", ctypes.c_int), ("LL", ctypes.c_double), ("ff", ctypes.c_double),
+                ("beta", ctypes.c_double)]
+
+
+class BBPlanStats(ctypes.Structure):
+    _fields_ = [("NN", ctypes.c_int), ("Nmax", ctypes.c_int), ("Nr", ctypes.c_int),
+                ("n_mtbin", ctypes.c_int), ("n_k1bin", ctypes.c_int), ("rcount", ctypes.c_int),
+                ("ncontr", ctypes.c_longlong), ("npairs", ctypes.c_longlong),
+                ("ngroups", ctypes.c_int), ("nfields_used", ctypes.c_int),
+                ("nbins", ctypes.c_int)]
+
+
+class BarbieError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Loads libbarbie_amd.so; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                               "or `make -C barbie_amd/csrc`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.bb_last_error.restype = ctypes.c_char_p
+        L.bb_version.restype = ctypes.c_char_p
+        L.bb_geometry_reference.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(BBGeometry)]
+        L.bb_plan_create.argtypes = [ctypes.POINTER(BBGeometry), ctypes.POINTER(ctypes.c_void_p)]
+        L.bb_plan_destroy.argtypes = [ctypes.c_void_p]
+        L.bb_plan_destroy.restype = None
+        L.bb_plan_stats_get.argtypes = [ctypes.c_void_p, ctypes.POINTER(BBPlanStats)]
+        L.bb_plan_ring_edges.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.bb_plan_triples.argtypes = [ctypes.c_void_p, _ip]
+        L.bb_plan_contractions.argtypes = [ctypes.c_void_p, _ip]
+        L.bb_estimator_create.argtypes = [ctypes.c_void_p, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_void_p)]
+        L.bb_estimator_destroy.argtypes = [ctypes.c_void_p]
+        L.bb_estimator_destroy.restype = None
+        L.bb_estimator_footprint.argtypes = [ctypes.c_void_p, ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_size_t)]
+        L.bb_estimate.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
+        L.bb_estimate_device.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 4
+        L.bb_estimator_contraction_sums.argtypes = [ctypes.c_void_p, _dp]
+        L.bb_estimator_triangle_counts.argtypes = [ctypes.c_void_p, _dp]
+        L.bb_estimator_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+        L.bb_estimator_timings.argtypes = [ctypes.c_void_p, _dp]
+        L.bb_write_outputs.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_char_p,
+                                       ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise BarbieError(rc, lib().bb_last_error().decode())
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+@dataclass
+class Geometry:
+    """Geometry and binning of one estimate (bb_geometry)."""
+    NN: int
+    UL_k1: int
+    LL_k1: int = 1
+    BW_k1: int = 1
+    Nmax: int = 100
+    Nr: int = 40
+    ring_rule: int = RINGS_REFERENCE
+    ring_width: float = 1.0
+    last_ring_to_nmax: int = 1
+    ring_low: Optional[np.ndarray] = None
+    ring_high: Optional[np.ndarray] = None
+    n_mtbin: int = 5
+    LL: float = 1.0
+    ff: float = 0.1
+    beta: float = 1.0
+    _keep: list = dc_field(default_factory=list, repr=False)
+
+    def to_c(self) -> BBGeometry:
+        g = BBGeometry()
+        g.NN, g.Nmax, g.Nr = self.NN, self.Nmax, self.Nr
+        g.ring_rule, g.ring_width = self.ring_rule, self.ring_width
+        g.last_ring_to_nmax = self.last_ring_to_nmax
+        if self.ring_low is not None:
+            lo = np.ascontiguousarray(self.ring_low, dtype=np.float64)
+            hi = np.ascontiguousarray(self.ring_high, dtype=np.float64)
+            self._keep[:] = [lo, hi]
+            g.ring_low, g.ring_high = _p(lo), _p(hi)
+        g.n_mtbin = self.n_mtbin
+        g.UL_k1, g.LL_k1, g.BW_k1 = self.UL_k1, self.LL_k1, self.BW_k1
+        g.LL, g.ff, g.beta = self.LL, self.ff, self.beta
+        return g
+
+    @property
+    def n_k1bin(self) -> int:
+        return int((self.UL_k1 - self.LL_k1) / (1.0 * self.BW_k1))
+
+    @property
+    def nbins(self) -> int:
+        return self.n_k1bin * self.n_mtbin * self.n_mtbin
+
+
+class Plan:
+    """Host-side geometry plan (bb_plan)."""
+
+    def __init__(self, geom: Geometry):
+        self.geom = geom
+        self._g = geom.to_c()
+        h = ctypes.c_void_p()
+        _check(lib().bb_plan_create(ctypes.byref(self._g), ctypes.byref(h)))
+        self.handle = h
+        s = BBPlanStats()
+        _check(lib().bb_plan_stats_get(h, ctypes.byref(s)))
+        self.stats = {k: getattr(s, k) for k, _ in BBPlanStats._fields_}
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.bb_plan_destroy(h)
+            self.handle = None
+
+    def ring_edges(self):
+        Nr = self.stats["Nr"]
+        lo, hi, av = np.zeros(Nr), np.zeros(Nr), np.zeros(Nr)
+        _check(lib().bb_plan_ring_edges(self.handle, _p(lo), _p(hi), _p(av)))
+        return lo, hi, av
+
+    def triples(self) -> np.ndarray:
+        out = np.zeros((self.stats["rcount"], 3), dtype=np.int32)
+        _check(lib().bb_plan_triples(self.handle, out.ctypes.data_as(_ip)))
+        return out
+
+    def contractions(self) -> np.ndarray:
+        """(ncontr, 8) int32: p1, r1, p2, r2, p3, r3, rp, flat_bin."""
+        out = np.zeros((self.stats["ncontr"], 8), dtype=np.int32)
+        _check(lib().bb_plan_contractions(self.handle, out.ctypes.data_as(_ip)))
+        return out
+
+    def footprint(self, precision: str) -> int:
+        b = ctypes.c_size_t()
+        _check(lib().bb_estimator_footprint(self.handle, PRECISIONS[precision], ctypes.byref(b)))
+        return b.value
+
+    def write_outputs(self, bs, norm, biana, outdir: str, NRi: int):
+        bs, norm, biana = (np.ascontiguousarray(a, dtype=np.float64) for a in (bs, norm, biana))
+        _check(lib().bb_write_outputs(self.handle, _p(bs), _p(norm), _p(biana),
+                                      outdir.encode(), NRi))
+
+
+class Estimator:
+    """Device estimator (bb_estimator) for one plan and precision."""
+
+    def __init__(self, plan: Plan, precision: str = "fp64"):
+        self.plan = plan
+        self.precision = precision
+        h = ctypes.c_void_p()
+        _check(lib().bb_estimator_create(plan.handle, PRECISIONS[precision], ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.bb_estimator_destroy(h)
+        self.handle = None
+
+    def __del__(self):
+        self.close()
+
+    def estimate(self, field_r2c: np.ndarray):
+        """field_r2c: float64 (NN, NN, NN+2).  Returns (bs, norm, biana),
+        each (n_k1bin, n_mtbin, n_mtbin, 16) float64, un-normalised."""
+        g = self.plan.geom
+        f = np.ascontiguousarray(field_r2c, dtype=np.float64)
+        assert f.size == g.NN * g.NN * (g.NN + 2), "field must be NN x NN x (NN+2) doubles"
+        shape = (g.n_k1bin, g.n_mtbin, g.n_mtbin, NPOLES)
+        bs, nm, bi = (np.zeros(shape) for _ in range(3))
+        _check(lib().bb_estimate(self.handle, _p(f), _p(bs), _p(nm), _p(bi)))
+        return bs, nm, bi
+
+    def estimate_device(self, d_field: int, d_bs: int, d_norm: int, d_biana: int):
+        """All arguments are device addresses (e.g. torch tensor data_ptr())."""
+        _check(lib().bb_estimate_device(self.handle, ctypes.c_void_p(d_field),
+                                        ctypes.c_void_p(d_bs), ctypes.c_void_p(d_norm),
+                                        ctypes.c_void_p(d_biana)))
+
+    def contraction_sums(self) -> np.ndarray:
+        S = np.zeros(self.plan.stats["ncontr"])
+        _check(lib().bb_estimator_contraction_sums(self.handle, _p(S)))
+        return S
+
+    def triangle_counts(self) -> np.ndarray:
+        n = np.zeros(self.plan.stats["rcount"])
+        _check(lib().bb_estimator_triangle_counts(self.handle, _p(n)))
+        return n
+
+    def field(self, plane: int, ring: int) -> np.ndarray:
+        NN = self.plan.geom.NN
+        out = np.zeros((NN, NN), dtype=np.complex128)
+        _check(lib().bb_estimator_field(self.handle, plane, ring, _p(out.view(np.float64))))
+        return out
+
+    def timings(self) -> np.ndarray:
+        t = np.zeros(4)
+        _check(lib().bb_estimator_timings(self.handle, _p(t)))
+        return t
+
+
+def normalise(geom: Geometry, bs, norm, biana):
+    """barbie.c:390-391: B = bs / (vol * norm), Biana / norm (per multipole)."""
+    vol = (geom.NN * geom.LL) ** 3
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return bs / (vol * norm), biana / norm

@@ -1,0 +1,152 @@
+/*
+ * barbie.h -- C ABI of the MI355X bispectrum-multipole estimator library
+ * (libbarbie_amd.so).
+ *
+ * The reference program (barbie.c) has no library interface: its stages talk
+ * through globals and files.  The entry points below replace those stages one
+ * for one; each declaration names the reference code it stands in for.
+ * Plain C types only.  Every function returns 0 on success or a negative
+ * BB_E* code, and bb_last_error() gives a message for the calling thread.
+ * The library never calls exit().
+ *
+ * Threading: a bb_plan is immutable after creation and may be shared between
+ * threads.  A bb_estimator owns device memory and one HIP stream; calls on
+ * one estimator must be serialised by the caller.  Different estimators may
+ * be used from different threads.
+ */
+#ifndef BARBIE_H
+#define BARBIE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BB_NPOLES 16
+
+enum {
+    BB_OK = 0,
+    BB_E_INVALID = -1,   /* invalid argument or geometry */
+    BB_E_NOMEM = -2,     /* host allocation failed */
+    BB_E_DEVICE = -3,    /* HIP error (no device, launch failure, ...) */
+    BB_E_DEVMEM = -4,    /* device allocation failed / does not fit in HBM */
+    BB_E_IO = -5,        /* file error */
+    BB_E_RANGE = -6      /* a triangle falls outside the bin grid */
+};
+
+/* Ring-edge rules. */
+enum {
+    BB_RINGS_REFERENCE = 0, /* barbie.c:110-130: blocks of ten, width b+1 in block b,
+                               -0.1 shift, last ring closed at Nmax; truncated at Nr */
+    BB_RINGS_UNIFORM = 1,   /* same construction with a fixed width */
+    BB_RINGS_EXPLICIT = 2   /* caller-supplied low/high edges */
+};
+
+/* Arithmetic of the contraction stage. */
+typedef enum {
+    BB_FP64 = 0,    /* FP64 fields, FP64 contraction (the FP64 GPU oracle) */
+    BB_FP32 = 1,    /* FP32 fields and contraction, FP64 across tiles */
+    BB_BF16X3 = 2   /* FP32 fields, BF16x3 split products on MFMA, FP32 in tiles,
+                       FP64 across tiles */
+} bb_precision;
+
+/* Geometry and binning of one estimate.  Replaces the constants hard-coded at
+ * barbie.c:58-84 and the ring construction at barbie.c:110-130. */
+typedef struct bb_geometry {
+    int NN;              /* grid size: even, factors 2,3,5 only */
+    int Nmax;            /* kz planes 0..Nmax-1 (barbie.c:71) */
+    int Nr;              /* number of rings (barbie.c:72) */
+    int ring_rule;       /* BB_RINGS_* */
+    double ring_width;   /* BB_RINGS_UNIFORM only */
+    int last_ring_to_nmax; /* BB_RINGS_UNIFORM: 1 closes the last ring at Nmax */
+    const double *ring_low;  /* BB_RINGS_EXPLICIT: Nr lower edges */
+    const double *ring_high; /* BB_RINGS_EXPLICIT: Nr upper edges */
+    int n_mtbin;         /* shape bins per axis (barbie.c:84) */
+    int UL_k1, LL_k1, BW_k1; /* size bins (argv[3..5], barbie.c:63-65) */
+    double LL;           /* cell size (barbie.c:67) */
+    double ff, beta;     /* analytic-comparison constants (barbie.c:79-80) */
+} bb_geometry;
+
+typedef struct bb_plan_stats {
+    int NN, Nmax, Nr, n_mtbin, n_k1bin;
+    int rcount;          /* ring triples (calCR, funcs.c:389-408) */
+    long long ncontr;    /* contractions visited by barbie.c:247-292 */
+    long long npairs;    /* distinct (p1,r1,p2,r2) pair fields */
+    int ngroups;         /* distinct (p1,p2) plane groups */
+    int nfields_used;    /* distinct (plane, ring) fields read by some contraction */
+    int nbins;           /* n_k1bin * n_mtbin * n_mtbin */
+} bb_plan_stats;
+
+typedef struct bb_plan bb_plan;
+typedef struct bb_estimator bb_estimator;
+
+const char *bb_last_error(void);
+const char *bb_version(void);
+
+/* Fills *g with the reference's geometry for the given CLI arguments:
+ * Nmax=100, Nr=40, piecewise rings, n_mtbin=5, LL=1, ff=0.1, beta=1
+ * (barbie.c:61-84). */
+int bb_geometry_reference(int NN, int UL_k1, int LL_k1, int BW_k1, bb_geometry *g);
+
+/* Host-side planning; no GPU is touched.  Builds the ring edges
+ * (barbie.c:110-130), ring masks as integer radius bounds (barbie.c:135-148),
+ * the ring triples (calCR's predicate, funcs.c:389-395) and the full list of
+ * contractions in the reference's enumeration order (barbie.c:247-292) with
+ * their bins (barbie.c:343-345).  Validates what the reference leaves
+ * unchecked (docs/REVIEW.md:10): even NN, BW>0, divisible size range,
+ * Nmax <= NN/2, rings inside the grid, every bin index in range. */
+int bb_plan_create(const bb_geometry *g, bb_plan **out);
+void bb_plan_destroy(bb_plan *p);
+int bb_plan_stats_get(const bb_plan *p, bb_plan_stats *s);
+/* k_perp_Low/High/avg of barbie.c:89-130 (Nr doubles each). */
+int bb_plan_ring_edges(const bb_plan *p, double *low, double *high, double *avg);
+/* Ring triples in calCR order (funcs.c:389-404): rcount * 3 ints. */
+int bb_plan_triples(const bb_plan *p, int *r123);
+/* Contractions in barbie.c:247-292 order: ncontr * 8 ints
+ * {p1, r1, p2, r2, p3, r3, rp, flat_bin}. */
+int bb_plan_contractions(const bb_plan *p, int *rec);
+
+/* Device side.  Creates an estimator for a plan on the current HIP device:
+ * allocates the plane-ring cache and work buffers in HBM and computes the
+ * triangle counts Num (calCR, funcs.c:328-424) on the GPU in FP64. */
+int bb_estimator_create(const bb_plan *p, bb_precision prec, bb_estimator **out);
+void bb_estimator_destroy(bb_estimator *e);
+/* Device bytes held by an estimator of this plan and precision. */
+int bb_estimator_footprint(const bb_plan *p, bb_precision prec, size_t *bytes);
+
+/* One estimate (barbie.c:176-373): field_r2c is the FFTW in-place r2c buffer
+ * [NN][NN][NN+2] doubles in host memory (the contents of
+ * field_N<NN>_R<NRi>.txt).  Outputs are the un-normalised FP64 bin sums
+ * bin_BS_real, bin_normalization and bin_Biana of barbie.c:87, each
+ * [n_k1bin][n_mtbin][n_mtbin][16] (any may be NULL). */
+int bb_estimate(bb_estimator *e, const double *field_r2c,
+                double *bin_bs, double *bin_norm, double *bin_biana);
+/* Same with the field already in device memory (pointer from hipMalloc) and
+ * device output pointers; enqueued on the estimator's stream, which is
+ * synchronised before return. */
+int bb_estimate_device(bb_estimator *e, const double *d_field_r2c,
+                       double *d_bin_bs, double *d_bin_norm, double *d_bin_biana);
+
+/* Diagnostics of the last estimate: per-contraction triple-product sums in
+ * the reference's enumeration order (ncontr doubles), the GPU triangle counts
+ * Num per ring triple (rcount doubles), and one plane-ring field
+ * D_{plane,ring} (NN*NN complex, interleaved) converted to FP64. */
+int bb_estimator_contraction_sums(bb_estimator *e, double *S);
+int bb_estimator_triangle_counts(bb_estimator *e, double *num);
+int bb_estimator_field(bb_estimator *e, int plane, int ring, double *out);
+/* Per-stage device times of the last estimate in milliseconds:
+ * [0] upload/gather, [1] plane-ring FFTs, [2] contraction, [3] binning. */
+int bb_estimator_timings(bb_estimator *e, double *ms4);
+
+/* Normalisation and the 16 output files of barbie.c:383-397:
+ * <dir>/BS_test_ii<ii>_N<NN>_R<NRi>.txt with the reference's row filter and
+ * "%d\t%d\t%d\t%e\t%e\t%f\n" format. */
+int bb_write_outputs(const bb_plan *p, const double *bin_bs, const double *bin_norm,
+                     const double *bin_biana, const char *dir, int NRi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BARBIE_H */

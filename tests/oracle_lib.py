@@ -1,0 +1,135 @@
+"""ctypes bindings to the CPU oracle (oracle/_build/liborc.so).
+
+Test infrastructure: only the tests, the smoke check and bench.py's
+CPU-baseline leg load this, and only as the checker.
+"""
+import ctypes
+import os
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "_build", "liborc.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [("NN", ctypes.c_int), ("Nmax", ctypes.c_int), ("Nr", ctypes.c_int),
+                ("n_mtbin", ctypes.c_int), ("UL_k1", ctypes.c_int), ("LL_k1", ctypes.c_int),
+                ("BW_k1", ctypes.c_int), ("LL", ctypes.c_double), ("ff", ctypes.c_double),
+                ("beta", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle` (or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_fft1d.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_ifft2.argtypes = [ctypes.c_int, _dp]
+        L.orc_edges_reference.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
+        L.orc_edges_uniform.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                        _dp, _dp, _dp]
+        L.orc_calcr.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int]
+        L.orc_df.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp]
+        L.orc_sort.argtypes = [_dp]
+        L.orc_cal_coeff.argtypes = [ctypes.c_double, ctypes.c_double, _dp, _dp]
+        L.orc_estimate.argtypes = [ctypes.POINTER(OrcParams), _dp, _dp, _dp, ctypes.c_int, _dp, _dp,
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp,
+                                   ctypes.c_long]
+        L.orc_estimate.restype = ctypes.c_long
+        L.orc_write_outputs.argtypes = [ctypes.POINTER(OrcParams), _dp, _dp, _dp, ctypes.c_char_p,
+                                        ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def fft1d(x, sign=+1):
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    assert lib().orc_fft1d(a.size, sign, _p(a.view(np.float64))) == 0
+    return a
+
+
+def ifft2(x):
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    assert lib().orc_ifft2(a.shape[0], _p(a.view(np.float64))) == 0
+    return a
+
+
+def edges_reference(Nr, Nmax):
+    lo, hi, av = (np.zeros(Nr) for _ in range(3))
+    assert lib().orc_edges_reference(Nr, Nmax, _p(lo), _p(hi), _p(av)) == 0
+    return lo, hi, av
+
+
+def edges_uniform(Nr, width, Nmax, last_to_nmax):
+    lo, hi, av = (np.zeros(Nr) for _ in range(3))
+    assert lib().orc_edges_uniform(Nr, width, Nmax, int(last_to_nmax), _p(lo), _p(hi), _p(av)) == 0
+    return lo, hi, av
+
+
+def calcr(NN, lo, hi, av, cap=200000):
+    Nr = lo.size
+    rdata = np.zeros(cap * 4)
+    n = lib().orc_calcr(NN, Nr, _p(lo), _p(hi), _p(av), _p(rdata), cap)
+    assert n >= 0, n
+    return rdata[: 4 * n].reshape(n, 4).copy()
+
+
+def df(field, NN, plane, lo, hi):
+    out = np.zeros((NN, NN), dtype=np.complex128)
+    f = np.ascontiguousarray(field, dtype=np.float64)
+    lib().orc_df(NN, _p(f), plane, float(lo), float(hi), _p(out.view(np.float64)))
+    return out
+
+
+def cal_coeff(px, pz):
+    bs, nm = np.zeros(16), np.zeros(16)
+    lib().orc_cal_coeff(float(px), float(pz), _p(bs), _p(nm))
+    return bs, nm
+
+
+def sort3(K):
+    a = np.ascontiguousarray(K, dtype=np.float64).reshape(3, 2).copy()
+    lib().orc_sort(_p(a))
+    return a
+
+
+def estimate(params, lo, hi, av, rdata, field, p1_begin=0, p1_end=None, nthreads=1,
+             record=False, cap=None):
+    """Runs the oracle estimator loop.  Returns (bs, norm, biana, count, contr)."""
+    P = OrcParams(**params)
+    nb = int((P.UL_k1 - P.LL_k1) / (1.0 * P.BW_k1)) * P.n_mtbin * P.n_mtbin
+    bs, nm, bi = np.zeros(nb * 16), np.zeros(nb * 16), np.zeros(nb * 16)
+    if p1_end is None:
+        p1_end = P.Nmax
+    rd = np.ascontiguousarray(rdata, dtype=np.float64).reshape(-1)
+    f = np.ascontiguousarray(field, dtype=np.float64).reshape(-1)
+    contr = None
+    cptr = None
+    if record:
+        cap = cap or 2_000_000
+        contr = np.zeros((cap, 8))
+        cptr = _p(contr)
+    n = lib().orc_estimate(ctypes.byref(P), _p(lo), _p(hi), _p(av), rd.size // 4, _p(rd), _p(f),
+                           p1_begin, p1_end, nthreads, _p(bs), _p(nm), _p(bi), cptr,
+                           cap if record else 0)
+    if n < 0:
+        raise RuntimeError(f"orc_estimate failed: {n}")
+    if record:
+        contr = contr[:n].copy()
+    return bs, nm, bi, n, contr
+
+
+def write_outputs(params, bs, nm, bi, outdir, NRi):
+    P = OrcParams(**params)
+    assert lib().orc_write_outputs(ctypes.byref(P), _p(bs), _p(nm), _p(bi),
+                                   outdir.encode(), NRi) == 0

@@ -1,0 +1,152 @@
+"""GPU parity: the HIP estimator against the CPU oracle on the same seeded
+inputs, through the C ABI.
+
+Tolerances (stated per check): FP64 path within 1e-10 of the oracle
+(relative to the largest magnitude of the compared quantity), FP32 path
+within 2e-5 norm-wise per multipole.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from geoms import tiny, small, medium, radix5, config1, oracle_params
+from test_oracle import oracle_edges
+import barbie_amd as B
+
+pytestmark = pytest.mark.gpu
+
+_ORC = {}
+
+
+def oracle_run(gf, NRi, zero=False):
+    key = (gf.__name__, NRi, zero)
+    if key not in _ORC:
+        g = gf()
+        lo, hi, av = oracle_edges(g)
+        rd = O.calcr(g.NN, lo, hi, av)
+        field = np.zeros((g.NN, g.NN, g.NN + 2)) if zero else B.make_field(g.NN, NRi=NRi)
+        bs, nm, bi, n, contr = O.estimate(oracle_params(g), lo, hi, av, rd, field, record=True)
+        _ORC[key] = dict(g=g, field=field, rd=rd, bs=bs, nm=nm, bi=bi, contr=contr,
+                         edges=(lo, hi, av))
+    return _ORC[key]
+
+
+def rel_err(a, b):
+    scale = np.max(np.abs(b))
+    return np.max(np.abs(a - b)) / scale if scale > 0 else np.max(np.abs(a))
+
+
+@pytest.mark.parametrize("gf", [tiny, small, radix5, medium, config1])
+def test_fp64_matches_oracle(gf):
+    o = oracle_run(gf, NRi=1)
+    g = o["g"]
+    plan = B.Plan(g)
+    est = B.Estimator(plan, "fp64")
+    bs, nm, bi = est.estimate(o["field"])
+    # triangle counts Num (calCR) within 1e-12
+    assert rel_err(est.triangle_counts(), o["rd"][:, 3]) <= 1e-12
+    # per-contraction sums within 1e-10 of the largest |S|
+    S = est.contraction_sums()
+    So = o["contr"][:, 6]
+    assert S.shape == So.shape
+    assert rel_err(S, So) <= 1e-10
+    big = np.abs(So) > 1e-3 * np.max(np.abs(So))
+    assert np.max(np.abs(S[big] - So[big]) / np.abs(So[big])) <= 1e-9
+    # exact-zero rule decided identically
+    assert np.array_equal(S == 0.0, So == 0.0)
+    # bins, per multipole
+    obs, onm, obi = (o[k].reshape(bs.shape) for k in ("bs", "nm", "bi"))
+    for ii in range(16):
+        assert rel_err(bs[..., ii], obs[..., ii]) <= 1e-10, ii
+        assert rel_err(nm[..., ii], onm[..., ii]) <= 1e-12, ii
+        assert rel_err(bi[..., ii], obi[..., ii]) <= 1e-10, ii
+
+
+@pytest.mark.parametrize("gf", [tiny, radix5, medium])
+def test_fp64_plane_ring_fields_match_oracle(gf):
+    o = oracle_run(gf, NRi=1)
+    g = o["g"]
+    lo, hi, _ = o["edges"]
+    plan = B.Plan(g)
+    est = B.Estimator(plan, "fp64")
+    est.estimate(o["field"])
+    pc = plan.contractions()
+    used = {(int(p), int(r)) for p, r in zip(pc[:, 0], pc[:, 1])}
+    used |= {(abs(int(p)), int(r)) for p, r in zip(pc[:, 2], pc[:, 3])}
+    for p, r in sorted(used)[:: max(1, len(used) // 12)]:
+        ref = O.df(o["field"], g.NN, p, lo[r], hi[r])
+        got = est.field(p, r)
+        assert np.max(np.abs(got - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref))), (p, r)
+
+
+def test_zero_signal_field():
+    o = oracle_run(small, NRi=1, zero=True)
+    g = o["g"]
+    est = B.Estimator(B.Plan(g), "fp64")
+    bs, nm, bi = est.estimate(o["field"])
+    assert np.all(bs == 0) and np.all(nm == 0) and np.all(bi == 0)
+    assert np.all(o["nm"] == 0)
+
+
+def test_fp64_deterministic_and_estimator_reuse():
+    g = medium()
+    est = B.Estimator(B.Plan(g), "fp64")
+    f1 = B.make_field(g.NN, NRi=1)
+    f2 = B.make_field(g.NN, NRi=2)
+    a = est.estimate(f1)
+    b = est.estimate(f2)
+    c = est.estimate(f1)
+    for x, y in zip(a, c):
+        assert np.array_equal(x, y)
+    assert not np.array_equal(a[0], b[0])
+
+
+@pytest.mark.parametrize("gf", [medium, config1])
+def test_fp32_within_tolerance(gf):
+    o = oracle_run(gf, NRi=1)
+    g = o["g"]
+    est = B.Estimator(B.Plan(g), "fp32")
+    bs, nm, bi = est.estimate(o["field"])
+    obs, onm = (o[k].reshape(bs.shape) for k in ("bs", "nm"))
+    for ii in range(16):
+        d = np.linalg.norm(bs[..., ii] - obs[..., ii]) / np.linalg.norm(obs[..., ii])
+        assert d <= 2e-5, (ii, d)
+        assert rel_err(nm[..., ii], onm[..., ii]) <= 1e-12
+
+
+def test_cli_end_to_end(tmp_path):
+    """`barbie NN NRi UL LL BW` in a directory holding field_N<NN>_R<NRi>.txt
+    writes the 16 files the reference writes; values match the oracle's own
+    writer to the printed precision and the FP64 dump to 1e-10."""
+    o = oracle_run(medium, NRi=1)
+    g = o["g"]
+    B.write_field(str(tmp_path / f"field_N{g.NN}_R7.txt"), o["field"])
+    env = dict(os.environ, BARBIE_NMAX=str(g.Nmax), BARBIE_NR=str(g.Nr))
+    r = subprocess.run([B.BIN_PATH, str(g.NN), "7", str(g.UL_k1), str(g.LL_k1), str(g.BW_k1)],
+                       cwd=tmp_path, env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    outdir = tmp_path / f"ResFiles_N{g.NN}"
+    refdir = tmp_path / "oracle"
+    refdir.mkdir()
+    O.write_outputs(oracle_params(g), o["bs"], o["nm"], o["bi"], str(refdir), 7)
+    for ii in range(16):
+        name = f"BS_test_ii{ii}_N{g.NN}_R7.txt"
+        a = np.loadtxt(outdir / name, ndmin=2)
+        b = np.loadtxt(refdir / name, ndmin=2)
+        assert a.shape == b.shape and a.shape[0] > 0
+        assert np.array_equal(a[:, :3], b[:, :3])
+        fin = np.isfinite(b[:, 3])
+        assert np.array_equal(fin, np.isfinite(a[:, 3]))
+        # 7 printed digits; m != 0 bins that cancel to ~0 compare against
+        # the column's scale
+        for col in (3, 4, 5):
+            scale = np.max(np.abs(b[fin, col])) if fin.any() else 0.0
+            assert np.allclose(a[fin, col], b[fin, col], rtol=2e-6, atol=1e-9 * scale), col
+    dump = np.fromfile(outdir / f"BS_bins_N{g.NN}_R7.f64").reshape(3, -1)
+    assert rel_err(dump[0], o["bs"]) <= 1e-10
+    assert rel_err(dump[1], o["nm"]) <= 1e-12
+    assert (outdir / f"time_{g.NN}_R7.txt").read_text().strip().splitlines()[-1].startswith(
+        "Total time")

@@ -1,0 +1,290 @@
+"""Pins of the CPU oracle (test infrastructure) against what the mathematics
+fixes: the DFT definition, closed-form ring edges, brute-force lattice
+triangle counts, brute-force direct triangle sums, Legendre functions.
+
+The reference ships no golden vectors, fixtures or tests (no tests/ in the
+reference; examples/README.md:3-5), so these are the whole basis.
+"""
+import numpy as np
+import pytest
+from scipy.special import eval_legendre, lpmv
+
+import oracle_lib as O
+from geoms import tiny, small, medium, radix5, config1, oracle_params
+from barbie_amd import make_field, RINGS_REFERENCE
+
+
+def oracle_edges(g):
+    if g.ring_rule == RINGS_REFERENCE:
+        return O.edges_reference(g.Nr, g.Nmax)
+    return O.edges_uniform(g.Nr, g.ring_width, g.Nmax, g.last_ring_to_nmax)
+
+
+def brute_dft2(x, sign=+1):
+    n = x.shape[0]
+    k = np.arange(n)
+    W = np.exp(sign * 2j * np.pi * np.outer(k, k) / n)
+    return W @ x @ W.T
+
+
+# ---------------------------------------------------------------- FFT ----
+@pytest.mark.parametrize("n", [2, 3, 4, 5, 6, 8, 12, 15, 16, 24, 40, 48, 96, 120, 128, 384, 512])
+def test_fft1d_matches_numpy(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ref = np.fft.ifft(x) * n       # exp(+2 pi i nk/N), unnormalised
+    got = O.fft1d(x, +1)
+    assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref)) * np.log2(n + 1)
+    got_f = O.fft1d(x, -1)
+    assert np.allclose(got_f, np.fft.fft(x), rtol=0, atol=1e-12 * np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("n", [6, 8, 10, 12])
+def test_ifft2_matches_brute_force_dft(n):
+    rng = np.random.default_rng(7 + n)
+    x = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    assert np.allclose(O.ifft2(x), brute_dft2(x, +1), rtol=0, atol=1e-12)
+
+
+def test_ifft2_matches_numpy_384():
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((384, 384)) + 1j * rng.standard_normal((384, 384))
+    ref = np.fft.ifft2(x) * 384 * 384
+    assert np.max(np.abs(O.ifft2(x) - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+# ----------------------------------------------------------- ring edges ----
+def test_reference_ring_edges_closed_form():
+    # barbie.c:110-130 at Nr=40, Nmax=100: blocks of ten rings with widths
+    # 1, 2, 3, 4; every edge but the first shifted by -0.1; last ring to 100.
+    lo, hi, av = O.edges_reference(40, 100)
+    starts = np.concatenate([np.arange(0, 10), np.arange(10, 30, 2), np.arange(30, 60, 3),
+                             np.arange(60, 100, 4)]).astype(float)
+    exp_lo = starts - 0.1
+    exp_lo[0] = 0.0
+    assert np.allclose(lo, exp_lo, rtol=0, atol=1e-12)
+    assert np.allclose(hi[:-1], exp_lo[1:], rtol=0, atol=1e-12) and hi[-1] == 100.0
+    assert av[0] == pytest.approx(0.45) and av[-1] == pytest.approx(97.95)
+    assert np.allclose(av, (lo + hi) / 2)
+
+
+def test_uniform_ring_edges():
+    lo, hi, av = O.edges_uniform(12, 1.0, 32, 0)
+    assert lo[0] == 0.0 and np.allclose(lo[1:], np.arange(1, 12) - 0.1)
+    assert hi[-1] == pytest.approx(11.9)
+    lo, hi, av = O.edges_uniform(12, 1.0, 32, 1)
+    assert hi[-1] == 32.0
+
+
+# ------------------------------------------------- ring triples and Num ----
+def ring_mask(NN, lo, hi):
+    k = np.fft.fftfreq(NN, d=1.0 / NN)
+    # reference index convention: i > NN/2 -> i - NN (so NN/2 stays positive)
+    ia = np.where(np.arange(NN) > NN // 2, np.arange(NN) - NN, np.arange(NN))
+    kk = np.sqrt((ia[:, None] ** 2 + ia[None, :] ** 2).astype(float))
+    del k
+    return (kk >= lo) & (kk < hi)
+
+
+def brute_triangle_count(NN, m1, m2, m3):
+    """#{k1 in r1, k2 in r2: -(k1+k2) mod NN in r3} (lattice triangles)."""
+    i1, j1 = np.nonzero(m1)
+    i2, j2 = np.nonzero(m2)
+    i3 = (-(i1[:, None] + i2[None, :])) % NN
+    j3 = (-(j1[:, None] + j2[None, :])) % NN
+    return int(m3[i3, j3].sum())
+
+
+@pytest.mark.parametrize("gf", [tiny, small, radix5, medium])
+def test_calcr_num_is_nn2_times_lattice_triangle_count(gf):
+    g = gf()
+    lo, hi, av = oracle_edges(g)
+    rd = O.calcr(g.NN, lo, hi, av)
+    masks = [ring_mask(g.NN, lo[r], hi[r]) for r in range(g.Nr)]
+    for r1, r2, r3, num in rd:
+        cnt = brute_triangle_count(g.NN, masks[int(r1)], masks[int(r2)], masks[int(r3)])
+        assert num == pytest.approx(g.NN ** 2 * cnt, rel=1e-12, abs=1e-6)
+
+
+def test_calcr_counts_reference_geometry():
+    # ring-triple counts of calCR's predicate (funcs.c:389-395)
+    lo, hi, av = O.edges_reference(40, 100)
+    assert len(O.calcr(384, lo, hi, av)) == 3620
+    assert len(O.calcr(128, *O.edges_uniform(12, 1.0, 32, 0))) == 197
+    assert len(O.calcr(128, *O.edges_uniform(12, 1.0, 32, 1))) == 168
+
+
+def test_calcr_triples_satisfy_closure():
+    lo, hi, av = O.edges_reference(40, 100)
+    rd = O.calcr(96, lo, hi, av)   # predicate is independent of NN
+    r1, r2, r3 = rd[:, 0].astype(int), rd[:, 1].astype(int), rd[:, 2].astype(int)
+    assert np.all(r1 >= r2) and np.all(r2 >= r3)
+    a1, a2, a3 = av[r1], av[r2], av[r3]
+    # t in [1/2, 1] and mu in [1/(2t), 1]  <=>  a1 - a2 <= a3 <= a2 (up to rounding)
+    assert np.all(a2 / a1 >= 0.5)
+    assert np.all(a3 <= a2 + 1e-12) and np.all(a3 >= a1 - a2 - 1e-12)
+
+
+# ------------------------------------------------------------------ Df ----
+@pytest.mark.parametrize("gf", [tiny, radix5])
+def test_df_matches_numpy_and_conventions(gf):
+    g = gf()
+    lo, hi, av = oracle_edges(g)
+    field = make_field(g.NN, NRi=3)
+    cplx = field.view(np.complex128)            # (NN, NN, NN/2+1)
+    for p in range(g.Nmax):
+        for r in range(g.Nr):
+            m = ring_mask(g.NN, lo[r], hi[r])
+            plane = cplx[:, :, p] * m
+            if p == 0:
+                plane = plane.copy()
+                plane[0, 0] = 0
+            ref = np.fft.ifft2(plane) * g.NN ** 2
+            got = O.df(field, g.NN, p, lo[r], hi[r])
+            assert np.max(np.abs(got - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
+            if p == 0:   # Hermitian plane -> real field
+                assert np.max(np.abs(got.imag)) <= 1e-10 * max(1.0, np.max(np.abs(got)))
+
+
+# ------------------------------------------------------------ cal_coeff ----
+# (l, m) of the 16 components (README.md:104)
+LM = [(0, 0), (2, 0), (2, 1), (2, 2), (4, 0), (4, 1), (4, 2), (4, 3), (4, 4),
+      (6, 0), (6, 1), (6, 2), (6, 3), (6, 4), (6, 5), (6, 6)]
+
+
+def test_cal_coeff_m0_are_legendre():
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        pz = rng.uniform(-1, 1)
+        px = rng.uniform(-1, 1) * np.sqrt(1 - pz * pz)
+        bs, nm = O.cal_coeff(px, pz)
+        assert bs[0] == 1.0 and nm[0] == 1.0
+        assert bs[1] == pytest.approx(4 * eval_legendre(2, pz), abs=1e-12)
+        assert bs[4] == pytest.approx(64 * eval_legendre(4, pz), abs=1e-10)
+        assert bs[9] == pytest.approx(256 * eval_legendre(6, pz), abs=1e-9)
+
+
+def test_cal_coeff_are_associated_legendre_times_cos_m_phi():
+    """bs[ii] = c_ii * P_l^m(pz) * cos(m phi) with cos(phi) = px / sqrt(1-pz^2);
+    norm[ii] = P_l^m(pz)^2 / d_ii.  Constant ratios over random directions pin
+    every polynomial up to its constant; the constants are pinned at the
+    special points below."""
+    rng = np.random.default_rng(11)
+    ratios_b, ratios_n = [], []
+    for _ in range(200):
+        pz = rng.uniform(-0.95, 0.95)
+        phi = rng.uniform(0, 2 * np.pi)
+        px = np.sqrt(1 - pz * pz) * np.cos(phi)
+        bs, nm = O.cal_coeff(px, pz)
+        rb, rn = [], []
+        for ii, (l, m) in enumerate(LM):
+            plm = lpmv(m, l, pz)
+            ang = np.cos(m * phi)
+            rb.append(bs[ii] / (plm * ang) if abs(plm * ang) > 1e-3 else np.nan)
+            rn.append(nm[ii] / plm ** 2 if abs(plm) > 1e-3 else np.nan)
+        ratios_b.append(rb)
+        ratios_n.append(rn)
+    rb = np.array(ratios_b)
+    rn = np.array(ratios_n)
+    for ii in range(16):
+        col = rb[:, ii][~np.isnan(rb[:, ii])]
+        assert len(col) > 50
+        assert np.allclose(col, col[0], rtol=1e-9), ii
+        coln = rn[:, ii][~np.isnan(rn[:, ii])]
+        assert np.allclose(coln, coln[0], rtol=1e-9), ii
+
+
+def test_cal_coeff_special_points():
+    # pz = 0, px = 1 (phi = 0): P_l^m(0) known; m odd and l-m odd vanish.
+    bs, nm = O.cal_coeff(1.0, 0.0)
+    assert bs[3] == pytest.approx(2 * np.sqrt(2 / 3))          # 2 sqrt(2/3) (2 - 1)
+    assert bs[15] == pytest.approx(32 / np.sqrt(231))          # 32 (32 - 48 + 18 - 1)/sqrt(231)
+    assert bs[8] == pytest.approx(8 * np.sqrt(0.05714285714285714) * 1.0)
+    for ii in (2, 5, 7, 10, 12, 14):                          # odd in pz
+        assert bs[ii] == 0.0
+    # pz = 1: only m = 0 survives, P_l(1) = 1
+    bs, nm = O.cal_coeff(0.0, 1.0)
+    assert bs[1] == 4 and bs[4] == 64 and bs[9] == 256
+    for ii in (2, 3, 5, 6, 7, 8, 10, 11, 12, 13, 14, 15):
+        assert bs[ii] == 0.0 and nm[ii] == 0.0
+    assert nm[1] == 4 and nm[4] == 64 and nm[9] == 256 ** 2 / 256
+
+
+def test_sort_descending_stable_on_ties():
+    K = np.array([[2.0, 1], [3.0, 2], [2.0, 3]])
+    out = O.sort3(K)
+    assert out[:, 0].tolist() == [3.0, 2.0, 2.0] and out[:, 1].tolist() == [2, 1, 3]
+    K = np.array([[1.0, -1], [1.0, 0], [1.0, 1]])
+    assert O.sort3(K)[:, 1].tolist() == [-1, 0, 1]
+
+
+# --------------------------------------------- contraction (FFT trick) ----
+def direct_triangle_sum(NN, cplx, lo, hi, p1, r1, p2, r2, p3, r3):
+    """NN^2 * Re sum over closed lattice triangles k1+k2+k3 = 0 (mod NN) of
+    Delta1(k1) Delta2~(k2) Delta3~(k3), with Delta~_p(k) = conj(Delta_|p|(-k))
+    for a negative plane and the DC mode of plane 0 removed."""
+    def coeffs(p, r):
+        m = ring_mask(NN, lo[r], hi[r])
+        a = cplx[:, :, abs(p)] * m
+        if p == 0:
+            a = a.copy()
+            a[0, 0] = 0
+        if p < 0:
+            idx = (-np.arange(NN)) % NN
+            a = np.conj(a[idx][:, idx])
+        return a
+    A, B, C = coeffs(p1, r1), coeffs(p2, r2), coeffs(p3, r3)
+    i1, j1 = np.nonzero(A)
+    i2, j2 = np.nonzero(B)
+    if len(i1) == 0 or len(i2) == 0:
+        return 0.0
+    i3 = (-(i1[:, None] + i2[None, :])) % NN
+    j3 = (-(j1[:, None] + j2[None, :])) % NN
+    tot = (A[i1, j1][:, None] * B[i2, j2][None, :] * C[i3, j3]).sum()
+    return NN * NN * tot.real
+
+
+def test_oracle_contractions_equal_direct_triangle_sums():
+    g = tiny()
+    lo, hi, av = oracle_edges(g)
+    rd = O.calcr(g.NN, lo, hi, av)
+    field = make_field(g.NN, NRi=2)
+    cplx = field.view(np.complex128)
+    _, _, _, n, contr = O.estimate(oracle_params(g), lo, hi, av, rd, field, record=True)
+    assert n == len(contr) and n > 50
+    scale = np.max(np.abs(contr[:, 6]))
+    for row in contr:
+        p1, r1, p2, r2, p3, r3 = (int(v) for v in row[:6])
+        ref = direct_triangle_sum(g.NN, cplx, lo, hi, p1, r1, p2, r2, p3, r3)
+        assert row[6] == pytest.approx(ref, rel=1e-9, abs=1e-11 * scale)
+    # the zero-signal rule fires exactly where a field is identically zero
+    zero = contr[:, 6] == 0.0
+    assert np.any(zero)
+    for row in contr[zero]:
+        assert (int(row[0]) == 0 and int(row[1]) == 0) or (int(row[2]) == 0 and int(row[3]) == 0) \
+            or (int(row[4]) == 0 and int(row[5]) == 0)
+
+
+def test_oracle_zero_field_gives_zero_bins():
+    g = small()
+    lo, hi, av = oracle_edges(g)
+    rd = O.calcr(g.NN, lo, hi, av)
+    field = np.zeros((g.NN, g.NN, g.NN + 2))
+    bs, nm, bi, n, _ = O.estimate(oracle_params(g), lo, hi, av, rd, field)
+    assert n > 0 and np.all(bs == 0) and np.all(nm == 0) and np.all(bi == 0)
+
+
+@pytest.mark.slow
+def test_oracle_config1_contraction_count_and_threads():
+    g = config1()
+    lo, hi, av = oracle_edges(g)
+    rd = O.calcr(g.NN, lo, hi, av)
+    field = make_field(g.NN, NRi=1)
+    p = oracle_params(g)
+    # p1 subset keeps this CPU test short; the count over all p1 is pinned
+    # by the planner tests (83 221).
+    bs1, nm1, bi1, n1, _ = O.estimate(p, lo, hi, av, rd, field, p1_begin=0, p1_end=4)
+    bs4, nm4, bi4, n4, _ = O.estimate(p, lo, hi, av, rd, field, p1_begin=0, p1_end=4, nthreads=4)
+    assert n1 == n4 > 0
+    assert np.allclose(bs1, bs4, rtol=1e-11, atol=1e-11 * np.max(np.abs(bs1)))
+    assert np.allclose(nm1, nm4, rtol=1e-12, atol=0)

@@ -1,0 +1,127 @@
+"""CPU tests of the C ABI: the library loads and exports every declared
+symbol, the host planner reproduces the oracle's enumeration and the
+reference geometry's counts, and invalid input is refused with an error."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from geoms import tiny, small, medium, radix5, config1, oracle_params
+from test_oracle import oracle_edges
+import barbie_amd as B
+from barbie_amd import native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "barbie.h")).read()
+    declared = set(re.findall(r"\b(bb_[a-z0-9_]+)\s*\(", hdr))
+    assert declared == set(native.EXPORTED)
+    L = ctypes.CDLL(B.LIB_PATH)
+    for name in declared:
+        assert hasattr(L, name), name
+    assert b"gfx950" in B.lib().bb_version()
+
+
+def test_reference_geometry_counts():
+    # barbie.c constants: Nmax=100, Nr=40, piecewise rings; UL=51 (config 2)
+    p = B.Plan(B.Geometry(NN=384, UL_k1=51))
+    s = p.stats
+    assert s["rcount"] == 3620
+    assert s["ncontr"] == 2756382
+    assert s["npairs"] == 387848
+    assert s["ngroups"] == 3826
+    assert s["nfields_used"] == 1189
+    assert s["n_k1bin"] == 50 and s["nbins"] == 50 * 25
+
+
+def test_other_workload_counts():
+    assert B.Plan(config1()).stats["ncontr"] == 83221
+    g = config1()
+    g.last_ring_to_nmax = 1
+    assert B.Plan(g).stats["ncontr"] == 65840
+    for NN, Nr, UL, n in [(128, 13, 18, 51371), (192, 19, 27, 269376), (256, 26, 36, 794278)]:
+        st = B.Plan(B.Geometry(NN=NN, UL_k1=UL, Nmax=NN // 4, Nr=Nr)).stats
+        assert st["ncontr"] == n
+
+
+@pytest.mark.parametrize("gf", [tiny, small, medium, radix5])
+def test_plan_matches_oracle_enumeration(gf):
+    g = gf()
+    plan = B.Plan(g)
+    lo, hi, av = oracle_edges(g)
+    plo, phi, pav = plan.ring_edges()
+    assert np.array_equal(lo, plo) and np.array_equal(hi, phi) and np.array_equal(av, pav)
+    rd = O.calcr(g.NN, lo, hi, av)
+    assert np.array_equal(plan.triples(), rd[:, :3].astype(np.int32))
+    field = make_small_field(g.NN)
+    _, _, _, n, contr = O.estimate(oracle_params(g), lo, hi, av, rd, field, record=True)
+    pc = plan.contractions()
+    assert n == len(pc)
+    assert np.array_equal(pc[:, :6], contr[:, :6].astype(np.int32))
+    assert np.array_equal(pc[:, 7], contr[:, 7].astype(np.int32))
+
+
+def make_small_field(NN):
+    return B.make_field(NN, NRi=1)
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(NN=15), "NN"), (dict(NN=14), "NN"), (dict(BW_k1=0), "BW_k1"),
+    (dict(UL_k1=1), "UL_k1"), (dict(UL_k1=8, BW_k1=3), "divisible"),
+    (dict(Nmax=9), "Nmax"), (dict(Nr=0), "Nr"),
+])
+def test_invalid_geometry_is_refused(kw, msg):
+    args = dict(NN=16, UL_k1=6, LL_k1=1, BW_k1=1, Nmax=4, Nr=4, ring_rule=B.RINGS_UNIFORM)
+    args.update(kw)
+    with pytest.raises(B.BarbieError) as ei:
+        B.Plan(B.Geometry(**args))
+    assert msg in str(ei.value)
+    assert ei.value.code == -1
+
+
+def test_ring_outside_grid_is_refused():
+    with pytest.raises(B.BarbieError):
+        B.Plan(B.Geometry(NN=16, UL_k1=6, Nmax=8, Nr=12, ring_rule=B.RINGS_UNIFORM,
+                          last_ring_to_nmax=0))
+
+
+def test_explicit_rings():
+    g = tiny()
+    lo, hi, av = oracle_edges(g)
+    ge = B.Geometry(NN=16, UL_k1=6, Nmax=4, Nr=4, ring_rule=B.RINGS_EXPLICIT, ring_low=lo,
+                    ring_high=hi)
+    assert np.array_equal(B.Plan(ge).contractions(), B.Plan(g).contractions())
+
+
+def test_cli_usage_and_validation(tmp_path):
+    import subprocess
+    r = subprocess.run([B.BIN_PATH], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == 1 and "Usage:" in r.stdout
+    r = subprocess.run([B.BIN_PATH, "384", "1", "51", "1", "0"], capture_output=True, text=True,
+                       cwd=tmp_path)
+    assert r.returncode == 2 and "BW_k1" in r.stderr
+    r = subprocess.run([B.BIN_PATH, "16", "1", "6", "1", "1"], capture_output=True, text=True,
+                       cwd=tmp_path, env=dict(os.environ, BARBIE_NMAX="4", BARBIE_NR="4",
+                                              BARBIE_RING_RULE="uniform"))
+    assert r.returncode == 3 and "field_N16_R1.txt" in r.stderr
+
+
+def test_write_outputs_format_matches_oracle(tmp_path):
+    g = tiny()
+    plan = B.Plan(g)
+    rng = np.random.default_rng(0)
+    nb = g.nbins * 16
+    bs, nm, bi = rng.standard_normal(nb), np.abs(rng.standard_normal(nb)), rng.standard_normal(nb)
+    nm[::7] = 0.0
+    (tmp_path / "a").mkdir()
+    (tmp_path / "b").mkdir()
+    plan.write_outputs(bs, nm, bi, str(tmp_path / "a"), 5)
+    O.write_outputs(oracle_params(g), bs, nm, bi, str(tmp_path / "b"), 5)
+    for ii in range(16):
+        name = f"BS_test_ii{ii}_N16_R5.txt"
+        assert (tmp_path / "a" / name).read_text() == (tmp_path / "b" / name).read_text()
