@@ -107,7 +107,8 @@ __global__ __launch_bounds__(64) void k_binning(const int32_t *__restrict__ seg,
         const double P0 = 1.0 / (k0 * k0), P1 = 1.0 / (k1 * k1), P2 = 1.0 / (k2 * k2);
         double Biana = 2. * bp.ff * (P0 * P1 + P1 * P2 + P2 * P0);
         Biana = Biana * (1 + bp.beta * m1 * m1) * (1 + bp.beta * m2 * m2) * (1 + bp.beta * m3 * m3);
-        const double Sv = S[R.cidx];
+        double Sv = 0.0;  // sum of the x-split partials, fixed order
+        for (int sp = 0; sp < bp.nsplit; ++sp) Sv += S[sp * bp.split_stride + R.pos];
         // barbie.c:355-357.  The sum is exactly zero iff one of its three
         // fields is identically zero; the fields' zero flags decide it.
         if (!nzflag[R.f1] || !nzflag[R.f2] || !nzflag[R.f3]) {

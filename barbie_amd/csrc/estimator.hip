@@ -44,6 +44,14 @@ bool make_fft_desc(int N, FftDesc &fd)
     return n == 1;
 }
 
+bool smooth235(int n)
+{
+    if (n < 1) return false;
+    for (int f : {2, 3, 5})
+        while (n % f == 0) n /= f;
+    return n == 1;
+}
+
 int isqrt_floor(int v)
 {
     if (v < 0) return -1;
@@ -70,27 +78,41 @@ int dev_alloc(T **out, size_t count, std::vector<void *> &owned)
     return BB_OK;
 }
 
-// Builds contraction tiles for the contractions of one plane group.
-// items: (r1, r2, r3, cidx).  fid1(r1), fid2(r2), fid3(r3) map rings to cache
-// slots; consecutive r3 of the group's third plane occupy consecutive slots.
-struct Item { int r1, r2, r3, cidx; };
+// ---------------------------------------------------------------------------
+// Work lists.  One contraction of a plane group: rings and its index.
+struct CItem { int r1, r2, r3, cid; };
 
-template <typename F1, typename F2, typename F3>
-void build_tiles(std::vector<Item> &items, int conj2, int conj3, F1 fid1, F2 fid2, F3 fid3,
-                 std::vector<Tile> &tiles, std::vector<int32_t> &outmap)
+struct WorkLists {
+    std::vector<WorkItem> items;
+    std::vector<MTile> mtiles;
+    int ntw = 0;            // 16x16 output tiles
+    int max_fields = 0;     // largest staged field count of an item
+    long long tile_slots = 0;  // ntw * 256
+};
+
+// slot_b(r): cache slot of ring r in block b (consecutive rings, consecutive
+// slots).  pos[cid] receives the output position of every contraction.
+template <typename S1, typename S2, typename S3>
+void build_group(std::vector<CItem> &cs, int conj2, int conj3, S1 slot1, S2 slot2, S3 slot3,
+                 WorkLists &W, std::vector<int32_t> &pos)
 {
-    // group by pair
-    std::sort(items.begin(), items.end(), [](const Item &a, const Item &b) {
+    constexpr int kWinRings = 16 * kMaxWin;
+    constexpr int kFieldCap = 128;
+    std::sort(cs.begin(), cs.end(), [](const CItem &a, const CItem &b) {
         if (a.r1 != b.r1) return a.r1 < b.r1;
         if (a.r2 != b.r2) return a.r2 < b.r2;
         return a.r3 < b.r3;
     });
+    // Pair entries: (r1, r2) with a contiguous r3 run no wider than the
+    // windows of one M-tile.
     struct Pair { int r1, r2, lo, hi; size_t b, e; };
     std::vector<Pair> pairs;
-    for (size_t k = 0; k < items.size();) {
+    for (size_t k = 0; k < cs.size();) {
         size_t e = k;
-        while (e < items.size() && items[e].r1 == items[k].r1 && items[e].r2 == items[k].r2) ++e;
-        pairs.push_back({items[k].r1, items[k].r2, items[k].r3, items[e - 1].r3, k, e});
+        while (e < cs.size() && cs[e].r1 == cs[k].r1 && cs[e].r2 == cs[k].r2 &&
+               cs[e].r3 - cs[k].r3 < kWinRings)
+            ++e;
+        pairs.push_back({cs[k].r1, cs[k].r2, cs[k].r3, cs[e - 1].r3, k, e});
         k = e;
     }
     std::sort(pairs.begin(), pairs.end(), [](const Pair &a, const Pair &b) {
@@ -99,74 +121,125 @@ void build_tiles(std::vector<Item> &items, int conj2, int conj3, F1 fid1, F2 fid
         if (a.r1 != b.r1) return a.r1 < b.r1;
         return a.r2 < b.r2;
     });
-    for (size_t pb = 0; pb < pairs.size(); pb += kTileMP) {
-        size_t pe = std::min(pairs.size(), pb + kTileMP);
-        int lo = pairs[pb].lo, hi = pairs[pb].hi;
-        for (size_t q = pb; q < pe; ++q) { lo = std::min(lo, pairs[q].lo); hi = std::max(hi, pairs[q].hi); }
-        for (int w = lo; w <= hi; w += kTileNT) {
-            Tile t;
-            std::memset(&t, 0, sizeof t);
-            int nt = std::min(kTileNT, hi - w + 1);
-            bool any = false;
-            t.out = (int32_t)outmap.size();
-            outmap.resize(outmap.size() + kTileMP * kTileNT, -1);
-            for (int i = 0; i < kTileMP; ++i) {
-                t.f1[i] = -1;
-                t.f2[i] = -1;
+    // M-tiles: up to 16 pairs whose joint r3 range fits the windows.
+    struct MT { size_t pb, pe; int lo, hi; };
+    std::vector<MT> mts;
+    for (size_t q = 0; q < pairs.size();) {
+        MT t{q, q, pairs[q].lo, pairs[q].hi};
+        while (t.pe < pairs.size() && t.pe - t.pb < 16) {
+            int lo = std::min(t.lo, pairs[t.pe].lo), hi = std::max(t.hi, pairs[t.pe].hi);
+            if (hi - lo + 1 > kWinRings) break;
+            t.lo = lo;
+            t.hi = hi;
+            t.pe++;
+        }
+        mts.push_back(t);
+        q = t.pe;
+    }
+    // Work items: runs of up to kMaxMT M-tiles whose staged rings fit.
+    for (size_t a = 0; a < mts.size();) {
+        size_t b = a;
+        int r1lo = 1 << 30, r1hi = -1, r2lo = 1 << 30, r2hi = -1, r3lo = 1 << 30, r3hi = -1;
+        while (b < mts.size() && b - a < (size_t)kMaxMT) {
+            int n1lo = r1lo, n1hi = r1hi, n2lo = r2lo, n2hi = r2hi;
+            for (size_t q = mts[b].pb; q < mts[b].pe; ++q) {
+                n1lo = std::min(n1lo, pairs[q].r1); n1hi = std::max(n1hi, pairs[q].r1);
+                n2lo = std::min(n2lo, pairs[q].r2); n2hi = std::max(n2hi, pairs[q].r2);
             }
-            for (size_t q = pb; q < pe; ++q) {
-                int i = (int)(q - pb);
-                t.f1[i] = fid1(pairs[q].r1);
-                t.f2[i] = fid2(pairs[q].r2);
-                for (size_t k = pairs[q].b; k < pairs[q].e; ++k) {
-                    int j = items[k].r3 - w;
-                    if (j >= 0 && j < nt) {
-                        outmap[t.out + i * kTileNT + j] = items[k].cidx;
-                        any = true;
-                    }
+            int n3lo = std::min(r3lo, mts[b].lo), n3hi = std::max(r3hi, mts[b].hi);
+            int nf = (n1hi - n1lo + 1) + (n2hi - n2lo + 1) + (n3hi - n3lo + 1);
+            if (b > a && nf > kFieldCap) break;
+            r1lo = n1lo; r1hi = n1hi; r2lo = n2lo; r2hi = n2hi; r3lo = n3lo; r3hi = n3hi;
+            ++b;
+        }
+        WorkItem it;
+        std::memset(&it, 0, sizeof it);
+        it.fbase[0] = slot1(r1lo);
+        it.fbase[1] = slot2(r2lo);
+        it.fbase[2] = slot3(r3lo);
+        it.nf[0] = (int16_t)(r1hi - r1lo + 1);
+        it.nf[1] = (int16_t)(r2hi - r2lo + 1);
+        it.nf[2] = (int16_t)(r3hi - r3lo + 1);
+        it.conj2 = (int16_t)conj2;
+        it.conj3 = (int16_t)conj3;
+        it.n_mt = (int16_t)(b - a);
+        it.mt0 = (int32_t)W.mtiles.size();
+        W.max_fields = std::max(W.max_fields, it.nf[0] + it.nf[1] + it.nf[2]);
+        for (size_t u = a; u < b; ++u) {
+            const MT &t = mts[u];
+            MTile M;
+            std::memset(&M, 0, sizeof M);
+            for (int m = 0; m < 16; ++m) { M.a1[m] = -1; M.a2[m] = 0; }
+            int nwin = (t.hi - t.lo) / 16 + 1;
+            M.tw = W.ntw;
+            for (int v = 0; v < kMaxWin; ++v) {
+                if (v < nwin) {
+                    const int ws = t.lo + 16 * v;
+                    M.w0[v] = (int16_t)(ws - r3lo);
+                    M.wn[v] = (int16_t)std::min(16, t.hi - ws + 1);
+                } else {
+                    M.w0[v] = 0;
+                    M.wn[v] = 0;
                 }
             }
-            if (!any) {
-                outmap.resize(t.out);
-                continue;
+            for (size_t q = t.pb; q < t.pe; ++q) {
+                const int m = (int)(q - t.pb);
+                M.a1[m] = (int16_t)(pairs[q].r1 - r1lo);
+                M.a2[m] = (int16_t)(pairs[q].r2 - r2lo);
+                for (size_t k = pairs[q].b; k < pairs[q].e; ++k) {
+                    const int d = cs[k].r3 - t.lo;
+                    const int v = d / 16, col = d % 16;
+                    pos[cs[k].cid] = (M.tw + v) * 256 + m * 16 + col;
+                }
             }
-            t.f3 = fid3(w);
-            t.nt = nt;
-            t.conj2 = conj2;
-            t.conj3 = conj3;
-            tiles.push_back(t);
+            W.ntw += nwin;
+            W.mtiles.push_back(M);
         }
+        W.items.push_back(it);
+        a = b;
     }
+    W.tile_slots = (long long)W.ntw * 256;
+}
+
+int auto_splits(int nitems)
+{
+    int s = (2048 + std::max(nitems, 1) - 1) / std::max(nitems, 1);
+    return std::max(1, std::min(s, 32));
 }
 
 }  // namespace
 
 struct bb_estimator {
     const bb_plan *plan = nullptr;
-    bb_precision prec = BB_FP64;
+    bb_options opt{};
+    CPrec cprec = CPrec::F64;
     hipStream_t stream = nullptr;
     FftDesc fd{};
-    int NN = 0;
-    long long npts = 0;
+    int NN = 0, NG = 0, kmax = 0;
+    double scale = 1.0;            // (NN/NG)^2
+    long long npts = 0;            // NG^2
     std::vector<int> rlo, rhi, base;  // per plane ring range and first slot
     int nfields = 0;
     std::vector<void *> owned;
     double2 *d_tw = nullptr;
-    double2 *d_field = nullptr;
+    double2 *d_field = nullptr;    // staging for bb_estimate (host input)
     double2 *d_planes = nullptr;
     void *d_cache = nullptr;
-    double2 *d_rowbuf = nullptr;  // FP32 path scratch
+    double2 *d_rowbuf = nullptr;   // FP32 caches: FP64 row-pass scratch
     int rowbuf_fields = 0;
     FieldTask *d_tasks = nullptr;
     int ntasks = 0;
     int max_rows = 0;
-    Tile *d_tiles = nullptr;
-    int ntiles = 0;
-    int32_t *d_outmap = nullptr;
-    double *d_S = nullptr;
+    WorkItem *d_items = nullptr;
+    MTile *d_mtiles = nullptr;
+    int nitems = 0, nsplit = 1, chunk = 32, max_fields = 0;
+    long long tile_slots = 0;
+    double *d_part = nullptr;      // [nsplit][tile_slots]
+    std::vector<int32_t> pos;      // contraction -> tile position
     BinRec *d_recs = nullptr;
     int32_t *d_seg = nullptr;
     double *d_num = nullptr;
+    std::vector<double> num_host;
     int *d_nz = nullptr;
     double *d_avg = nullptr;
     double *d_bins = nullptr;
@@ -182,6 +255,7 @@ struct bb_estimator {
     }
     ~bb_estimator()
     {
+        if (stream) (void)hipStreamSynchronize(stream);
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
         for (void *p : owned) (void)hipFree(p);
@@ -197,25 +271,47 @@ int upload(void *dst, const void *src, size_t bytes, hipStream_t s)
     return BB_OK;
 }
 
-// FFT pipeline for a list of tasks into a cache of the given element type.
+// FFT pipeline for a list of tasks into a cache (FP64 in place, or FP32 via
+// the FP64 row-pass scratch).
 int run_fields(bb_estimator *E, const double2 *planes, const FieldTask *d_tasks, int ntasks,
                int max_rows, void *cache, bool f32, int *nz)
 {
-    const long long NN2 = E->npts;
+    const long long NG2 = E->npts;
     if (!f32) {
-        HIPCHK(launch_rowpass(planes, (double2 *)cache, NN2, d_tasks, ntasks, max_rows, E->fd,
+        HIPCHK(launch_rowpass(planes, (double2 *)cache, NG2, d_tasks, ntasks, max_rows, E->fd,
                               E->d_tw, nz, E->stream));
-        HIPCHK(launch_colpass_f64((double2 *)cache, NN2, (double2 *)cache, d_tasks, ntasks, E->fd,
+        HIPCHK(launch_colpass_f64((double2 *)cache, NG2, (double2 *)cache, d_tasks, ntasks, E->fd,
                                   E->d_tw, E->stream));
         return BB_OK;
     }
     for (int t0 = 0; t0 < ntasks; t0 += E->rowbuf_fields) {
         int nb = std::min(E->rowbuf_fields, ntasks - t0);
-        HIPCHK(launch_rowpass(planes, E->d_rowbuf, -NN2, d_tasks + t0, nb, max_rows, E->fd,
+        HIPCHK(launch_rowpass(planes, E->d_rowbuf, -NG2, d_tasks + t0, nb, max_rows, E->fd,
                               E->d_tw, nz, E->stream));
-        HIPCHK(launch_colpass_f32(E->d_rowbuf, -NN2, (float2 *)cache, d_tasks + t0, nb, E->fd,
+        HIPCHK(launch_colpass_f32(E->d_rowbuf, -NG2, (float2 *)cache, d_tasks + t0, nb, E->fd,
                                   E->d_tw, E->stream));
     }
+    return BB_OK;
+}
+
+int choose_grid(bb_estimator *E, const PlanData &d)
+{
+    const int NN = d.g.NN;
+    int kmax = 0;
+    for (int r = 0; r < d.g.Nr; ++r)
+        if (d.nhi[r] > 0) kmax = std::max(kmax, isqrt_floor(d.nhi[r] - 1));
+    E->kmax = kmax;
+    const int need = 3 * kmax + 1;
+    E->NG = NN;
+    if (!E->opt.full_grid && NN >= need) {
+        for (int n = need; n <= NN; ++n)
+            if ((n % 2 == 0) && smooth235(n)) {
+                E->NG = n;
+                break;
+            }
+    }
+    E->scale = ((double)NN * NN) / ((double)E->NG * E->NG);
+    E->npts = (long long)E->NG * E->NG;
     return BB_OK;
 }
 
@@ -224,8 +320,9 @@ int estimator_init(bb_estimator *E)
     const PlanData &d = E->plan->d;
     const int NN = d.g.NN, Nmax = d.g.Nmax, Nr = d.g.Nr;
     E->NN = NN;
-    E->npts = (long long)NN * NN;
-    if (!make_fft_desc(NN, E->fd)) return set_error(BB_E_INVALID, "unsupported FFT length");
+    choose_grid(E, d);
+    const int NG = E->NG;
+    if (!make_fft_desc(NG, E->fd)) return set_error(BB_E_INVALID, "unsupported FFT length");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
         (void)hipGetLastError();
@@ -234,61 +331,83 @@ int estimator_init(bb_estimator *E)
     HIPCHK(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
     for (auto &e : E->ev) HIPCHK(hipEventCreate(&e));
     int rc;
-    const bool f32 = (E->prec != BB_FP64);
+    const bool f32 = (E->cprec != CPrec::F64);
 
-    // Twiddles exp(+2 pi i t / N), FP64.
-    std::vector<double2> tw(NN);
-    for (int t = 0; t < NN; ++t) {
-        long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)t / NN;
+    // Twiddles exp(+2 pi i t / NG), FP64.
+    std::vector<double2> tw(NG);
+    for (int t = 0; t < NG; ++t) {
+        long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)t / NG;
         tw[t] = make_double2((double)cosl(a), (double)sinl(a));
     }
-    if ((rc = dev_alloc(&E->d_tw, NN, E->owned))) return rc;
-    if ((rc = upload(E->d_tw, tw.data(), sizeof(double2) * NN, E->stream))) return rc;
+    if ((rc = dev_alloc(&E->d_tw, NG, E->owned))) return rc;
+    if ((rc = upload(E->d_tw, tw.data(), sizeof(double2) * NG, E->stream))) return rc;
 
-    // Ring bounds and row bands.
     std::vector<int> rowband(Nr);
     for (int r = 0; r < Nr; ++r)
-        rowband[r] = d.nhi[r] > 0 ? std::min(NN / 2, isqrt_floor(d.nhi[r] - 1)) : -1;
+        rowband[r] = d.nhi[r] > 0 ? std::min(NG / 2, isqrt_floor(d.nhi[r] - 1)) : -1;
 
     // --- Triangle counts Num on the GPU (calCR, funcs.c:328-424) -----------
+    // Unit fields I_r = IFFT2(mask_r) in FP64, then Num = sum_x I1 I2 I3 for
+    // every ring triple with the FP64 contraction kernel.
     {
         const int rcount = (int)(d.tri.size() / 3);
         std::vector<FieldTask> ut(Nr);
         int mr = 0;
         for (int r = 0; r < Nr; ++r) {
             ut[r] = FieldTask{-1, r, r, rowband[r], d.nlo[r], d.nhi[r]};
-            mr = std::max(mr, std::min(NN, 2 * rowband[r] + 1));
+            mr = std::max(mr, std::min(NG, 2 * rowband[r] + 1));
         }
-        std::vector<void *> tmp_owned;
+        std::vector<CItem> cs;
+        for (int rp = 0; rp < rcount; ++rp)
+            cs.push_back({d.tri[3 * rp], d.tri[3 * rp + 1], d.tri[3 * rp + 2], rp});
+        WorkLists W;
+        std::vector<int32_t> upos(std::max(rcount, 1), 0);
+        auto id = [](int r) { return r; };
+        if (rcount > 0) build_group(cs, 0, 0, id, id, id, W, upos);
+        const int usplit = auto_splits((int)W.items.size());
+        const int uchunk = contract_chunk(CPrec::F64, std::max(W.max_fields, 1));
+        std::vector<void *> tmp;
         FieldTask *d_ut;
         double2 *d_unit;
-        Tile *d_ut_tiles;
-        int32_t *d_ut_out;
-        std::vector<Item> items;
-        for (int rp = 0; rp < rcount; ++rp)
-            items.push_back({d.tri[3 * rp], d.tri[3 * rp + 1], d.tri[3 * rp + 2], rp});
-        std::vector<Tile> tiles;
-        std::vector<int32_t> outmap;
-        auto id = [](int r) { return r; };
-        build_tiles(items, 0, 0, id, id, id, tiles, outmap);
-        if ((rc = dev_alloc(&E->d_num, std::max(rcount, 1), E->owned))) return rc;
-        rc = dev_alloc(&d_ut, Nr, tmp_owned);
-        if (!rc) rc = dev_alloc(&d_unit, (size_t)Nr * E->npts, tmp_owned);
-        if (!rc) rc = dev_alloc(&d_ut_tiles, tiles.size(), tmp_owned);
-        if (!rc) rc = dev_alloc(&d_ut_out, outmap.size(), tmp_owned);
+        WorkItem *d_uitems;
+        MTile *d_umt;
+        double *d_upart;
+        rc = dev_alloc(&d_ut, Nr, tmp);
+        if (!rc) rc = dev_alloc(&d_unit, (size_t)Nr * E->npts, tmp);
+        if (!rc) rc = dev_alloc(&d_uitems, W.items.size(), tmp);
+        if (!rc) rc = dev_alloc(&d_umt, W.mtiles.size(), tmp);
+        if (!rc) rc = dev_alloc(&d_upart, (size_t)usplit * std::max<long long>(W.tile_slots, 1), tmp);
         if (!rc) rc = upload(d_ut, ut.data(), sizeof(FieldTask) * Nr, E->stream);
-        if (!rc) rc = upload(d_ut_tiles, tiles.data(), sizeof(Tile) * tiles.size(), E->stream);
-        if (!rc) rc = upload(d_ut_out, outmap.data(), sizeof(int32_t) * outmap.size(), E->stream);
+        if (!rc && !W.items.empty())
+            rc = upload(d_uitems, W.items.data(), sizeof(WorkItem) * W.items.size(), E->stream);
+        if (!rc && !W.mtiles.empty())
+            rc = upload(d_umt, W.mtiles.data(), sizeof(MTile) * W.mtiles.size(), E->stream);
         if (!rc) rc = run_fields(E, nullptr, d_ut, Nr, mr, d_unit, false, nullptr);
-        if (!rc) {
-            hipError_t e = launch_contract_f64(d_unit, E->npts, d_ut_tiles, (int)tiles.size(),
-                                               d_ut_out, E->d_num, E->stream);
+        if (!rc && rcount > 0) {
+            hipError_t e = launch_contract(CPrec::F64, d_unit, E->npts, d_uitems,
+                                           (int)W.items.size(), d_umt, usplit, uchunk,
+                                           W.max_fields, d_upart, W.tile_slots, E->scale,
+                                           E->stream);
             if (e != hipSuccess) rc = set_error(BB_E_DEVICE, hipGetErrorString(e));
         }
+        std::vector<double> part((size_t)usplit * std::max<long long>(W.tile_slots, 1));
+        if (!rc && hipMemcpyAsync(part.data(), d_upart, sizeof(double) * part.size(),
+                                  hipMemcpyDeviceToHost, E->stream) != hipSuccess)
+            rc = set_error(BB_E_DEVICE, "triangle-count copy failed");
         if (!rc && hipStreamSynchronize(E->stream) != hipSuccess)
             rc = set_error(BB_E_DEVICE, "triangle-count kernels failed");
-        for (void *p : tmp_owned) (void)hipFree(p);
+        for (void *p : tmp) (void)hipFree(p);
         if (rc) return rc;
+        E->num_host.assign(std::max(rcount, 1), 0.0);
+        for (int rp = 0; rp < rcount; ++rp) {
+            double v = 0.0;
+            for (int s = 0; s < usplit; ++s) v += part[(size_t)s * W.tile_slots + upos[rp]];
+            E->num_host[rp] = v;
+        }
+        if ((rc = dev_alloc(&E->d_num, E->num_host.size(), E->owned))) return rc;
+        if ((rc = upload(E->d_num, E->num_host.data(), sizeof(double) * E->num_host.size(),
+                         E->stream)))
+            return rc;
     }
 
     // --- Field layout: per plane the contiguous range of rings in use ------
@@ -308,29 +427,33 @@ int estimator_init(bb_estimator *E)
         if (E->rlo[p] < 0) continue;
         for (int r = E->rlo[p]; r <= E->rhi[p]; ++r) {
             tasks.push_back(FieldTask{p, r, E->nfields, rowband[r], d.nlo[r], d.nhi[r]});
-            E->max_rows = std::max(E->max_rows, std::min(NN, 2 * rowband[r] + 1));
+            E->max_rows = std::max(E->max_rows, std::min(NG, 2 * rowband[r] + 1));
             E->nfields++;
         }
     }
     E->ntasks = (int)tasks.size();
 
-    // --- Contraction tiles per plane group ----------------------------------
-    std::map<std::pair<int, int>, std::vector<Item>> groups;
+    // --- Contraction work lists per plane group ------------------------------
+    std::map<std::pair<int, int>, std::vector<CItem>> groups;
     for (size_t c = 0; c < d.contr.size(); ++c) {
         const Contraction &k = d.contr[c];
         groups[{k.p1, k.p2}].push_back({k.r1, k.r2, k.r3, (int)c});
     }
-    std::vector<Tile> tiles;
-    std::vector<int32_t> outmap;
+    WorkLists W;
+    E->pos.assign(std::max<size_t>(d.contr.size(), 1), 0);
     for (auto &kv : groups) {
         const int p1 = kv.first.first, p2 = kv.first.second, p3 = -p1 - p2;
         const int a2 = std::abs(p2), a3 = std::abs(p3);
-        auto f1 = [&](int r) { return E->fid(p1, r); };
-        auto f2 = [&](int r) { return E->fid(a2, r); };
-        auto f3 = [&](int r) { return E->fid(a3, r); };
-        build_tiles(kv.second, p2 < 0, p3 < 0, f1, f2, f3, tiles, outmap);
+        auto s1 = [&](int r) { return E->fid(p1, r); };
+        auto s2 = [&](int r) { return E->fid(a2, r); };
+        auto s3 = [&](int r) { return E->fid(a3, r); };
+        build_group(kv.second, p2 < 0, p3 < 0, s1, s2, s3, W, E->pos);
     }
-    E->ntiles = (int)tiles.size();
+    E->nitems = (int)W.items.size();
+    E->max_fields = std::max(W.max_fields, 1);
+    E->tile_slots = std::max<long long>(W.tile_slots, 1);
+    E->nsplit = E->opt.x_splits > 0 ? E->opt.x_splits : auto_splits(E->nitems);
+    E->chunk = contract_chunk(E->cprec, E->max_fields);
 
     // --- Binning records, sorted by bin --------------------------------------
     std::vector<BinRec> recs(d.contr.size());
@@ -338,60 +461,59 @@ int estimator_init(bb_estimator *E)
     for (const Contraction &k : d.contr) seg[k.bin + 1]++;
     for (int b = 0; b < d.nbins; ++b) seg[b + 1] += seg[b];
     {
-        std::vector<int32_t> pos(seg.begin(), seg.end() - 1);
+        std::vector<int32_t> at(seg.begin(), seg.end() - 1);
         for (size_t c = 0; c < d.contr.size(); ++c) {
             const Contraction &k = d.contr[c];
             BinRec R;
             R.p1 = k.p1; R.p2 = k.p2; R.p3 = k.p3;
             R.r1 = k.r1; R.r2 = k.r2; R.r3 = k.r3;
             R.rp = k.rp;
-            R.cidx = (int32_t)c;
+            R.pos = E->pos[c];
             R.f1 = E->fid(k.p1, k.r1);
             R.f2 = E->fid(std::abs(k.p2), k.r2);
             R.f3 = E->fid(std::abs(k.p3), k.r3);
-            recs[pos[k.bin]++] = R;
+            recs[at[k.bin]++] = R;
         }
     }
     E->bp.nbins = d.nbins;
     E->bp.n_mtbin = d.g.n_mtbin;
     E->bp.NN = NN;
-    E->bp.LL_k1 = d.g.LL_k1;
-    E->bp.BW_k1 = d.g.BW_k1;
+    E->bp.nsplit = E->nsplit;
+    E->bp.split_stride = E->tile_slots;
     E->bp.Cunits = 2. * M_PI / (NN * d.g.LL);
     E->bp.ff = d.g.ff;
     E->bp.beta = d.g.beta;
 
     // --- Device buffers --------------------------------------------------------
-    const size_t H = NN / 2 + 1;
-    if ((rc = dev_alloc(&E->d_field, (size_t)NN * NN * H, E->owned))) return rc;
     if ((rc = dev_alloc(&E->d_planes, (size_t)Nmax * E->npts, E->owned))) return rc;
+    const size_t nf = (size_t)std::max(E->nfields, 1);
     if (f32) {
         float2 *c;
-        if ((rc = dev_alloc(&c, (size_t)std::max(E->nfields, 1) * E->npts, E->owned))) return rc;
+        if ((rc = dev_alloc(&c, nf * E->npts, E->owned))) return rc;
         E->d_cache = c;
         E->rowbuf_fields = std::max(1, std::min(E->ntasks, 128));
         if ((rc = dev_alloc(&E->d_rowbuf, (size_t)E->rowbuf_fields * E->npts, E->owned))) return rc;
     } else {
         double2 *c;
-        if ((rc = dev_alloc(&c, (size_t)std::max(E->nfields, 1) * E->npts, E->owned))) return rc;
+        if ((rc = dev_alloc(&c, nf * E->npts, E->owned))) return rc;
         E->d_cache = c;
     }
     if ((rc = dev_alloc(&E->d_tasks, tasks.size(), E->owned))) return rc;
-    if ((rc = dev_alloc(&E->d_tiles, tiles.size(), E->owned))) return rc;
-    if ((rc = dev_alloc(&E->d_outmap, outmap.size(), E->owned))) return rc;
-    if ((rc = dev_alloc(&E->d_S, d.contr.size(), E->owned))) return rc;
+    if ((rc = dev_alloc(&E->d_items, W.items.size(), E->owned))) return rc;
+    if ((rc = dev_alloc(&E->d_mtiles, W.mtiles.size(), E->owned))) return rc;
+    if ((rc = dev_alloc(&E->d_part, (size_t)E->nsplit * E->tile_slots, E->owned))) return rc;
     if ((rc = dev_alloc(&E->d_recs, recs.size(), E->owned))) return rc;
     if ((rc = dev_alloc(&E->d_seg, seg.size(), E->owned))) return rc;
-    if ((rc = dev_alloc(&E->d_nz, std::max(E->nfields, 1), E->owned))) return rc;
+    if ((rc = dev_alloc(&E->d_nz, nf, E->owned))) return rc;
     if ((rc = dev_alloc(&E->d_avg, Nr, E->owned))) return rc;
     if ((rc = dev_alloc(&E->d_bins, (size_t)3 * d.nbins * BB_NPOLES, E->owned))) return rc;
-    if ((rc = upload(E->d_tasks, tasks.data(), sizeof(FieldTask) * tasks.size(), E->stream))) return rc;
-    if ((rc = upload(E->d_tiles, tiles.data(), sizeof(Tile) * tiles.size(), E->stream))) return rc;
-    if ((rc = upload(E->d_outmap, outmap.data(), sizeof(int32_t) * outmap.size(), E->stream))) return rc;
-    if ((rc = upload(E->d_recs, recs.data(), sizeof(BinRec) * recs.size(), E->stream))) return rc;
+    if (!tasks.empty() && (rc = upload(E->d_tasks, tasks.data(), sizeof(FieldTask) * tasks.size(), E->stream))) return rc;
+    if (!W.items.empty() && (rc = upload(E->d_items, W.items.data(), sizeof(WorkItem) * W.items.size(), E->stream))) return rc;
+    if (!W.mtiles.empty() && (rc = upload(E->d_mtiles, W.mtiles.data(), sizeof(MTile) * W.mtiles.size(), E->stream))) return rc;
+    if (!recs.empty() && (rc = upload(E->d_recs, recs.data(), sizeof(BinRec) * recs.size(), E->stream))) return rc;
     if ((rc = upload(E->d_seg, seg.data(), sizeof(int32_t) * seg.size(), E->stream))) return rc;
     if ((rc = upload(E->d_avg, d.avg.data(), sizeof(double) * Nr, E->stream))) return rc;
-    HIPCHK(hipMemsetAsync(E->d_S, 0, sizeof(double) * std::max<size_t>(d.contr.size(), 1), E->stream));
+    HIPCHK(hipMemsetAsync(E->d_part, 0, sizeof(double) * E->nsplit * E->tile_slots, E->stream));
     HIPCHK(hipStreamSynchronize(E->stream));
     return BB_OK;
 }
@@ -400,30 +522,43 @@ int estimate_device(bb_estimator *E, const double *d_field_in, double *d_bs, dou
                     double *d_biana)
 {
     const PlanData &d = E->plan->d;
-    const bool f32 = (E->prec != BB_FP64);
+    const bool f32 = (E->cprec != CPrec::F64);
     hipStream_t s = E->stream;
     HIPCHK(hipEventRecord(E->ev[0], s));
     const int nf = std::max(E->nfields, 1);
     hipLaunchKernelGGL(k_fill_int, dim3((nf + 255) / 256), dim3(256), 0, s, E->d_nz, nf, 0);
     HIPCHK(hipGetLastError());
-    HIPCHK(launch_gather((const double2 *)d_field_in, E->d_planes, E->NN, d.g.Nmax, s));
+    HIPCHK(launch_gather((const double2 *)d_field_in, E->d_planes, E->NN, E->NG, E->kmax,
+                         d.g.Nmax, s));
     HIPCHK(hipEventRecord(E->ev[1], s));
     int rc = run_fields(E, E->d_planes, E->d_tasks, E->ntasks, E->max_rows, E->d_cache, f32, E->d_nz);
     if (rc) return rc;
     HIPCHK(hipEventRecord(E->ev[2], s));
-    if (f32)
-        HIPCHK(launch_contract_f32((const float2 *)E->d_cache, E->npts, E->d_tiles, E->ntiles,
-                                   E->d_outmap, E->d_S, s));
-    else
-        HIPCHK(launch_contract_f64((const double2 *)E->d_cache, E->npts, E->d_tiles, E->ntiles,
-                                   E->d_outmap, E->d_S, s));
+    HIPCHK(launch_contract(E->cprec, E->d_cache, E->npts, E->d_items, E->nitems, E->d_mtiles,
+                           E->nsplit, E->chunk, E->max_fields, E->d_part, E->tile_slots,
+                           E->scale, s));
     HIPCHK(hipEventRecord(E->ev[3], s));
-    HIPCHK(launch_binning(E->d_seg, E->d_recs, E->d_S, E->d_num, E->d_nz, E->d_avg, E->bp, d_bs,
-                          d_norm, d_biana, s));
+    HIPCHK(launch_binning(E->d_seg, E->d_recs, E->d_part, E->d_num, E->d_nz, E->d_avg, E->bp,
+                          d_bs, d_norm, d_biana, s));
     HIPCHK(hipEventRecord(E->ev[4], s));
     HIPCHK(hipStreamSynchronize(s));
     for (int k = 0; k < 4; ++k) HIPCHK(hipEventElapsedTime(&E->ms[k], E->ev[k], E->ev[k + 1]));
     return BB_OK;
+}
+
+size_t cache_fields(const PlanData &d)
+{
+    size_t nf = 0;
+    for (int pl = 0; pl < d.g.Nmax; ++pl) {
+        int lo = -1, hi = -2;
+        for (int r = 0; r < d.g.Nr; ++r)
+            if (d.field_used[(size_t)pl * d.g.Nr + r]) {
+                if (lo < 0) lo = r;
+                hi = r;
+            }
+        if (lo >= 0) nf += (size_t)(hi - lo + 1);
+    }
+    return nf;
 }
 
 }  // namespace
@@ -434,34 +569,36 @@ int bb_estimator_footprint(const bb_plan *p, bb_precision prec, size_t *bytes)
 {
     if (!p || !bytes) return set_error(BB_E_INVALID, "null argument");
     const PlanData &d = p->d;
+    // Upper bound: full-size grid.
     const size_t NN = d.g.NN, NN2 = NN * NN, Nr = d.g.Nr, Nmax = d.g.Nmax;
-    size_t nf = 0;
-    for (size_t pl = 0; pl < Nmax; ++pl) {
-        int lo = -1, hi = -2;
-        for (size_t r = 0; r < Nr; ++r)
-            if (d.field_used[pl * Nr + r]) { if (lo < 0) lo = (int)r; hi = (int)r; }
-        if (lo >= 0) nf += (size_t)(hi - lo + 1);
-    }
+    const size_t nf = cache_fields(d);
     size_t b = NN * NN * (NN / 2 + 1) * 16 + Nmax * NN2 * 16;
     b += nf * NN2 * (prec == BB_FP64 ? 16 : 8);
     if (prec != BB_FP64) b += std::min<size_t>(nf, 128) * NN2 * 16;
-    b += d.contr.size() * (8 + sizeof(BinRec) + 4 * kTileNT);
+    b += d.contr.size() * (sizeof(BinRec) + 8 * 3 * 2);
     b += Nr * NN2 * 16;  // transient unit fields (triangle counts)
     *bytes = b;
     return BB_OK;
 }
 
-int bb_estimator_create(const bb_plan *p, bb_precision prec, bb_estimator **out)
+int bb_estimator_create_opts(const bb_plan *p, const bb_options *opt, bb_estimator **out)
 {
-    if (!p || !out) return set_error(BB_E_INVALID, "null argument");
+    if (!p || !opt || !out) return set_error(BB_E_INVALID, "null argument");
     *out = nullptr;
-    if (prec != BB_FP64 && prec != BB_FP32)
-        return set_error(BB_E_INVALID, prec == BB_BF16X3 ? "BF16x3 contraction is not built yet"
-                                                         : "unknown precision");
+    CPrec cp;
+    switch (opt->precision) {
+    case BB_FP64: cp = CPrec::F64; break;
+    case BB_FP32: cp = CPrec::F32; break;
+    case BB_BF16X3: cp = CPrec::BF16X3; break;
+    default: return set_error(BB_E_INVALID, "unknown precision");
+    }
+    if (opt->x_splits < 0 || opt->x_splits > 1024)
+        return set_error(BB_E_INVALID, "x_splits must be in [0, 1024]");
     bb_estimator *E = new (std::nothrow) bb_estimator;
     if (!E) return set_error(BB_E_NOMEM, "out of host memory");
     E->plan = p;
-    E->prec = prec;
+    E->opt = *opt;
+    E->cprec = cp;
     int rc;
     try {
         rc = estimator_init(E);
@@ -476,7 +613,22 @@ int bb_estimator_create(const bb_plan *p, bb_precision prec, bb_estimator **out)
     return BB_OK;
 }
 
+int bb_estimator_create(const bb_plan *p, bb_precision prec, bb_estimator **out)
+{
+    bb_options o;
+    std::memset(&o, 0, sizeof o);
+    o.precision = prec;
+    return bb_estimator_create_opts(p, &o, out);
+}
+
 void bb_estimator_destroy(bb_estimator *e) { delete e; }
+
+int bb_estimator_grid(const bb_estimator *e, int *NG)
+{
+    if (!e || !NG) return set_error(BB_E_INVALID, "null argument");
+    *NG = e->NG;
+    return BB_OK;
+}
 
 int bb_estimate_device(bb_estimator *e, const double *d_field, double *d_bs, double *d_norm,
                        double *d_biana)
@@ -491,6 +643,10 @@ int bb_estimate(bb_estimator *e, const double *field, double *bs, double *norm, 
     const PlanData &d = e->plan->d;
     const size_t NN = d.g.NN;
     const size_t bytes = NN * NN * (NN + 2) * sizeof(double);
+    if (!e->d_field) {
+        int rc = dev_alloc(&e->d_field, NN * NN * (NN / 2 + 1), e->owned);
+        if (rc) return rc;
+    }
     HIPCHK(hipMemcpyAsync(e->d_field, field, bytes, hipMemcpyHostToDevice, e->stream));
     const size_t nb = (size_t)d.nbins * BB_NPOLES;
     double *B = e->d_bins;
@@ -505,15 +661,22 @@ int bb_estimate(bb_estimator *e, const double *field, double *bs, double *norm, 
 int bb_estimator_contraction_sums(bb_estimator *e, double *S)
 {
     if (!e || !S) return set_error(BB_E_INVALID, "null argument");
-    HIPCHK(hipMemcpy(S, e->d_S, e->plan->d.contr.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double> part((size_t)e->nsplit * e->tile_slots);
+    HIPCHK(hipMemcpy(part.data(), e->d_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost));
+    const size_t n = e->plan->d.contr.size();
+    for (size_t c = 0; c < n; ++c) {
+        double v = 0.0;
+        for (int s = 0; s < e->nsplit; ++s) v += part[(size_t)s * e->tile_slots + e->pos[c]];
+        S[c] = v;
+    }
     return BB_OK;
 }
 
 int bb_estimator_triangle_counts(bb_estimator *e, double *num)
 {
     if (!e || !num) return set_error(BB_E_INVALID, "null argument");
-    HIPCHK(hipMemcpy(num, e->d_num, (e->plan->d.tri.size() / 3) * sizeof(double),
-                     hipMemcpyDeviceToHost));
+    const size_t n = e->plan->d.tri.size() / 3;
+    for (size_t k = 0; k < n; ++k) num[k] = e->num_host[k];
     return BB_OK;
 }
 
@@ -523,7 +686,7 @@ int bb_estimator_field(bb_estimator *e, int plane, int ring, double *out)
     const int f = e->fid(plane, ring);
     if (f < 0) return set_error(BB_E_INVALID, "field not held in the cache (unused by the plan)");
     const size_t n = (size_t)e->npts;
-    if (e->prec == BB_FP64) {
+    if (e->cprec == CPrec::F64) {
         HIPCHK(hipMemcpy(out, (const double2 *)e->d_cache + (size_t)f * n, n * sizeof(double2),
                          hipMemcpyDeviceToHost));
     } else {

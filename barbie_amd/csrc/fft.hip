@@ -101,32 +101,47 @@ __device__ double2 *lds_fft(double2 *a, double2 *b, int nb, const FftDesc &fd,
     return a;
 }
 
-// planes[p][i][j] = field[i][j][p] (complex) for p < Nmax.  32x32 transpose
-// tiles through LDS so both the read (along p) and the write (along j) are
-// coalesced.
+// planes[p][i][j] (NG x NG) = field[i mod NN][j mod NN][p] (complex) for
+// p < Nmax, where (i, j) on the NG grid stand for the signed modes
+// ia = i <= NG/2 ? i : i - NG (likewise ja).  Modes with |ia| or |ja| above
+// kmax lie outside every ring and are set to zero.  With NG = NN this is the
+// plain plane gather of Df (funcs.c:97-100).  32x32 transpose tiles through
+// LDS keep both the read (along p) and the write (along j) coalesced.
 __global__ __launch_bounds__(256) void k_gather(const double2 *__restrict__ field,
-                                                double2 *__restrict__ planes, int NN, int Nmax)
+                                                double2 *__restrict__ planes, int NN, int NG,
+                                                int kmax, int Nmax)
 {
     __shared__ double2 tile[32][33];
     const int H = NN / 2 + 1;
     const int i = blockIdx.z;
+    const int ia = (i > NG / 2) ? i - NG : i;
+    const bool rowok = (ia <= kmax && -ia <= kmax) || NG == NN;
+    const int is = ia < 0 ? ia + NN : ia;
     const int j0 = blockIdx.x * 32, p0 = blockIdx.y * 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
     for (int jj = ty; jj < 32; jj += 8) {
-        int j = j0 + jj, p = p0 + tx;
-        if (j < NN && p < Nmax) tile[jj][tx] = field[((long long)i * NN + j) * H + p];
+        const int j = j0 + jj, p = p0 + tx;
+        if (j < NG && p < Nmax) {
+            const int ja = (j > NG / 2) ? j - NG : j;
+            const bool ok = rowok && ((ja <= kmax && -ja <= kmax) || NG == NN);
+            const int js = ja < 0 ? ja + NN : ja;
+            double2 v = make_double2(0.0, 0.0);
+            if (ok) v = field[((long long)is * NN + js) * H + p];
+            tile[jj][tx] = v;
+        }
     }
     __syncthreads();
     for (int pp = ty; pp < 32; pp += 8) {
-        int p = p0 + pp, j = j0 + tx;
-        if (j < NN && p < Nmax) planes[((long long)p * NN + i) * NN + j] = tile[tx][pp];
+        const int p = p0 + pp, j = j0 + tx;
+        if (j < NG && p < Nmax) planes[((long long)p * NG + i) * NG + j] = tile[tx][pp];
     }
 }
 
-hipError_t launch_gather(const double2 *field, double2 *planes, int NN, int Nmax, hipStream_t s)
+hipError_t launch_gather(const double2 *field, double2 *planes, int NN, int NG, int kmax,
+                         int Nmax, hipStream_t s)
 {
-    dim3 grid((NN + 31) / 32, (Nmax + 31) / 32, NN);
-    hipLaunchKernelGGL(k_gather, grid, dim3(256), 0, s, field, planes, NN, Nmax);
+    dim3 grid((NG + 31) / 32, (Nmax + 31) / 32, NG);
+    hipLaunchKernelGGL(k_gather, grid, dim3(256), 0, s, field, planes, NN, NG, kmax, Nmax);
     return hipGetLastError();
 }
 

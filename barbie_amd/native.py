@@ -26,7 +26,8 @@ RINGS_REFERENCE, RINGS_UNIFORM, RINGS_EXPLICIT = 0, 1, 2
 EXPORTED = [
     "bb_last_error", "bb_version", "bb_geometry_reference", "bb_plan_create",
     "bb_plan_destroy", "bb_plan_stats_get", "bb_plan_ring_edges", "bb_plan_triples",
-    "bb_plan_contractions", "bb_estimator_create", "bb_estimator_destroy",
+    "bb_plan_contractions", "bb_estimator_create", "bb_estimator_create_opts",
+    "bb_estimator_destroy", "bb_estimator_grid",
     "bb_estimator_footprint", "bb_estimate", "bb_estimate_device",
     "bb_estimator_contraction_sums", "bb_estimator_triangle_counts", "bb_estimator_field",
     "bb_estimator_timings", "bb_write_outputs",
@@ -51,6 +52,11 @@ class BBPlanStats(ctypes.Structure):
                 ("ncontr", ctypes.c_longlong), ("npairs", ctypes.c_longlong),
                 ("ngroups", ctypes.c_int), ("nfields_used", ctypes.c_int),
                 ("nbins", ctypes.c_int)]
+
+
+class BBOptions(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("full_grid", ctypes.c_int),
+                ("x_splits", ctypes.c_int)]
 
 
 class BarbieError(RuntimeError):
@@ -82,6 +88,9 @@ def lib() -> ctypes.CDLL:
         L.bb_plan_contractions.argtypes = [ctypes.c_void_p, _ip]
         L.bb_estimator_create.argtypes = [ctypes.c_void_p, ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_void_p)]
+        L.bb_estimator_create_opts.argtypes = [ctypes.c_void_p, ctypes.POINTER(BBOptions),
+                                               ctypes.POINTER(ctypes.c_void_p)]
+        L.bb_estimator_grid.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
         L.bb_estimator_destroy.argtypes = [ctypes.c_void_p]
         L.bb_estimator_destroy.restype = None
         L.bb_estimator_footprint.argtypes = [ctypes.c_void_p, ctypes.c_int,
@@ -201,12 +210,17 @@ class Plan:
 class Estimator:
     """Device estimator (bb_estimator) for one plan and precision."""
 
-    def __init__(self, plan: Plan, precision: str = "fp64"):
+    def __init__(self, plan: Plan, precision: str = "fp64", full_grid: bool = False,
+                 x_splits: int = 0):
         self.plan = plan
         self.precision = precision
         h = ctypes.c_void_p()
-        _check(lib().bb_estimator_create(plan.handle, PRECISIONS[precision], ctypes.byref(h)))
+        o = BBOptions(PRECISIONS[precision], int(full_grid), int(x_splits))
+        _check(lib().bb_estimator_create_opts(plan.handle, ctypes.byref(o), ctypes.byref(h)))
         self.handle = h
+        ng = ctypes.c_int()
+        _check(lib().bb_estimator_grid(h, ctypes.byref(ng)))
+        self.grid = ng.value
 
     def close(self):
         h = getattr(self, "handle", None)
@@ -245,8 +259,9 @@ class Estimator:
         return n
 
     def field(self, plane: int, ring: int) -> np.ndarray:
-        NN = self.plan.geom.NN
-        out = np.zeros((NN, NN), dtype=np.complex128)
+        """Plane-ring field D_{plane,ring} on the estimator's grid (NG x NG)."""
+        NG = self.grid
+        out = np.zeros((NG, NG), dtype=np.complex128)
         _check(lib().bb_estimator_field(self.handle, plane, ring, _p(out.view(np.float64))))
         return out
 

@@ -44,13 +44,25 @@ enum {
     BB_RINGS_EXPLICIT = 2   /* caller-supplied low/high edges */
 };
 
-/* Arithmetic of the contraction stage. */
+/* Arithmetic of the contraction stage (the FFTs always run in FP64; the
+ * per-bin sums, triangle counts and Y_lm projection are FP64 in every mode). */
 typedef enum {
-    BB_FP64 = 0,    /* FP64 fields, FP64 contraction (the FP64 GPU oracle) */
-    BB_FP32 = 1,    /* FP32 fields and contraction, FP64 across tiles */
-    BB_BF16X3 = 2   /* FP32 fields, BF16x3 split products on MFMA, FP32 in tiles,
-                       FP64 across tiles */
+    BB_FP64 = 0,    /* FP64 fields, FP64 MFMA contraction (the FP64 GPU oracle) */
+    BB_FP32 = 1,    /* FP32 fields, FP32 MFMA contraction, FP64 across LDS chunks */
+    BB_BF16X3 = 2   /* FP32 fields, BF16x3 split products on MFMA (hi*hi + hi*lo +
+                       lo*hi), FP32 within an LDS chunk, FP64 across chunks */
 } bb_precision;
+
+/* Execution options of an estimator (bb_estimator_create_opts). */
+typedef struct bb_options {
+    bb_precision precision;
+    int full_grid;  /* 0: run the plane-ring transforms and triple sums on the
+                       smallest 2,3,5-smooth grid NG >= 3*kmax+1 (kmax = largest
+                       |k_x| of any ring mode) when NG < NN; the sums are then
+                       exactly (NN/NG)^2 times the NN-grid sums, because no
+                       closed triangle aliases on either grid.  1: always use NN. */
+    int x_splits;   /* partial sums per contraction over the points (0: automatic) */
+} bb_options;
 
 /* Geometry and binning of one estimate.  Replaces the constants hard-coded at
  * barbie.c:58-84 and the ring construction at barbie.c:110-130. */
@@ -112,7 +124,10 @@ int bb_plan_contractions(const bb_plan *p, int *rec);
  * allocates the plane-ring cache and work buffers in HBM and computes the
  * triangle counts Num (calCR, funcs.c:328-424) on the GPU in FP64. */
 int bb_estimator_create(const bb_plan *p, bb_precision prec, bb_estimator **out);
+int bb_estimator_create_opts(const bb_plan *p, const bb_options *opt, bb_estimator **out);
 void bb_estimator_destroy(bb_estimator *e);
+/* Grid the estimator's transforms and sums run on (NG; NN with full_grid). */
+int bb_estimator_grid(const bb_estimator *e, int *NG);
 /* Device bytes held by an estimator of this plan and precision. */
 int bb_estimator_footprint(const bb_plan *p, bb_precision prec, size_t *bytes);
 
@@ -130,9 +145,10 @@ int bb_estimate_device(bb_estimator *e, const double *d_field_r2c,
                        double *d_bin_bs, double *d_bin_norm, double *d_bin_biana);
 
 /* Diagnostics of the last estimate: per-contraction triple-product sums in
- * the reference's enumeration order (ncontr doubles), the GPU triangle counts
- * Num per ring triple (rcount doubles), and one plane-ring field
- * D_{plane,ring} (NN*NN complex, interleaved) converted to FP64. */
+ * the reference's enumeration order (ncontr doubles, scaled to the NN grid),
+ * the GPU triangle counts Num per ring triple (rcount doubles, NN grid), and
+ * one plane-ring field D_{plane,ring} on the estimator's grid (NG*NG complex,
+ * interleaved) converted to FP64. */
 int bb_estimator_contraction_sums(bb_estimator *e, double *S);
 int bb_estimator_triangle_counts(bb_estimator *e, double *num);
 int bb_estimator_field(bb_estimator *e, int plane, int ring, double *out);

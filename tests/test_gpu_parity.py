@@ -39,12 +39,15 @@ def rel_err(a, b):
     return np.max(np.abs(a - b)) / scale if scale > 0 else np.max(np.abs(a))
 
 
+@pytest.mark.parametrize("full_grid", [False, True])
 @pytest.mark.parametrize("gf", [tiny, small, radix5, medium, config1])
-def test_fp64_matches_oracle(gf):
+def test_fp64_matches_oracle(gf, full_grid):
     o = oracle_run(gf, NRi=1)
     g = o["g"]
     plan = B.Plan(g)
-    est = B.Estimator(plan, "fp64")
+    est = B.Estimator(plan, "fp64", full_grid=full_grid)
+    if full_grid:
+        assert est.grid == g.NN
     bs, nm, bi = est.estimate(o["field"])
     # triangle counts Num (calCR) within 1e-12
     assert rel_err(est.triangle_counts(), o["rd"][:, 3]) <= 1e-12
@@ -71,7 +74,7 @@ def test_fp64_plane_ring_fields_match_oracle(gf):
     g = o["g"]
     lo, hi, _ = o["edges"]
     plan = B.Plan(g)
-    est = B.Estimator(plan, "fp64")
+    est = B.Estimator(plan, "fp64", full_grid=True)
     est.estimate(o["field"])
     pc = plan.contractions()
     used = {(int(p), int(r)) for p, r in zip(pc[:, 0], pc[:, 1])}
@@ -104,17 +107,33 @@ def test_fp64_deterministic_and_estimator_reuse():
     assert not np.array_equal(a[0], b[0])
 
 
+# Norm-wise relative error per multipole, ||est - orc||_2 / ||orc||_2 over
+# all bins: FP32 within 2e-6, BF16x3 within 1e-4 of the FP64 oracle.
+@pytest.mark.parametrize("prec,tol", [("fp32", 2e-6), ("bf16x3", 1e-4)])
 @pytest.mark.parametrize("gf", [medium, config1])
-def test_fp32_within_tolerance(gf):
+def test_reduced_precision_within_tolerance(gf, prec, tol):
     o = oracle_run(gf, NRi=1)
     g = o["g"]
-    est = B.Estimator(B.Plan(g), "fp32")
+    est = B.Estimator(B.Plan(g), prec)
     bs, nm, bi = est.estimate(o["field"])
     obs, onm = (o[k].reshape(bs.shape) for k in ("bs", "nm"))
+    errs = []
     for ii in range(16):
         d = np.linalg.norm(bs[..., ii] - obs[..., ii]) / np.linalg.norm(obs[..., ii])
-        assert d <= 2e-5, (ii, d)
+        errs.append(d)
         assert rel_err(nm[..., ii], onm[..., ii]) <= 1e-12
+    print(prec, gf.__name__, "per-multipole rel err", ["%.1e" % e for e in errs])
+    assert max(errs) <= tol, errs
+
+
+@pytest.mark.parametrize("x_splits", [1, 3, 7])
+def test_split_count_does_not_change_result(x_splits):
+    g = medium()
+    f = B.make_field(g.NN, NRi=4)
+    ref = B.Estimator(B.Plan(g), "fp64", x_splits=1).estimate(f)
+    got = B.Estimator(B.Plan(g), "fp64", x_splits=x_splits).estimate(f)
+    for a, b in zip(got, ref):
+        assert rel_err(a, b) <= 1e-13
 
 
 def test_cli_end_to_end(tmp_path):
