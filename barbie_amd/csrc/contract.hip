@@ -2,27 +2,38 @@
 //
 // For a contraction (p1,r1,p2,r2,p3,r3) the reference computes
 //     S = sum_x Re[ D_{p1,r1}(x) * D~_{p2,r2}(x) * D~_{p3,r3}(x) ]
-// (barbie.c:326-337; D~ = conj(D_{|p|,r}) for a negative plane).  Here a
-// plane group's contractions become 16x16 GEMM tiles with K = points:
-//     S[m][n] = sum_x Pr_m(x) Cr_n(x) + Pi_m(x) (-Ci_n(x)),
-// rows m = pair fields P = D1 * D~2 (formed in registers from LDS), columns
-// n = 16 consecutive r3 rings C = D~3.  The real and imaginary halves are two
-// MFMA chains into one accumulator, so every lane forms one pair product per
-// point and nothing is computed twice.
+// (barbie.c:326-337; D~ = conj(D_{|p|,r}) for a negative plane).  Here the
+// contractions of a plane group become 16x16 GEMM tiles with K = points:
+//     S[m][n] = sum_x Pr_m(x) Cr_n(x) + Ai_m(x) Ci_n(x),
+// rows m = pair fields P = D1 * D~2 (formed in registers from LDS),
+// Ai = -Im(P) (or +Im(P) when the third plane is negative, since then
+// -Im(D~3) = +Ci), columns n = 16 consecutive r3 rings C = D_{|p3|}.  The real
+// and imaginary halves are two MFMA chains into one accumulator, so each lane
+// forms one pair product per point and nothing is computed twice.  The two
+// conjugations are sign-bit flips.
 //
 // Arithmetic by precision:
 //  - F64:    v_mfma_f64_16x16x4_f64, FP64 fields, FP64 accumulation;
-//  - F32:    v_mfma_f32_16x16x4_f32, FP32 fields, FP32 accumulation within an
-//            LDS chunk, FP64 across chunks;
-//  - BF16X3: v_mfma_f32_16x16x32_bf16 on hi/lo bf16 splits of the FP32 pair
-//            and column values (hi*hi + hi*lo + lo*hi), FP32 accumulation
-//            within a chunk, FP64 across chunks.
+//  - F32:    v_mfma_f32_16x16x4_f32, FP32 fields, FP32 accumulation over
+//            kFlush chunks, FP64 across;
+//  - BF16X3: v_mfma_f32_16x16x32_bf16 on hi/lo bf16 splits (hi*hi + hi*lo +
+//            lo*hi) of the FP32 pair values and of the r3 fields (the latter
+//            split once, by the FFT pass), FP32 accumulation over kFlush
+//            chunks, FP64 across.
+// Rows of a tile whose pair slot is empty and columns past the window's end
+// read arbitrary staged rings; MFMA keeps rows and columns independent, and
+// the outputs of those slots are never read.
 //
 // Lane layouts (checked on gfx950): for the K=4 forms lane l holds
 // A[l%16][l/16] and B[l/16][l%16]; for 16x16x32 bf16 lane l holds
 // A[l%16][8(l/16)..8(l/16)+7] and B[8(l/16)..+7][l%16].  Output lane l,
 // register j is D[4j + l/16][l%16] for f64 and D[4(l/16) + j][l%16] for the
 // f32-accumulating forms.
+//
+// Staging is software-pipelined: each thread holds its share of the next
+// chunk in registers while the waves work on the current one from LDS.  Every
+// staged ring is 256 bytes of one chunk (plus 16 bytes of padding, so that 16
+// lanes reading 16 different rings at one offset hit distinct banks).
 #include <hip/hip_runtime.h>
 
 #include "device.hpp"
@@ -31,37 +42,84 @@ namespace bb {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-constexpr int kThreads = 256;
-constexpr int kMTW = kMaxMT / 4;   // M-tiles per wave
+constexpr int kTableBytes = 2 * kRowCap * 4;   // row -> cache slot, row -> split flag
+constexpr int kRowBytes = 256;                 // bytes of one ring per chunk
+constexpr int kPitch = kRowBytes + 16;         // LDS bytes per staged ring
+constexpr int kFlush = 4;                      // chunks between FP32 -> FP64 flushes
+// 8-byte staging loads per thread: 30 rings per wave, 32 units per ring.
+constexpr int kPF = 15;
 
-// LDS row pitch (elements) for a chunk: padded so that 16 lanes reading 16
-// different fields at the same point hit distinct banks.
-__host__ __device__ constexpr int pitch_k4_f64(int chunk) { return chunk + 1; }
-__host__ __device__ constexpr int pitch_f32(int chunk) { return chunk + 2; }
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Blocks are dealt round-robin to the 8 XCDs; remap so each XCD works on a
-// contiguous range of work items (neighbouring plane groups share planes, so
-// their fields stay in that XCD's L2).  grid = 8 * per.
-__device__ __forceinline__ int xcd_remap(int b, int per) { return (b & 7) * per + (b >> 3); }
-
-template <typename C2>
-__device__ __forceinline__ void stage_chunk(C2 *st, int pitch, const C2 *__restrict__ cache,
-                                            long long npts, const WorkItem &it, int nft,
-                                            long long x0, int chunk)
+// Fills the row tables of an item (row -> cache slot, row -> split flag).
+template <int NT>
+__device__ __forceinline__ void row_tables(const WorkItem &it, int *row_slot, int *row_split)
 {
-    const int nf0 = it.nf[0], nf01 = it.nf[0] + it.nf[1];
-    for (int idx = threadIdx.x; idx < nft * chunk; idx += kThreads) {
-        const int f = idx / chunk, xi = idx - f * chunk;
-        const int fid = f < nf0 ? it.fbase[0] + f
-                      : f < nf01 ? it.fbase[1] + (f - nf0) : it.fbase[2] + (f - nf01);
-        C2 v;
-        v.x = 0;
-        v.y = 0;
-        if (x0 + xi < npts) v = cache[(long long)fid * npts + x0 + xi];
-        st[f * pitch + xi] = v;
+    for (int r = threadIdx.x; r < it.nrows; r += NT) {
+        int b = 0, start = 0;
+        while (b < it.nblk - 1 && r >= start + it.blk_rows[b]) {
+            start += it.blk_rows[b];
+            ++b;
+        }
+        row_slot[r] = it.blk_slot[b] + (r - start);
+        row_split[r] = it.blk_split[b];
     }
+}
+
+// Staging of one chunk: the item's rings, each kRowBytes contiguous bytes of
+// its field (raw values, or the bf16 split operands for split rows), moved as
+// 8-byte units.  Branch-free: every thread moves kPF units; units past the
+// item's rings read ring 0 and land in a spare LDS ring (row kRowCap), and
+// bytes past the end of the field read as zero.
+template <int NT>
+struct Stager {
+    const char *cache;
+    const char *split;
+    long long field_bytes;   // bytes of one field
+    int nunits;              // 8-byte units this chunk: nrows * kRowBytes / 8
+    int nrows;
+    uint2 pf[kPF];
+
+    __device__ __forceinline__ void load(const int *row_slot, const int *row_split,
+                                         long long chunk_off)
+    {
+#pragma unroll
+        for (int e = 0; e < kPF; ++e) {
+            const int u = threadIdx.x + e * NT;
+            const int r = min(u >> 5, nrows - 1), k = u & 31;   // 32 units per ring
+            const long long off = chunk_off + k * 8;
+            const long long offc = off < field_bytes ? off : 0;
+            const char *base = row_split[r] ? split : cache;
+            uint2 v = *reinterpret_cast<const uint2 *>(base + (long long)row_slot[r] * field_bytes + offc);
+            const bool keep = off < field_bytes;
+            v.x = keep ? v.x : 0u;
+            v.y = keep ? v.y : 0u;
+            pf[e] = v;
+        }
+    }
+    __device__ __forceinline__ void store(unsigned char *rows)
+    {
+#pragma unroll
+        for (int e = 0; e < kPF; ++e) {
+            const int u = threadIdx.x + e * NT;
+            const int r = u < nunits ? (u >> 5) : kRowCap, k = u & 31;
+            *reinterpret_cast<uint2 *>(rows + r * kPitch + k * 8) = pf[e];
+        }
+    }
+};
+
+// Flips the sign of x when the mask's top bit is set.
+__device__ __forceinline__ double flip(double x, unsigned mask)
+{
+    unsigned long long b = __double_as_longlong(x) ^ ((unsigned long long)mask << 32);
+    return __longlong_as_double(b);
+}
+__device__ __forceinline__ float flip(float x, unsigned mask)
+{
+    return __uint_as_float(__float_as_uint(x) ^ mask);
 }
 
 // ------------------------------------------------------------------ K = 4 --
@@ -71,335 +129,317 @@ template <>
 struct K4<double> {
     using C2 = double2;
     using Acc = f64x4;
+    static constexpr int TWW = kTWW;
     static __device__ __forceinline__ Acc mfma(double a, double b, Acc c)
     {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
     static __device__ __forceinline__ int row(int lane, int j) { return 4 * j + (lane >> 4); }
-    static constexpr int pitch(int chunk) { return pitch_k4_f64(chunk); }
+    static constexpr bool kFP64Acc = true;
 };
 template <>
 struct K4<float> {
     using C2 = float2;
     using Acc = f32x4;
+    static constexpr int TWW = kTWWF32;
     static __device__ __forceinline__ Acc mfma(float a, float b, Acc c)
     {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
     static __device__ __forceinline__ int row(int lane, int j) { return 4 * (lane >> 4) + j; }
-    static constexpr int pitch(int chunk) { return pitch_f32(chunk); }
+    static constexpr bool kFP64Acc = false;
 };
 
-template <typename R>
-__global__ __launch_bounds__(kThreads) void k_contract_k4(
+// TWW: tile windows per wave; MINB: workgroups per CU the register budget
+// is sized for.
+template <typename R, int NW, int TWW, int MINB>
+__global__ __launch_bounds__(64 * NW, MINB) void k_contract_k4(
     const typename K4<R>::C2 *__restrict__ cache, long long npts,
-    const WorkItem *__restrict__ items, int nitems, const MTile *__restrict__ mtiles, int nsplit,
-    int chunk, int per, double *__restrict__ out, long long split_stride, double scale)
+    const WorkItem *__restrict__ items, int nitems, const TWin *__restrict__ tws, int nsplit,
+    double *__restrict__ out, long long split_stride, double scale)
 {
     using T = K4<R>;
     using C2 = typename T::C2;
     using Acc = typename T::Acc;
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    C2 *st = reinterpret_cast<C2 *>(lds_raw);
+    constexpr int CH = kRowBytes / (int)sizeof(C2);   // points per chunk
+    extern __shared__ __align__(16) unsigned char lds[];
+    int *row_slot = reinterpret_cast<int *>(lds);
+    int *row_split = row_slot + kRowCap;
+    unsigned char *rows = lds + kTableBytes;
 
-    const int logical = xcd_remap(blockIdx.x, per);
-    if (logical >= nitems * nsplit) return;
-    const int item_id = logical / nsplit, split = logical - item_id * nsplit;
-    const WorkItem it = items[item_id];
-    const int nf0 = it.nf[0], nf01 = it.nf[0] + it.nf[1];
-    const int nft = nf01 + it.nf[2];
-    const int pitch = T::pitch(chunk);
-    const long long nchunks = (npts + chunk - 1) / chunk;
+    const int item_id = blockIdx.x / nsplit, split = blockIdx.x - item_id * nsplit;
+    if (item_id >= nitems) return;
+    const WorkItem &it = items[item_id];
+    const int nrows = uni(it.nrows);
+    row_tables<64 * NW>(it, row_slot, row_split);
+    const long long nchunks = (npts + CH - 1) / CH;
     const long long cb = nchunks * split / nsplit, ce = nchunks * (split + 1) / nsplit;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = uni(threadIdx.x >> 6);
     const int m = lane & 15, kq = lane >> 4;
-    const bool c2 = it.conj2 != 0, c3 = it.conj3 != 0;
 
-    bool has[kMTW];
-    int a1[kMTW], a2[kMTW], w0[kMTW][kMaxWin], wn[kMTW][kMaxWin], tw[kMTW];
+    // Per tile window slot (every slot is filled; unused ones are dummies
+    // whose output index is -1): lane byte offsets of its A rows and B row,
+    // and flags.
+    const TWin *TW = tws + it.tw0 + wave * kTWW;
+    int oa1[TWW], oa2[TWW], ob[TWW], fl[TWW];
 #pragma unroll
-    for (int t = 0; t < kMTW; ++t) {
-        const int idx = wave + 4 * t;
-        has[t] = idx < it.n_mt;
-        a1[t] = -1;
-        a2[t] = 0;
-        tw[t] = 0;
-#pragma unroll
-        for (int v = 0; v < kMaxWin; ++v) { w0[t][v] = 0; wn[t][v] = 0; }
-        if (has[t]) {
-            const MTile &M = mtiles[it.mt0 + idx];
-            a1[t] = M.a1[m];
-            a2[t] = M.a2[m];
-            tw[t] = M.tw;
-#pragma unroll
-            for (int v = 0; v < kMaxWin; ++v) { w0[t][v] = M.w0[v]; wn[t][v] = M.wn[v]; }
-        }
+    for (int j = 0; j < TWW; ++j) {
+        oa1[j] = max((int)TW[j].a1[m], 0) * kPitch + kq * (int)sizeof(C2);
+        oa2[j] = (int)TW[j].a2[m] * kPitch + kq * (int)sizeof(C2);
+        ob[j] = min((int)TW[j].b0 + m, nrows - 1) * kPitch + kq * (int)sizeof(C2);
+        fl[j] = uni(TW[j].flags);
     }
 
-    double accd[kMTW][kMaxWin][4];
+    Acc acc[TWW];
+    double accd[TWW][4];
 #pragma unroll
-    for (int t = 0; t < kMTW; ++t)
+    for (int j = 0; j < TWW; ++j) {
+        acc[j] = Acc{0, 0, 0, 0};
 #pragma unroll
-        for (int v = 0; v < kMaxWin; ++v)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) accd[t][v][j] = 0.0;
+        for (int q = 0; q < 4; ++q) accd[j][q] = 0.0;
+    }
+    Stager<64 * NW> S;
+    S.cache = reinterpret_cast<const char *>(cache);
+    S.split = S.cache;
+    S.field_bytes = npts * (long long)sizeof(C2);
+    S.nunits = nrows * (kRowBytes / 8);
+    S.nrows = nrows;
+    __syncthreads();
 
+    if (cb < ce) S.load(row_slot, row_split, cb * kRowBytes);
+    int since = 0;
     for (long long c = cb; c < ce; ++c) {
-        const long long x0 = c * chunk;
         __syncthreads();
-        stage_chunk<C2>(st, pitch, cache, npts, it, nft, x0, chunk);
+        S.store(rows);
         __syncthreads();
+        if (c + 1 < ce) S.load(row_slot, row_split, (c + 1) * kRowBytes);
 #pragma unroll
-        for (int t = 0; t < kMTW; ++t) {
-            if (!has[t]) continue;
-            Acc acc[kMaxWin];
+        for (int s = 0; s < CH / 4; ++s) {   // 4 points per MFMA pair
+            const int so = s * 4 * (int)sizeof(C2);
+            R pr = 0, ai = 0;
+            C2 bcur = *reinterpret_cast<const C2 *>(rows + ob[0] + so);
 #pragma unroll
-            for (int v = 0; v < kMaxWin; ++v) acc[v] = Acc{0, 0, 0, 0};
-            const C2 *r1p = st + (a1[t] < 0 ? 0 : a1[t]) * pitch;
-            const C2 *r2p = st + (nf0 + a2[t]) * pitch;
-            const R live = a1[t] < 0 ? R(0) : R(1);
-            for (int s = 0; s < chunk; s += 4) {
-                const int pt = s + kq;
-                const C2 d1 = r1p[pt], d2 = r2p[pt];
-                R pr, pi;
-                if (c2) {
-                    pr = d1.x * d2.x + d1.y * d2.y;
-                    pi = d1.y * d2.x - d1.x * d2.y;
-                } else {
-                    pr = d1.x * d2.x - d1.y * d2.y;
-                    pi = d1.x * d2.y + d1.y * d2.x;
+            for (int j = 0; j < TWW; ++j) {
+                const bool newa = (fl[j] & TW_NEWA) != 0;
+                C2 d1, d2;
+                d1.x = d1.y = d2.x = d2.y = 0;
+                if (newa) {
+                    d1 = *reinterpret_cast<const C2 *>(rows + oa1[j] + so);
+                    d2 = *reinterpret_cast<const C2 *>(rows + oa2[j] + so);
                 }
-                pr *= live;
-                pi *= live;
-#pragma unroll
-                for (int v = 0; v < kMaxWin; ++v) {
-                    if (wn[t][v] == 0) continue;
-                    C2 d3;
-                    d3.x = 0;
-                    d3.y = 0;
-                    if (m < wn[t][v]) d3 = st[(nf01 + w0[t][v] + m) * pitch + pt];
-                    const R nci = c3 ? d3.y : -d3.y;   // -Im(D~3)
-                    acc[v] = T::mfma(pr, d3.x, acc[v]);
-                    acc[v] = T::mfma(pi, nci, acc[v]);
+                C2 bnext = bcur;
+                if (j + 1 < TWW) bnext = *reinterpret_cast<const C2 *>(rows + ob[j + 1] + so);
+                if (newa) {
+                    const unsigned m2 = (fl[j] & TW_CONJ2) ? 0x80000000u : 0u;
+                    const unsigned m3 = (fl[j] & TW_CONJ3) ? 0u : 0x80000000u;
+                    const R e2 = flip(d2.y, m2);
+                    pr = d1.x * d2.x - d1.y * e2;
+                    ai = flip(d1.x * e2 + d1.y * d2.x, m3);
                 }
+                acc[j] = T::mfma(pr, bcur.x, acc[j]);
+                acc[j] = T::mfma(ai, bcur.y, acc[j]);
+                bcur = bnext;
             }
+        }
+        if (!T::kFP64Acc && (++since == kFlush || c + 1 == ce)) {
+            since = 0;
 #pragma unroll
-            for (int v = 0; v < kMaxWin; ++v)
+            for (int j = 0; j < TWW; ++j) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) accd[t][v][j] += (double)acc[v][j];
+                for (int q = 0; q < 4; ++q) accd[j][q] += (double)acc[j][q];
+                acc[j] = Acc{0, 0, 0, 0};
+            }
         }
     }
 
     double *o = out + (long long)split * split_stride;
 #pragma unroll
-    for (int t = 0; t < kMTW; ++t) {
-        if (!has[t]) continue;
+    for (int j = 0; j < TWW; ++j) {
+        const int ot_idx = TW[j].out;
+        if (ot_idx >= 0) {
+            double *ot = o + (long long)ot_idx * 256;
 #pragma unroll
-        for (int v = 0; v < kMaxWin; ++v) {
-            if (wn[t][v] == 0) continue;
-            double *ot = o + (long long)(tw[t] + v) * 256;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) ot[T::row(lane, j) * 16 + m] = accd[t][v][j] * scale;
+            for (int q = 0; q < 4; ++q) {
+                const double v = T::kFP64Acc ? (double)acc[j][q] : accd[j][q];
+                ot[T::row(lane, q) * 16 + m] = v * scale;
+            }
         }
     }
 }
 
 // ------------------------------------------------------------- BF16 x 3 --
-__device__ __forceinline__ void split8(const float *x, bf16x8 &hi, bf16x8 &lo)
-{
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        const __bf16 h = (__bf16)x[e];
-        hi[e] = h;
-        lo[e] = (__bf16)(x[e] - (float)h);
-    }
-}
-
 __device__ __forceinline__ f32x4 mfma_bf16(const bf16x8 &a, const bf16x8 &b, f32x4 c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-__global__ __launch_bounds__(kThreads) void k_contract_bf16x3(
-    const float2 *__restrict__ cache, long long npts, const WorkItem *__restrict__ items,
-    int nitems, const MTile *__restrict__ mtiles, int nsplit, int chunk, int per,
+// A staged ring of 32 points: raw rows hold float2 per point; split rows
+// hold, per group of 8 points, [Re hi x8][Re lo x8][Im hi x8][Im lo x8] bf16.
+template <int NW>
+__global__ __launch_bounds__(64 * NW, 8 / NW) void k_contract_bf16x3(
+    const float2 *__restrict__ cache, const __bf16 *__restrict__ splitc, long long npts,
+    const WorkItem *__restrict__ items, int nitems, const TWin *__restrict__ tws, int nsplit,
     double *__restrict__ out, long long split_stride, double scale)
 {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    float2 *st = reinterpret_cast<float2 *>(lds_raw);
+    constexpr int TWW = kTWW32;
+    extern __shared__ __align__(16) unsigned char lds[];
+    int *row_slot = reinterpret_cast<int *>(lds);
+    int *row_split = row_slot + kRowCap;
+    unsigned char *rows = lds + kTableBytes;
 
-    const int logical = xcd_remap(blockIdx.x, per);
-    if (logical >= nitems * nsplit) return;
-    const int item_id = logical / nsplit, split = logical - item_id * nsplit;
-    const WorkItem it = items[item_id];
-    const int nf0 = it.nf[0], nf01 = it.nf[0] + it.nf[1];
-    const int nft = nf01 + it.nf[2];
-    const int pitch = pitch_f32(chunk);
-    const long long nchunks = (npts + chunk - 1) / chunk;
+    const int item_id = blockIdx.x / nsplit, split = blockIdx.x - item_id * nsplit;
+    if (item_id >= nitems) return;
+    const WorkItem &it = items[item_id];
+    const int nrows = uni(it.nrows);
+    row_tables<64 * NW>(it, row_slot, row_split);
+    const long long nchunks = (npts + 31) / 32;
     const long long cb = nchunks * split / nsplit, ce = nchunks * (split + 1) / nsplit;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = uni(threadIdx.x >> 6);
     const int m = lane & 15, kq = lane >> 4;
-    const bool c2 = it.conj2 != 0, c3 = it.conj3 != 0;
 
-    bool has[kMTW];
-    int a1[kMTW], a2[kMTW], w0[kMTW][kMaxWin], wn[kMTW][kMaxWin], tw[kMTW];
+    const TWin *TW = tws + it.tw0 + wave * kTWW;
+    int oa1[TWW], oa2[TWW], ob[TWW], fl[TWW];
 #pragma unroll
-    for (int t = 0; t < kMTW; ++t) {
-        const int idx = wave + 4 * t;
-        has[t] = idx < it.n_mt;
-        a1[t] = -1;
-        a2[t] = 0;
-        tw[t] = 0;
-#pragma unroll
-        for (int v = 0; v < kMaxWin; ++v) { w0[t][v] = 0; wn[t][v] = 0; }
-        if (has[t]) {
-            const MTile &M = mtiles[it.mt0 + idx];
-            a1[t] = M.a1[m];
-            a2[t] = M.a2[m];
-            tw[t] = M.tw;
-#pragma unroll
-            for (int v = 0; v < kMaxWin; ++v) { w0[t][v] = M.w0[v]; wn[t][v] = M.wn[v]; }
-        }
+    for (int j = 0; j < TWW; ++j) {
+        oa1[j] = max((int)TW[j].a1[m], 0) * kPitch + kq * 64;
+        oa2[j] = (int)TW[j].a2[m] * kPitch + kq * 64;
+        ob[j] = min((int)TW[j].b0 + m, nrows - 1) * kPitch + kq * 64;
+        fl[j] = uni(TW[j].flags);
     }
 
-    double accd[kMTW][kMaxWin][4];
+    f32x4 acc[TWW];
+    double accd[TWW][4];
 #pragma unroll
-    for (int t = 0; t < kMTW; ++t)
+    for (int j = 0; j < TWW; ++j) {
+        acc[j] = f32x4{0, 0, 0, 0};
 #pragma unroll
-        for (int v = 0; v < kMaxWin; ++v)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) accd[t][v][j] = 0.0;
+        for (int q = 0; q < 4; ++q) accd[j][q] = 0.0;
+    }
+    Stager<64 * NW> S;
+    S.cache = reinterpret_cast<const char *>(cache);
+    S.split = reinterpret_cast<const char *>(splitc);
+    S.field_bytes = npts * 8;
+    S.nunits = nrows * (kRowBytes / 8);
+    S.nrows = nrows;
+    __syncthreads();
 
+    if (cb < ce) S.load(row_slot, row_split, cb * kRowBytes);
+    int since = 0;
     for (long long c = cb; c < ce; ++c) {
-        const long long x0 = c * chunk;
         __syncthreads();
-        stage_chunk<float2>(st, pitch, cache, npts, it, nft, x0, chunk);
+        S.store(rows);
         __syncthreads();
+        if (c + 1 < ce) S.load(row_slot, row_split, (c + 1) * kRowBytes);
+        bf16x8 arh = {}, arl = {}, aih = {}, ail = {};
 #pragma unroll
-        for (int t = 0; t < kMTW; ++t) {
-            if (!has[t]) continue;
-            f32x4 acc[kMaxWin];
+        for (int j = 0; j < TWW; ++j) {
+            // B first, so its latency overlaps the pair products.
+            const unsigned char *bp = rows + ob[j];
+            const bf16x8 brh = *reinterpret_cast<const bf16x8 *>(bp);
+            const bf16x8 brl = *reinterpret_cast<const bf16x8 *>(bp + 16);
+            const bf16x8 bih = *reinterpret_cast<const bf16x8 *>(bp + 32);
+            const bf16x8 bil = *reinterpret_cast<const bf16x8 *>(bp + 48);
+            if (fl[j] & TW_NEWA) {
+                const float4 *r1p = reinterpret_cast<const float4 *>(rows + oa1[j]);
+                const float4 *r2p = reinterpret_cast<const float4 *>(rows + oa2[j]);
+                const unsigned m2 = (fl[j] & TW_CONJ2) ? 0x80000000u : 0u;
+                const unsigned m3 = (fl[j] & TW_CONJ3) ? 0u : 0x80000000u;
+                f32x8 pr, ai;
 #pragma unroll
-            for (int v = 0; v < kMaxWin; ++v) acc[v] = f32x4{0, 0, 0, 0};
-            const float2 *r1p = st + (a1[t] < 0 ? 0 : a1[t]) * pitch;
-            const float2 *r2p = st + (nf0 + a2[t]) * pitch;
-            const float live = a1[t] < 0 ? 0.f : 1.f;
-            for (int s = 0; s < chunk; s += 32) {
-                const int p0 = s + 8 * kq;   // this lane's 8 points
-                float pr[8], pi[8];
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const float2 d1 = r1p[p0 + e], d2 = r2p[p0 + e];
-                    if (c2) {
-                        pr[e] = d1.x * d2.x + d1.y * d2.y;
-                        pi[e] = d1.y * d2.x - d1.x * d2.y;
-                    } else {
-                        pr[e] = d1.x * d2.x - d1.y * d2.y;
-                        pi[e] = d1.x * d2.y + d1.y * d2.x;
-                    }
-                    pr[e] *= live;
-                    pi[e] *= live;
+                for (int h = 0; h < 4; ++h) {   // two points per float4
+                    const float4 a = r1p[h], b = r2p[h];
+                    const float e0 = flip(b.y, m2), e1 = flip(b.w, m2);
+                    pr[2 * h] = a.x * b.x - a.y * e0;
+                    pr[2 * h + 1] = a.z * b.z - a.w * e1;
+                    ai[2 * h] = flip(a.x * e0 + a.y * b.x, m3);
+                    ai[2 * h + 1] = flip(a.z * e1 + a.w * b.z, m3);
                 }
-                bf16x8 arh, arl, aih, ail;
-                split8(pr, arh, arl);
-                split8(pi, aih, ail);
-#pragma unroll
-                for (int v = 0; v < kMaxWin; ++v) {
-                    if (wn[t][v] == 0) continue;
-                    float cr[8], nci[8];
-                    const bool ok = m < wn[t][v];
-                    const float2 *r3p = st + (nf01 + w0[t][v] + (ok ? m : 0)) * pitch;
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        float2 d3 = r3p[p0 + e];
-                        if (!ok) { d3.x = 0.f; d3.y = 0.f; }
-                        cr[e] = d3.x;
-                        nci[e] = c3 ? d3.y : -d3.y;
-                    }
-                    bf16x8 brh, brl, bih, bil;
-                    split8(cr, brh, brl);
-                    split8(nci, bih, bil);
-                    acc[v] = mfma_bf16(arh, brh, acc[v]);
-                    acc[v] = mfma_bf16(arh, brl, acc[v]);
-                    acc[v] = mfma_bf16(arl, brh, acc[v]);
-                    acc[v] = mfma_bf16(aih, bih, acc[v]);
-                    acc[v] = mfma_bf16(aih, bil, acc[v]);
-                    acc[v] = mfma_bf16(ail, bih, acc[v]);
-                }
+                arh = __builtin_convertvector(pr, bf16x8);
+                arl = __builtin_convertvector(pr - __builtin_convertvector(arh, f32x8), bf16x8);
+                aih = __builtin_convertvector(ai, bf16x8);
+                ail = __builtin_convertvector(ai - __builtin_convertvector(aih, f32x8), bf16x8);
             }
+            acc[j] = mfma_bf16(arh, brh, acc[j]);
+            acc[j] = mfma_bf16(aih, bih, acc[j]);
+            acc[j] = mfma_bf16(arh, brl, acc[j]);
+            acc[j] = mfma_bf16(aih, bil, acc[j]);
+            acc[j] = mfma_bf16(arl, brh, acc[j]);
+            acc[j] = mfma_bf16(ail, bih, acc[j]);
+        }
+        if (++since == kFlush || c + 1 == ce) {
+            since = 0;
 #pragma unroll
-            for (int v = 0; v < kMaxWin; ++v)
+            for (int j = 0; j < TWW; ++j) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) accd[t][v][j] += (double)acc[v][j];
+                for (int q = 0; q < 4; ++q) accd[j][q] += (double)acc[j][q];
+                acc[j] = f32x4{0, 0, 0, 0};
+            }
         }
     }
 
     double *o = out + (long long)split * split_stride;
 #pragma unroll
-    for (int t = 0; t < kMTW; ++t) {
-        if (!has[t]) continue;
+    for (int j = 0; j < TWW; ++j) {
+        const int ot_idx = TW[j].out;
+        if (ot_idx >= 0) {
+            double *ot = o + (long long)ot_idx * 256;
 #pragma unroll
-        for (int v = 0; v < kMaxWin; ++v) {
-            if (wn[t][v] == 0) continue;
-            double *ot = o + (long long)(tw[t] + v) * 256;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) ot[(4 * kq + j) * 16 + m] = accd[t][v][j] * scale;
+            for (int q = 0; q < 4; ++q) ot[(4 * kq + q) * 16 + m] = accd[j][q] * scale;
         }
     }
 }
 
 // ----------------------------------------------------------------- host --
-static size_t elem_bytes(CPrec p) { return p == CPrec::F64 ? sizeof(double2) : sizeof(float2); }
+int contract_chunk(CPrec p) { return p == CPrec::F64 ? kRowBytes / 16 : kRowBytes / 8; }
 
-static int pitch_of(CPrec p, int chunk)
+template <int NW>
+static hipError_t launch_nw(CPrec p, int tww, const void *cache, const void *split,
+                            long long npts, const WorkItem *items, int nitems, const TWin *tws,
+                            int nsplit, int max_rows, double *out, long long split_stride,
+                            double scale, hipStream_t s)
 {
-    return p == CPrec::F64 ? pitch_k4_f64(chunk) : pitch_f32(chunk);
-}
-
-static size_t lds_bytes(CPrec p, int chunk, int max_fields)
-{
-    return (size_t)max_fields * pitch_of(p, chunk) * elem_bytes(p);
-}
-
-constexpr size_t kLdsBudget = 64 * 1024;
-
-int contract_chunk(CPrec p, int max_fields)
-{
-    const int step = (p == CPrec::BF16X3) ? 32 : 4;
-    int chunk = (p == CPrec::F64) ? 32 : 64;
-    while (chunk > step && lds_bytes(p, chunk, max_fields) > kLdsBudget) chunk /= 2;
-    return chunk;
-}
-
-hipError_t launch_contract(CPrec p, const void *cache, long long npts, const WorkItem *items,
-                           int nitems, const MTile *mtiles, int nsplit, int chunk, int max_fields,
-                           double *out, long long split_stride, double scale, hipStream_t s)
-{
-    if (nitems <= 0) return hipSuccess;
-    const long long total = (long long)nitems * nsplit;
-    const int per = (int)((total + 7) / 8);
-    dim3 grid(8 * per);
-    const size_t lds = lds_bytes(p, chunk, max_fields);
-    if (p == CPrec::F64) {
-        (void)hipFuncSetAttribute((const void *)k_contract_k4<double>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(k_contract_k4<double>, grid, dim3(kThreads), lds, s,
-                           (const double2 *)cache, npts, items, nitems, mtiles, nsplit, chunk,
-                           per, out, split_stride, scale);
+    dim3 grid((unsigned)((long long)nitems * nsplit));
+    (void)max_rows;
+    const size_t lds = kTableBytes + (size_t)(kRowCap + 1) * kPitch;   // + spare ring
+    if (p == CPrec::F64 && tww <= kTWWF64Narrow) {
+        auto k = k_contract_k4<double, NW, kTWWF64Narrow, 16 / NW>;
+        (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+        hipLaunchKernelGGL(k, grid, dim3(64 * NW), lds, s, (const double2 *)cache, npts, items,
+                           nitems, tws, nsplit, out, split_stride, scale);
+    } else if (p == CPrec::F64) {
+        auto k = k_contract_k4<double, NW, kTWW, 8 / NW>;
+        (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+        hipLaunchKernelGGL(k, grid, dim3(64 * NW), lds, s, (const double2 *)cache, npts, items,
+                           nitems, tws, nsplit, out, split_stride, scale);
     } else if (p == CPrec::F32) {
-        (void)hipFuncSetAttribute((const void *)k_contract_k4<float>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(k_contract_k4<float>, grid, dim3(kThreads), lds, s,
-                           (const float2 *)cache, npts, items, nitems, mtiles, nsplit, chunk,
-                           per, out, split_stride, scale);
+        auto k = k_contract_k4<float, NW, kTWWF32, 8 / NW>;
+        (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+        hipLaunchKernelGGL(k, grid, dim3(64 * NW), lds, s, (const float2 *)cache, npts, items,
+                           nitems, tws, nsplit, out, split_stride, scale);
     } else {
-        (void)hipFuncSetAttribute((const void *)k_contract_bf16x3,
+        (void)hipFuncSetAttribute((const void *)k_contract_bf16x3<NW>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(k_contract_bf16x3, grid, dim3(kThreads), lds, s,
-                           (const float2 *)cache, npts, items, nitems, mtiles, nsplit, chunk, per,
-                           out, split_stride, scale);
+        hipLaunchKernelGGL((k_contract_bf16x3<NW>), grid, dim3(64 * NW), lds, s,
+                           (const float2 *)cache, (const __bf16 *)split, npts, items, nitems,
+                           tws, nsplit, out, split_stride, scale);
     }
     return hipGetLastError();
+}
+
+hipError_t launch_contract(CPrec p, int waves, int tww, const void *cache, const void *split,
+                           long long npts, const WorkItem *items, int nitems, const TWin *tws,
+                           int nsplit, int max_rows, double *out, long long split_stride,
+                           double scale, hipStream_t s)
+{
+    if (nitems <= 0) return hipSuccess;
+    if (waves == 4)
+        return launch_nw<4>(p, tww, cache, split, npts, items, nitems, tws, nsplit, max_rows,
+                            out, split_stride, scale, s);
+    return launch_nw<8>(p, tww, cache, split, npts, items, nitems, tws, nsplit, max_rows, out,
+                        split_stride, scale, s);
 }
 
 }  // namespace bb

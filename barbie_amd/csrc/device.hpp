@@ -29,31 +29,44 @@ struct FieldTask {
 //
 // A plane group (p1, p2, p3 = -p1-p2) contributes contractions
 //   S = sum_x Re[ D_{p1,r1}(x) D~_{p2,r2}(x) D~_{p3,r3}(x) ].
-// Its (r1, r2) pairs are packed 16 to an M-tile; the r3 rings an M-tile needs
-// are covered by up to kMaxWin windows of 16 consecutive rings.  One
-// (M-tile, window) is one 16x16 output tile, evaluated as a GEMM with
-// K = points: rows P_m(x) = D1 D~2, columns D~3.
+// Its (r1, r2) pairs are packed 16 to a pair block; the r3 rings a pair block
+// needs are covered by windows of 16 consecutive rings.  One (pair block,
+// window) is one 16x16 output tile ("tile window"), a GEMM with K = points:
+// rows P_m(x) = D1 D~2, columns the window's r3 fields.
 //
-// A work item is one group's run of up to kMaxMT M-tiles; a workgroup stages
-// the item's fields (three contiguous ring blocks, one per plane) through LDS
-// one x-chunk at a time.
-constexpr int kMaxWin = 3;
-constexpr int kMaxMT = 8;       // 4 waves x 2 M-tiles
+// A work item gathers the tile windows of several plane groups that share
+// their planes (groups with the same set {p1, |p2|, |p3|}).  A workgroup of
+// kWaves waves stages the item's rings through LDS one chunk of points at a
+// time; each wave owns up to kTWW tile windows and keeps their sums in
+// registers across all chunks of its x-range.
+constexpr int kWaves = 8;       // most waves of a contraction workgroup (4 or 8 are used)
+constexpr int kTWW = 12;        // tile windows per wave (slots reserved per wave)
+constexpr int kTWW32 = 8;       // tile windows per wave, BF16x3 form
+constexpr int kTWWF32 = 6;      // tile windows per wave, FP32 form
+constexpr int kTWWF64Narrow = 6;  // FP64 form sized for two workgroups per CU
+constexpr int kMaxBlk = 12;     // ring blocks per item
+constexpr int kRowCap = 240;    // staged rings per item (8 waves; 30 per wave)
+inline int rowcap_for(int waves) { return 30 * waves; }
 
-struct MTile {
-    int16_t a1[16];      // row m: local ring index in block 0 (-1: empty row)
-    int16_t a2[16];      // row m: local ring index in block 1
-    int16_t w0[kMaxWin]; // window v: first local ring index in block 2
-    int16_t wn[kMaxWin]; // window v: valid columns (0: no window)
-    int32_t tw;          // output tile index of window 0 (windows follow)
+enum : int16_t { TW_CONJ2 = 1, TW_CONJ3 = 2, TW_NEWA = 4 };
+
+struct TWin {
+    int16_t a1[16];   // row m: LDS row of D1 (-1: empty row)
+    int16_t a2[16];   // row m: LDS row of D2
+    int16_t b0;       // LDS row of the window's first r3 ring
+    int16_t nb;       // valid columns
+    int16_t flags;    // TW_*
+    int16_t pad;
+    int32_t out;      // output tile index
 };
 
 struct WorkItem {
-    int32_t fbase[3];    // cache slot of the first ring of each block
-    int16_t nf[3];       // rings per block
-    int16_t conj2, conj3;
-    int16_t n_mt;        // M-tiles in this item (<= kMaxMT)
-    int32_t mt0;         // first M-tile
+    int32_t blk_slot[kMaxBlk];   // cache slot of the block's first ring
+    int16_t blk_rows[kMaxBlk];   // rings in the block (consecutive slots)
+    int16_t blk_split[kMaxBlk];  // 1: staged as bf16 hi/lo B operands
+    int16_t nblk, nrows;
+    int32_t tw0;                 // first of kWaves * kTWW reserved TWins
+    int8_t cnt[kWaves];          // TWins per wave
 };
 
 // Contraction record used by the binning kernel (bin-sorted order).
@@ -83,17 +96,28 @@ hipError_t launch_rowpass(const double2 *planes, double2 *rowbuf, long long buf_
 hipError_t launch_colpass_f64(const double2 *rowbuf, long long buf_stride, double2 *cache,
                               const FieldTask *tasks, int ntasks, const FftDesc &fd,
                               const double2 *tw, hipStream_t s);
+// split (may be null): also write bf16 hi/lo operands, 8 bytes per point.
 hipError_t launch_colpass_f32(const double2 *rowbuf, long long buf_stride, float2 *cache,
-                              const FieldTask *tasks, int ntasks, const FftDesc &fd,
+                              void *split, const FieldTask *tasks, int ntasks, const FftDesc &fd,
                               const double2 *tw, hipStream_t s);
 
 // Contraction arithmetic.
 enum class CPrec { F64 = 0, F32 = 1, BF16X3 = 2 };
-// Points per LDS chunk for a given precision and maximum staged fields.
-int contract_chunk(CPrec p, int max_fields);
-hipError_t launch_contract(CPrec p, const void *cache, long long npts, const WorkItem *items,
-                           int nitems, const MTile *mtiles, int nsplit, int chunk, int max_fields,
-                           double *out, long long split_stride, double scale, hipStream_t s);
+// Points per LDS chunk of each precision.
+int contract_chunk(CPrec p);
+// Tile windows per wave of each precision.
+inline int contract_tww(CPrec p)
+{
+    return p == CPrec::F64 ? kTWW : p == CPrec::F32 ? kTWWF32 : kTWW32;
+}
+// cache: field values (FP64 or FP32); split: bf16 hi/lo operands of the same
+// fields (BF16x3 only, rows flagged blk_split are read from it).
+// waves: 4 or 8 waves per workgroup; tww: tile windows per wave (the work
+// lists must be packed for both).
+hipError_t launch_contract(CPrec p, int waves, int tww, const void *cache, const void *split,
+                           long long npts, const WorkItem *items, int nitems, const TWin *tws,
+                           int nsplit, int max_rows, double *out, long long split_stride,
+                           double scale, hipStream_t s);
 
 hipError_t launch_binning(const int32_t *seg, const BinRec *recs, const double *S,
                           const double *num, const int *nzflag, const double *avg,

@@ -6,7 +6,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <array>
 #include <map>
 #include <new>
 #include <string>
@@ -82,29 +84,26 @@ int dev_alloc(T **out, size_t count, std::vector<void *> &owned)
 // Work lists.  One contraction of a plane group: rings and its index.
 struct CItem { int r1, r2, r3, cid; };
 
-struct WorkLists {
-    std::vector<WorkItem> items;
-    std::vector<MTile> mtiles;
-    int ntw = 0;            // 16x16 output tiles
-    int max_fields = 0;     // largest staged field count of an item
-    long long tile_slots = 0;  // ntw * 256
+// A pair block: up to 16 (r1, r2) rows of one plane group whose r3 rings
+// span at most kWinRings, covered by nwin windows starting at lo.
+constexpr int kWinRings = 16 * 3;
+struct PairBlock {
+    int pa, pb, pc;           // planes of D1, D2, D3 (p1, |p2|, |p3|)
+    int flags;                // TW_CONJ2 | TW_CONJ3
+    int r1[16], r2[16];
+    int nrow;
+    int lo, hi, nwin;
+    std::vector<std::array<int, 3>> cs;   // (row, r3, cid)
 };
 
-// slot_b(r): cache slot of ring r in block b (consecutive rings, consecutive
-// slots).  pos[cid] receives the output position of every contraction.
-template <typename S1, typename S2, typename S3>
-void build_group(std::vector<CItem> &cs, int conj2, int conj3, S1 slot1, S2 slot2, S3 slot3,
-                 WorkLists &W, std::vector<int32_t> &pos)
+void pair_blocks(std::vector<CItem> &cs, int pa, int pb, int pc, int flags,
+                 std::vector<PairBlock> &out)
 {
-    constexpr int kWinRings = 16 * kMaxWin;
-    constexpr int kFieldCap = 128;
     std::sort(cs.begin(), cs.end(), [](const CItem &a, const CItem &b) {
         if (a.r1 != b.r1) return a.r1 < b.r1;
         if (a.r2 != b.r2) return a.r2 < b.r2;
         return a.r3 < b.r3;
     });
-    // Pair entries: (r1, r2) with a contiguous r3 run no wider than the
-    // windows of one M-tile.
     struct Pair { int r1, r2, lo, hi; size_t b, e; };
     std::vector<Pair> pairs;
     for (size_t k = 0; k < cs.size();) {
@@ -121,90 +120,198 @@ void build_group(std::vector<CItem> &cs, int conj2, int conj3, S1 slot1, S2 slot
         if (a.r1 != b.r1) return a.r1 < b.r1;
         return a.r2 < b.r2;
     });
-    // M-tiles: up to 16 pairs whose joint r3 range fits the windows.
-    struct MT { size_t pb, pe; int lo, hi; };
-    std::vector<MT> mts;
     for (size_t q = 0; q < pairs.size();) {
-        MT t{q, q, pairs[q].lo, pairs[q].hi};
-        while (t.pe < pairs.size() && t.pe - t.pb < 16) {
-            int lo = std::min(t.lo, pairs[t.pe].lo), hi = std::max(t.hi, pairs[t.pe].hi);
-            if (hi - lo + 1 > kWinRings) break;
-            t.lo = lo;
-            t.hi = hi;
-            t.pe++;
+        size_t e = q + 1;
+        int lo = pairs[q].lo, hi = pairs[q].hi;
+        while (e < pairs.size() && e - q < 16) {
+            const int l2 = std::min(lo, pairs[e].lo), h2 = std::max(hi, pairs[e].hi);
+            if (h2 - l2 + 1 > kWinRings) break;
+            lo = l2;
+            hi = h2;
+            ++e;
         }
-        mts.push_back(t);
-        q = t.pe;
+        PairBlock B;
+        B.pa = pa; B.pb = pb; B.pc = pc; B.flags = flags;
+        B.nrow = (int)(e - q);
+        B.lo = lo; B.hi = hi; B.nwin = (hi - lo) / 16 + 1;
+        for (int m = 0; m < 16; ++m) { B.r1[m] = -1; B.r2[m] = -1; }
+        for (size_t u = q; u < e; ++u) {
+            const int m = (int)(u - q);
+            B.r1[m] = pairs[u].r1;
+            B.r2[m] = pairs[u].r2;
+            for (size_t k = pairs[u].b; k < pairs[u].e; ++k) B.cs.push_back({m, cs[k].r3, cs[k].cid});
+        }
+        out.push_back(std::move(B));
+        q = e;
     }
-    // Work items: runs of up to kMaxMT M-tiles whose staged rings fit.
-    for (size_t a = 0; a < mts.size();) {
+}
+
+struct WorkLists {
+    std::vector<WorkItem> items;
+    std::vector<TWin> tws;
+    int ntw = 0;               // output tiles
+    int max_rows = 0;          // largest staged ring count of an item
+    long long tile_slots = 0;  // ntw * 256
+    long long tw_used = 0, tw_slots = 0;   // wave balance: sum cnt, sum waves * max cnt
+};
+
+// Packs pair blocks (already grouped so that neighbours share planes) into
+// work items.  slot(plane, ring) gives the cache slot; split_b: stage the r3
+// rings as bf16 hi/lo operands (BF16x3) instead of raw values.
+template <typename Slot>
+int pack_items(std::vector<PairBlock> &blocks, bool split_b, int tww, int waves, Slot slot,
+               WorkLists &W, std::vector<int32_t> &pos)
+{
+    const int rowcap = rowcap_for(waves);
+    struct Key { int plane, split; bool operator<(const Key &o) const { return plane != o.plane ? plane < o.plane : split < o.split; } };
+    struct Range { int lo, hi; };
+    size_t a = 0;
+    std::vector<std::pair<long long, size_t>> order;   // (work, item index) for LPT ordering
+    while (a < blocks.size()) {
+        std::map<Key, Range> keys;
+        int load[kWaves] = {0};
+        std::vector<std::vector<size_t>> wave_blocks(kWaves);
         size_t b = a;
-        int r1lo = 1 << 30, r1hi = -1, r2lo = 1 << 30, r2hi = -1, r3lo = 1 << 30, r3hi = -1;
-        while (b < mts.size() && b - a < (size_t)kMaxMT) {
-            int n1lo = r1lo, n1hi = r1hi, n2lo = r2lo, n2hi = r2hi;
-            for (size_t q = mts[b].pb; q < mts[b].pe; ++q) {
-                n1lo = std::min(n1lo, pairs[q].r1); n1hi = std::max(n1hi, pairs[q].r1);
-                n2lo = std::min(n2lo, pairs[q].r2); n2hi = std::max(n2hi, pairs[q].r2);
+        for (; b < blocks.size(); ++b) {
+            const PairBlock &B = blocks[b];
+            std::map<Key, Range> k2 = keys;
+            auto add = [&](Key k, int lo, int hi) {
+                auto it = k2.find(k);
+                if (it == k2.end()) k2[k] = {lo, hi};
+                else { it->second.lo = std::min(it->second.lo, lo); it->second.hi = std::max(it->second.hi, hi); }
+            };
+            for (int m = 0; m < B.nrow; ++m) {
+                add({B.pa, 0}, B.r1[m], B.r1[m]);
+                add({B.pb, 0}, B.r2[m], B.r2[m]);
             }
-            int n3lo = std::min(r3lo, mts[b].lo), n3hi = std::max(r3hi, mts[b].hi);
-            int nf = (n1hi - n1lo + 1) + (n2hi - n2lo + 1) + (n3hi - n3lo + 1);
-            if (b > a && nf > kFieldCap) break;
-            r1lo = n1lo; r1hi = n1hi; r2lo = n2lo; r2hi = n2hi; r3lo = n3lo; r3hi = n3hi;
-            ++b;
+            add({B.pc, split_b ? 1 : 0}, B.lo, B.hi);
+            int rows = 0;
+            for (auto &kv : k2) rows += kv.second.hi - kv.second.lo + 1;
+            int w = 0;
+            for (int v = 1; v < waves; ++v)
+                if (load[v] < load[w]) w = v;
+            const bool fits = rows <= rowcap && (int)k2.size() <= kMaxBlk && load[w] + B.nwin <= tww;
+            if (!fits && b == a)
+                return set_error(BB_E_INVALID, "a pair block needs " + std::to_string(rows) +
+                                                   " staged rings; the contraction kernel holds " +
+                                                   std::to_string(rowcap) + " (too many rings)");
+            if (!fits) break;
+            keys.swap(k2);
+            load[w] += B.nwin;
+            wave_blocks[w].push_back(b);
         }
+        // Lay out the item: blocks in key order.
         WorkItem it;
         std::memset(&it, 0, sizeof it);
-        it.fbase[0] = slot1(r1lo);
-        it.fbase[1] = slot2(r2lo);
-        it.fbase[2] = slot3(r3lo);
-        it.nf[0] = (int16_t)(r1hi - r1lo + 1);
-        it.nf[1] = (int16_t)(r2hi - r2lo + 1);
-        it.nf[2] = (int16_t)(r3hi - r3lo + 1);
-        it.conj2 = (int16_t)conj2;
-        it.conj3 = (int16_t)conj3;
-        it.n_mt = (int16_t)(b - a);
-        it.mt0 = (int32_t)W.mtiles.size();
-        W.max_fields = std::max(W.max_fields, it.nf[0] + it.nf[1] + it.nf[2]);
-        for (size_t u = a; u < b; ++u) {
-            const MT &t = mts[u];
-            MTile M;
-            std::memset(&M, 0, sizeof M);
-            for (int m = 0; m < 16; ++m) { M.a1[m] = -1; M.a2[m] = 0; }
-            int nwin = (t.hi - t.lo) / 16 + 1;
-            M.tw = W.ntw;
-            for (int v = 0; v < kMaxWin; ++v) {
-                if (v < nwin) {
-                    const int ws = t.lo + 16 * v;
-                    M.w0[v] = (int16_t)(ws - r3lo);
-                    M.wn[v] = (int16_t)std::min(16, t.hi - ws + 1);
-                } else {
-                    M.w0[v] = 0;
-                    M.wn[v] = 0;
-                }
-            }
-            for (size_t q = t.pb; q < t.pe; ++q) {
-                const int m = (int)(q - t.pb);
-                M.a1[m] = (int16_t)(pairs[q].r1 - r1lo);
-                M.a2[m] = (int16_t)(pairs[q].r2 - r2lo);
-                for (size_t k = pairs[q].b; k < pairs[q].e; ++k) {
-                    const int d = cs[k].r3 - t.lo;
-                    const int v = d / 16, col = d % 16;
-                    pos[cs[k].cid] = (M.tw + v) * 256 + m * 16 + col;
-                }
-            }
-            W.ntw += nwin;
-            W.mtiles.push_back(M);
+        std::map<Key, int> row0;
+        int nrows = 0;
+        for (auto &kv : keys) {
+            const int bi = it.nblk++;
+            it.blk_slot[bi] = slot(kv.first.plane, kv.second.lo);
+            it.blk_rows[bi] = (int16_t)(kv.second.hi - kv.second.lo + 1);
+            it.blk_split[bi] = (int16_t)kv.first.split;
+            row0[kv.first] = nrows - kv.second.lo;   // row of ring r = row0 + r
+            nrows += it.blk_rows[bi];
         }
+        it.nrows = (int16_t)nrows;
+        it.tw0 = (int32_t)W.tws.size();
+        W.max_rows = std::max(W.max_rows, nrows);
+        {
+            TWin dummy;   // computes into an accumulator that is never written out
+            std::memset(&dummy, 0, sizeof dummy);
+            for (int m = 0; m < 16; ++m) dummy.a1[m] = -1;
+            dummy.flags = TW_NEWA;
+            dummy.out = -1;
+            W.tws.resize(W.tws.size() + kWaves * kTWW, dummy);
+        }
+        long long work = 0;
+        for (int w = 0; w < waves; ++w) {
+            int j = 0;
+            for (size_t bi : wave_blocks[w]) {
+                const PairBlock &B = blocks[bi];
+                const int ra = row0[{B.pa, 0}], rb = row0[{B.pb, 0}];
+                const int rc = row0[{B.pc, split_b ? 1 : 0}];
+                const int tile0 = W.ntw;
+                for (int v = 0; v < B.nwin; ++v) {
+                    TWin &T = W.tws[it.tw0 + w * kTWW + j];
+                    std::memset(&T, 0, sizeof T);
+                    for (int m = 0; m < 16; ++m) {
+                        T.a1[m] = (int16_t)(B.r1[m] >= 0 ? ra + B.r1[m] : -1);
+                        T.a2[m] = (int16_t)(B.r2[m] >= 0 ? rb + B.r2[m] : 0);
+                    }
+                    const int ws = B.lo + 16 * v;
+                    T.b0 = (int16_t)(rc + ws);
+                    T.nb = (int16_t)std::min(16, B.hi - ws + 1);
+                    T.flags = (int16_t)(B.flags | (v == 0 ? TW_NEWA : 0));
+                    T.out = W.ntw++;
+                    ++j;
+                }
+                for (const auto &c : B.cs) {
+                    const int d = c[1] - B.lo;
+                    pos[c[2]] = (tile0 + d / 16) * 256 + c[0] * 16 + d % 16;
+                }
+            }
+            it.cnt[w] = (int8_t)j;
+            work = std::max<long long>(work, j);
+            W.tw_used += j;
+        }
+        W.tw_slots += (long long)waves * work;
+        order.push_back({work * 1000000LL + nrows, W.items.size()});
         W.items.push_back(it);
         a = b;
     }
+    // Largest items first (longest-processing-time order).
+    std::stable_sort(order.begin(), order.end(),
+                     [](const std::pair<long long, size_t> &x, const std::pair<long long, size_t> &y) {
+                         return x.first > y.first;
+                     });
+    std::vector<WorkItem> sorted;
+    sorted.reserve(W.items.size());
+    for (auto &o : order) sorted.push_back(W.items[o.second]);
+    W.items.swap(sorted);
     W.tile_slots = (long long)W.ntw * 256;
+    return BB_OK;
+}
+
+// Checks the invariants the contraction kernels rely on (every LDS row index
+// inside the item's staged rows, every output slot used once).
+int check_lists(const WorkLists &W, const std::vector<int32_t> &pos, size_t ncontr)
+{
+    std::vector<uint8_t> seen((size_t)std::max<long long>(W.tile_slots, 1), 0);
+    for (size_t c = 0; c < ncontr; ++c) {
+        if (pos[c] < 0 || pos[c] >= W.tile_slots || seen[pos[c]])
+            return set_error(BB_E_INVALID, "internal: contraction output slots are not unique");
+        seen[pos[c]] = 1;
+    }
+    for (const WorkItem &it : W.items) {
+        int rows = 0;
+        for (int b = 0; b < it.nblk; ++b) rows += it.blk_rows[b];
+        if (rows != it.nrows || it.nrows > kRowCap || it.nblk > kMaxBlk)
+            return set_error(BB_E_INVALID, "internal: work item row blocks are inconsistent");
+        for (int w = 0; w < kWaves; ++w) {
+            if (it.cnt[w] < 0 || it.cnt[w] > kTWW)
+                return set_error(BB_E_INVALID, "internal: too many tile windows in a wave");
+            for (int j = it.cnt[w]; j < kTWW; ++j)
+                if (W.tws[(size_t)it.tw0 + w * kTWW + j].out != -1)
+                    return set_error(BB_E_INVALID, "internal: unused tile window slot is live");
+            for (int j = 0; j < it.cnt[w]; ++j) {
+                const TWin &T = W.tws[(size_t)it.tw0 + w * kTWW + j];
+                bool ok = T.b0 >= 0 && T.b0 < it.nrows && T.nb >= 1 && T.nb <= 16 &&
+                          T.out >= 0 && T.out < W.ntw && (j > 0 || (T.flags & TW_NEWA));
+                for (int m = 0; m < 16; ++m)
+                    ok = ok && T.a1[m] >= -1 && T.a1[m] < it.nrows && T.a2[m] >= 0 &&
+                         T.a2[m] < it.nrows;
+                if (!ok) return set_error(BB_E_INVALID, "internal: tile window out of range");
+            }
+        }
+    }
+    return BB_OK;
 }
 
 int auto_splits(int nitems)
 {
-    int s = (2048 + std::max(nitems, 1) - 1) / std::max(nitems, 1);
-    return std::max(1, std::min(s, 32));
+    int s = (1024 + std::max(nitems, 1) - 1) / std::max(nitems, 1);
+    return std::max(1, std::min(s, 64));
 }
 
 }  // namespace
@@ -225,14 +332,17 @@ struct bb_estimator {
     double2 *d_field = nullptr;    // staging for bb_estimate (host input)
     double2 *d_planes = nullptr;
     void *d_cache = nullptr;
+    void *d_split = nullptr;       // BF16x3: bf16 hi/lo operands of every field
     double2 *d_rowbuf = nullptr;   // FP32 caches: FP64 row-pass scratch
     int rowbuf_fields = 0;
     FieldTask *d_tasks = nullptr;
     int ntasks = 0;
     int max_rows = 0;
     WorkItem *d_items = nullptr;
-    MTile *d_mtiles = nullptr;
-    int nitems = 0, nsplit = 1, chunk = 32, max_fields = 0;
+    TWin *d_tws = nullptr;
+    int nitems = 0, nsplit = 1, chunk = 32, max_fields = 0, waves = kWaves, tww = kTWW;
+    double wave_balance = 1.0;
+    long long staged_rows = 0;
     long long tile_slots = 0;
     double *d_part = nullptr;      // [nsplit][tile_slots]
     std::vector<int32_t> pos;      // contraction -> tile position
@@ -288,8 +398,8 @@ int run_fields(bb_estimator *E, const double2 *planes, const FieldTask *d_tasks,
         int nb = std::min(E->rowbuf_fields, ntasks - t0);
         HIPCHK(launch_rowpass(planes, E->d_rowbuf, -NG2, d_tasks + t0, nb, max_rows, E->fd,
                               E->d_tw, nz, E->stream));
-        HIPCHK(launch_colpass_f32(E->d_rowbuf, -NG2, (float2 *)cache, d_tasks + t0, nb, E->fd,
-                                  E->d_tw, E->stream));
+        HIPCHK(launch_colpass_f32(E->d_rowbuf, -NG2, (float2 *)cache, E->d_split, d_tasks + t0,
+                                  nb, E->fd, E->d_tw, E->stream));
     }
     return BB_OK;
 }
@@ -362,35 +472,39 @@ int estimator_init(bb_estimator *E)
             cs.push_back({d.tri[3 * rp], d.tri[3 * rp + 1], d.tri[3 * rp + 2], rp});
         WorkLists W;
         std::vector<int32_t> upos(std::max(rcount, 1), 0);
-        auto id = [](int r) { return r; };
-        if (rcount > 0) build_group(cs, 0, 0, id, id, id, W, upos);
+        std::vector<PairBlock> blocks;
+        if (rcount > 0) pair_blocks(cs, 0, 0, 0, 0, blocks);
+        rc = pack_items(blocks, false, contract_tww(CPrec::F64), kWaves,
+                        [](int, int r) { return r; }, W, upos);
+        if (!rc) rc = check_lists(W, upos, rcount);
+        if (rc) return rc;
         const int usplit = auto_splits((int)W.items.size());
-        const int uchunk = contract_chunk(CPrec::F64, std::max(W.max_fields, 1));
+        const long long uslots = std::max<long long>(W.tile_slots, 1);
         std::vector<void *> tmp;
         FieldTask *d_ut;
         double2 *d_unit;
         WorkItem *d_uitems;
-        MTile *d_umt;
+        TWin *d_utw;
         double *d_upart;
         rc = dev_alloc(&d_ut, Nr, tmp);
         if (!rc) rc = dev_alloc(&d_unit, (size_t)Nr * E->npts, tmp);
         if (!rc) rc = dev_alloc(&d_uitems, W.items.size(), tmp);
-        if (!rc) rc = dev_alloc(&d_umt, W.mtiles.size(), tmp);
-        if (!rc) rc = dev_alloc(&d_upart, (size_t)usplit * std::max<long long>(W.tile_slots, 1), tmp);
+        if (!rc) rc = dev_alloc(&d_utw, W.tws.size(), tmp);
+        if (!rc) rc = dev_alloc(&d_upart, (size_t)usplit * uslots, tmp);
         if (!rc) rc = upload(d_ut, ut.data(), sizeof(FieldTask) * Nr, E->stream);
         if (!rc && !W.items.empty())
             rc = upload(d_uitems, W.items.data(), sizeof(WorkItem) * W.items.size(), E->stream);
-        if (!rc && !W.mtiles.empty())
-            rc = upload(d_umt, W.mtiles.data(), sizeof(MTile) * W.mtiles.size(), E->stream);
+        if (!rc && !W.tws.empty())
+            rc = upload(d_utw, W.tws.data(), sizeof(TWin) * W.tws.size(), E->stream);
         if (!rc) rc = run_fields(E, nullptr, d_ut, Nr, mr, d_unit, false, nullptr);
         if (!rc && rcount > 0) {
-            hipError_t e = launch_contract(CPrec::F64, d_unit, E->npts, d_uitems,
-                                           (int)W.items.size(), d_umt, usplit, uchunk,
-                                           W.max_fields, d_upart, W.tile_slots, E->scale,
-                                           E->stream);
+            hipError_t e = launch_contract(CPrec::F64, kWaves, contract_tww(CPrec::F64), d_unit,
+                                           nullptr, E->npts, d_uitems,
+                                           (int)W.items.size(), d_utw, usplit, W.max_rows,
+                                           d_upart, uslots, E->scale, E->stream);
             if (e != hipSuccess) rc = set_error(BB_E_DEVICE, hipGetErrorString(e));
         }
-        std::vector<double> part((size_t)usplit * std::max<long long>(W.tile_slots, 1));
+        std::vector<double> part((size_t)usplit * uslots);
         if (!rc && hipMemcpyAsync(part.data(), d_upart, sizeof(double) * part.size(),
                                   hipMemcpyDeviceToHost, E->stream) != hipSuccess)
             rc = set_error(BB_E_DEVICE, "triangle-count copy failed");
@@ -401,7 +515,7 @@ int estimator_init(bb_estimator *E)
         E->num_host.assign(std::max(rcount, 1), 0.0);
         for (int rp = 0; rp < rcount; ++rp) {
             double v = 0.0;
-            for (int s = 0; s < usplit; ++s) v += part[(size_t)s * W.tile_slots + upos[rp]];
+            for (int s = 0; s < usplit; ++s) v += part[(size_t)s * uslots + upos[rp]];
             E->num_host[rp] = v;
         }
         if ((rc = dev_alloc(&E->d_num, E->num_host.size(), E->owned))) return rc;
@@ -433,27 +547,51 @@ int estimator_init(bb_estimator *E)
     }
     E->ntasks = (int)tasks.size();
 
-    // --- Contraction work lists per plane group ------------------------------
+    // --- Contraction work lists ------------------------------------------------
+    // Pair blocks per plane group; groups with the same plane set
+    // {p1, |p2|, |p3|} are adjacent so that work items share staged rings.
     std::map<std::pair<int, int>, std::vector<CItem>> groups;
     for (size_t c = 0; c < d.contr.size(); ++c) {
         const Contraction &k = d.contr[c];
         groups[{k.p1, k.p2}].push_back({k.r1, k.r2, k.r3, (int)c});
     }
-    WorkLists W;
-    E->pos.assign(std::max<size_t>(d.contr.size(), 1), 0);
+    std::map<std::array<int, 3>, std::vector<std::pair<int, int>>> by_set;
     for (auto &kv : groups) {
         const int p1 = kv.first.first, p2 = kv.first.second, p3 = -p1 - p2;
-        const int a2 = std::abs(p2), a3 = std::abs(p3);
-        auto s1 = [&](int r) { return E->fid(p1, r); };
-        auto s2 = [&](int r) { return E->fid(a2, r); };
-        auto s3 = [&](int r) { return E->fid(a3, r); };
-        build_group(kv.second, p2 < 0, p3 < 0, s1, s2, s3, W, E->pos);
+        std::array<int, 3> key{p1, std::abs(p2), std::abs(p3)};
+        std::sort(key.begin(), key.end());
+        by_set[key].push_back(kv.first);
     }
+    std::vector<PairBlock> blocks;
+    for (auto &kv : by_set)
+        for (auto &gk : kv.second) {
+            const int p1 = gk.first, p2 = gk.second, p3 = -p1 - p2;
+            const int flags = (p2 < 0 ? TW_CONJ2 : 0) | (p3 < 0 ? TW_CONJ3 : 0);
+            pair_blocks(groups[gk], p1, std::abs(p2), std::abs(p3), flags, blocks);
+        }
+    WorkLists W;
+    E->pos.assign(std::max<size_t>(d.contr.size(), 1), 0);
+    E->waves = kWaves;
+    if (const char *ev = std::getenv("BARBIE_CONTRACT_WAVES")) {
+        const int w = std::atoi(ev);
+        if (w == 4 || w == 8) E->waves = w;
+    }
+    E->tww = contract_tww(E->cprec);
+    if (const char *ev = std::getenv("BARBIE_CONTRACT_TWW")) {
+        const int t = std::atoi(ev);
+        if (E->cprec == CPrec::F64 && (t == kTWW || t == kTWWF64Narrow)) E->tww = t;
+    }
+    rc = pack_items(blocks, E->cprec == CPrec::BF16X3, E->tww, E->waves,
+                    [&](int plane, int ring) { return E->fid(plane, ring); }, W, E->pos);
+    E->wave_balance = W.tw_slots ? (double)W.tw_used / (double)W.tw_slots : 1.0;
+    if (!rc) rc = check_lists(W, E->pos, d.contr.size());
+    if (rc) return rc;
     E->nitems = (int)W.items.size();
-    E->max_fields = std::max(W.max_fields, 1);
+    E->max_fields = std::max(W.max_rows, 1);
+    for (const WorkItem &it : W.items) E->staged_rows += it.nrows;
     E->tile_slots = std::max<long long>(W.tile_slots, 1);
     E->nsplit = E->opt.x_splits > 0 ? E->opt.x_splits : auto_splits(E->nitems);
-    E->chunk = contract_chunk(E->cprec, E->max_fields);
+    E->chunk = contract_chunk(E->cprec);
 
     // --- Binning records, sorted by bin --------------------------------------
     std::vector<BinRec> recs(d.contr.size());
@@ -492,6 +630,11 @@ int estimator_init(bb_estimator *E)
         if ((rc = dev_alloc(&c, nf * E->npts, E->owned))) return rc;
         E->d_cache = c;
         E->rowbuf_fields = std::max(1, std::min(E->ntasks, 128));
+        if (E->cprec == CPrec::BF16X3) {
+            float2 *sp;   // 8 bytes per point, like the FP32 values
+            if ((rc = dev_alloc(&sp, nf * E->npts, E->owned))) return rc;
+            E->d_split = sp;
+        }
         if ((rc = dev_alloc(&E->d_rowbuf, (size_t)E->rowbuf_fields * E->npts, E->owned))) return rc;
     } else {
         double2 *c;
@@ -500,7 +643,7 @@ int estimator_init(bb_estimator *E)
     }
     if ((rc = dev_alloc(&E->d_tasks, tasks.size(), E->owned))) return rc;
     if ((rc = dev_alloc(&E->d_items, W.items.size(), E->owned))) return rc;
-    if ((rc = dev_alloc(&E->d_mtiles, W.mtiles.size(), E->owned))) return rc;
+    if ((rc = dev_alloc(&E->d_tws, W.tws.size(), E->owned))) return rc;
     if ((rc = dev_alloc(&E->d_part, (size_t)E->nsplit * E->tile_slots, E->owned))) return rc;
     if ((rc = dev_alloc(&E->d_recs, recs.size(), E->owned))) return rc;
     if ((rc = dev_alloc(&E->d_seg, seg.size(), E->owned))) return rc;
@@ -509,7 +652,7 @@ int estimator_init(bb_estimator *E)
     if ((rc = dev_alloc(&E->d_bins, (size_t)3 * d.nbins * BB_NPOLES, E->owned))) return rc;
     if (!tasks.empty() && (rc = upload(E->d_tasks, tasks.data(), sizeof(FieldTask) * tasks.size(), E->stream))) return rc;
     if (!W.items.empty() && (rc = upload(E->d_items, W.items.data(), sizeof(WorkItem) * W.items.size(), E->stream))) return rc;
-    if (!W.mtiles.empty() && (rc = upload(E->d_mtiles, W.mtiles.data(), sizeof(MTile) * W.mtiles.size(), E->stream))) return rc;
+    if (!W.tws.empty() && (rc = upload(E->d_tws, W.tws.data(), sizeof(TWin) * W.tws.size(), E->stream))) return rc;
     if (!recs.empty() && (rc = upload(E->d_recs, recs.data(), sizeof(BinRec) * recs.size(), E->stream))) return rc;
     if ((rc = upload(E->d_seg, seg.data(), sizeof(int32_t) * seg.size(), E->stream))) return rc;
     if ((rc = upload(E->d_avg, d.avg.data(), sizeof(double) * Nr, E->stream))) return rc;
@@ -534,9 +677,8 @@ int estimate_device(bb_estimator *E, const double *d_field_in, double *d_bs, dou
     int rc = run_fields(E, E->d_planes, E->d_tasks, E->ntasks, E->max_rows, E->d_cache, f32, E->d_nz);
     if (rc) return rc;
     HIPCHK(hipEventRecord(E->ev[2], s));
-    HIPCHK(launch_contract(E->cprec, E->d_cache, E->npts, E->d_items, E->nitems, E->d_mtiles,
-                           E->nsplit, E->chunk, E->max_fields, E->d_part, E->tile_slots,
-                           E->scale, s));
+    HIPCHK(launch_contract(E->cprec, E->waves, E->tww, E->d_cache, E->d_split, E->npts, E->d_items, E->nitems, E->d_tws,
+                           E->nsplit, E->max_fields, E->d_part, E->tile_slots, E->scale, s));
     HIPCHK(hipEventRecord(E->ev[3], s));
     HIPCHK(launch_binning(E->d_seg, E->d_recs, E->d_part, E->d_num, E->d_nz, E->d_avg, E->bp,
                           d_bs, d_norm, d_biana, s));
@@ -574,6 +716,7 @@ int bb_estimator_footprint(const bb_plan *p, bb_precision prec, size_t *bytes)
     const size_t nf = cache_fields(d);
     size_t b = NN * NN * (NN / 2 + 1) * 16 + Nmax * NN2 * 16;
     b += nf * NN2 * (prec == BB_FP64 ? 16 : 8);
+    if (prec == BB_BF16X3) b += nf * NN2 * 8;   // bf16 hi/lo operand copy
     if (prec != BB_FP64) b += std::min<size_t>(nf, 128) * NN2 * 16;
     b += d.contr.size() * (sizeof(BinRec) + 8 * 3 * 2);
     b += Nr * NN2 * 16;  // transient unit fields (triangle counts)
@@ -698,6 +841,22 @@ int bb_estimator_field(bb_estimator *e, int plane, int ring, double *out)
             out[2 * k + 1] = tmp[k].y;
         }
     }
+    return BB_OK;
+}
+
+int bb_estimator_worklist(const bb_estimator *e, long long *o)
+{
+    if (!e || !o) return set_error(BB_E_INVALID, "null argument");
+    o[0] = e->nitems;
+    o[1] = e->nsplit;
+    o[2] = e->tile_slots / 256;
+    o[3] = e->max_fields;
+    o[4] = e->staged_rows;
+    o[5] = e->staged_rows * e->npts * (e->cprec == CPrec::F64 ? 16 : 8);
+    o[6] = e->NG;
+    o[7] = e->chunk;
+    o[8] = e->waves;
+    o[9] = (long long)std::llround(e->wave_balance * 1000.0);
     return BB_OK;
 }
 

@@ -229,11 +229,15 @@ __device__ __forceinline__ float2 to_out<float2>(double2 v)
 // Column pass.  grid = (ceil(N / kColsPerBlock), ntasks).  Reads the row-pass
 // output of task t (slot as in the row pass), transforms along i and writes
 // cache[fid][i][j].  Rows outside the ring's band were never written by the
-// row pass and are taken as zero.
+// row pass and are taken as zero.  With `split` set (FP32 caches), the field
+// is also written as bf16 hi/lo operands: for every 8 consecutive points,
+// 32 bf16 values [Re hi x8][Re lo x8][Im hi x8][Im lo x8], with
+// hi = bf16(v) and lo = bf16(v - hi) of the FP32 value v.
 template <typename OUT2>
 __global__ __launch_bounds__(256) void k_colpass(const double2 *__restrict__ rowbuf,
                                                  long long buf_stride, int use_fid,
                                                  OUT2 *__restrict__ cache,
+                                                 __bf16 *__restrict__ split,
                                                  const FieldTask *__restrict__ tasks, FftDesc fd,
                                                  const double2 *__restrict__ tw)
 {
@@ -256,17 +260,30 @@ __global__ __launch_bounds__(256) void k_colpass(const double2 *__restrict__ row
     }
     __syncthreads();
     double2 *res = lds_fft(a, b, kColsPerBlock, fd, tw);
-    OUT2 *dst = cache + (long long)T.fid * N * N;
+    const long long npts = (long long)N * N;
+    OUT2 *dst = cache + (long long)T.fid * npts;
     for (int idx = threadIdx.x; idx < kColsPerBlock * N; idx += blockDim.x) {
         const int i = idx / kColsPerBlock, c = idx - i * kColsPerBlock;
-        if (j0 + c < N) dst[(long long)i * N + j0 + c] = to_out<OUT2>(res[c * N + i]);
+        if (j0 + c >= N) continue;
+        const long long x = (long long)i * N + j0 + c;
+        const OUT2 o = to_out<OUT2>(res[c * N + i]);
+        dst[x] = o;
+        if (split) {
+            const float vr = (float)o.x, vi = (float)o.y;
+            const __bf16 hr = (__bf16)vr, hi = (__bf16)vi;
+            __bf16 *g = split + (long long)T.fid * npts * 4 + (x >> 3) * 32 + (x & 7);
+            g[0] = hr;
+            g[8] = (__bf16)(vr - (float)hr);
+            g[16] = hi;
+            g[24] = (__bf16)(vi - (float)hi);
+        }
     }
 }
 
 template <typename OUT2>
 static hipError_t launch_colpass(const double2 *rowbuf, long long buf_stride, OUT2 *cache,
-                                 const FieldTask *tasks, int ntasks, const FftDesc &fd,
-                                 const double2 *tw, hipStream_t s)
+                                 __bf16 *split, const FieldTask *tasks, int ntasks,
+                                 const FftDesc &fd, const double2 *tw, hipStream_t s)
 {
     int use_fid = buf_stride > 0 ? 1 : 0;
     long long stride = buf_stride > 0 ? buf_stride : -buf_stride;
@@ -275,7 +292,7 @@ static hipError_t launch_colpass(const double2 *rowbuf, long long buf_stride, OU
     (void)hipFuncSetAttribute((const void *)k_colpass<OUT2>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(k_colpass<OUT2>, grid, dim3(256), lds, s, rowbuf, stride, use_fid, cache,
-                       tasks, fd, tw);
+                       split, tasks, fd, tw);
     return hipGetLastError();
 }
 
@@ -283,14 +300,15 @@ hipError_t launch_colpass_f64(const double2 *rowbuf, long long buf_stride, doubl
                               const FieldTask *tasks, int ntasks, const FftDesc &fd,
                               const double2 *tw, hipStream_t s)
 {
-    return launch_colpass<double2>(rowbuf, buf_stride, cache, tasks, ntasks, fd, tw, s);
+    return launch_colpass<double2>(rowbuf, buf_stride, cache, nullptr, tasks, ntasks, fd, tw, s);
 }
 
 hipError_t launch_colpass_f32(const double2 *rowbuf, long long buf_stride, float2 *cache,
-                              const FieldTask *tasks, int ntasks, const FftDesc &fd,
+                              void *split, const FieldTask *tasks, int ntasks, const FftDesc &fd,
                               const double2 *tw, hipStream_t s)
 {
-    return launch_colpass<float2>(rowbuf, buf_stride, cache, tasks, ntasks, fd, tw, s);
+    return launch_colpass<float2>(rowbuf, buf_stride, cache, (__bf16 *)split, tasks, ntasks, fd,
+                                  tw, s);
 }
 
 }  // namespace bb

@@ -30,7 +30,7 @@ EXPORTED = [
     "bb_estimator_destroy", "bb_estimator_grid",
     "bb_estimator_footprint", "bb_estimate", "bb_estimate_device",
     "bb_estimator_contraction_sums", "bb_estimator_triangle_counts", "bb_estimator_field",
-    "bb_estimator_timings", "bb_write_outputs",
+    "bb_estimator_timings", "bb_estimator_worklist", "bb_write_outputs",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -101,6 +101,7 @@ def lib() -> ctypes.CDLL:
         L.bb_estimator_triangle_counts.argtypes = [ctypes.c_void_p, _dp]
         L.bb_estimator_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
         L.bb_estimator_timings.argtypes = [ctypes.c_void_p, _dp]
+        L.bb_estimator_worklist.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)]
         L.bb_write_outputs.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_char_p,
                                        ctypes.c_int]
         _lib = L
@@ -264,6 +265,13 @@ class Estimator:
         out = np.zeros((NG, NG), dtype=np.complex128)
         _check(lib().bb_estimator_field(self.handle, plane, ring, _p(out.view(np.float64))))
         return out
+
+    def worklist(self) -> dict:
+        o = (ctypes.c_longlong * 10)()
+        _check(lib().bb_estimator_worklist(self.handle, o))
+        keys = ("items", "x_splits", "tiles", "max_rings", "staged_rings", "staged_bytes",
+                "grid", "chunk", "waves", "wave_balance_x1000")
+        return dict(zip(keys, list(o)))
 
     def timings(self) -> np.ndarray:
         t = np.zeros(4)

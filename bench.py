@@ -53,6 +53,7 @@ def geometry(NN):
 def time_path(B, torch, plan, precision, d_field, steps, warmup, full_grid=False):
     est = B.Estimator(plan, precision, full_grid=full_grid)
     grid = est.grid
+    wl = est.worklist()
     nb = plan.geom.nbins * 16
     out = torch.zeros(3 * nb, dtype=torch.float64, device="cuda")
     ptr = out.data_ptr()
@@ -71,7 +72,7 @@ def time_path(B, torch, plan, precision, d_field, steps, warmup, full_grid=False
     st = np.mean(np.array(stage), axis=0)
     bins = out.cpu().numpy()
     est.close()
-    return (t1 - t0) * 1e3 / steps, st, bins, grid
+    return (t1 - t0) * 1e3 / steps, st, bins, grid, wl
 
 
 def cpu_baseline(B, plan, field, target_s):
@@ -120,12 +121,13 @@ def main():
     field = B.make_field(g.NN, NRi=1, fnl=0.3)
     d_field = torch.from_numpy(field).to("cuda")
 
-    ms, st, bins, grid = time_path(B, torch, plan, a.precision, d_field, a.steps, a.warmup,
+    ms, st, bins, grid, wl = time_path(B, torch, plan, a.precision, d_field, a.steps, a.warmup,
                                    a.full_grid)
     res = {"fast": (ms, st, bins)}
     if not a.skip_fp64:
-        res["fp64"] = time_path(B, torch, plan, "fp64", d_field, a.steps, a.warmup,
-                                a.full_grid)[:3]
+        r64 = time_path(B, torch, plan, "fp64", d_field, a.steps, a.warmup, a.full_grid)
+        res["fp64"] = r64[:3]
+        wl64 = r64[4]
 
     # useful contraction flops actually performed: Re(P C) = 2 FMAs per point
     # per contraction on the grid the sums run on (NG^2 points)
@@ -154,6 +156,7 @@ def main():
                    "precision": a.precision, "grid": grid},
         "stages_ms": {"gather": st[0], "fft_cache": st[1], "contraction": st[2],
                       "binning": st[3]},
+        "worklist": wl,
         "roofline": {"bound": "mfma", "kernel": "contraction",
                      "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,
@@ -166,7 +169,8 @@ def main():
                        "stages_ms": {"gather": st64[0], "fft_cache": st64[1],
                                      "contraction": st64[2], "binning": st64[3]},
                        "contraction_TFLOPs": w_contr / (st64[2] * 1e-3) / 1e12,
-                       "contraction_frac_fp64_peak": w_contr / (st64[2] * 1e-3) / PEAK["fp64"]}
+                       "contraction_frac_fp64_peak": w_contr / (st64[2] * 1e-3) / PEAK["fp64"],
+                       "worklist": wl64}
         out["speedup_vs_fp64"] = ms64 / ms
         nb = g.nbins * 16
         err = {}

@@ -152,6 +152,14 @@ int bb_estimate_device(bb_estimator *e, const double *d_field_r2c,
 int bb_estimator_contraction_sums(bb_estimator *e, double *S);
 int bb_estimator_triangle_counts(bb_estimator *e, double *num);
 int bb_estimator_field(bb_estimator *e, int plane, int ring, double *out);
+/* Contraction work lists: out[0] work items, [1] x-splits, [2] 16x16 output
+ * tiles, [3] largest staged ring count of an item, [4] sum of staged rings
+ * over items, [5] bytes the contraction reads from the cache per estimate,
+ * [6] grid NG, [7] points per LDS chunk, [8] waves per workgroup, [9] wave
+ * balance x1000 (tile windows / (waves x busiest wave), summed over items).
+ * The environment variable BARBIE_CONTRACT_WAVES (4 or 8) picks the
+ * workgroup width when the estimator is created. */
+int bb_estimator_worklist(const bb_estimator *e, long long *out10);
 /* Per-stage device times of the last estimate in milliseconds:
  * [0] upload/gather, [1] plane-ring FFTs, [2] contraction, [3] binning. */
 int bb_estimator_timings(bb_estimator *e, double *ms4);
