@@ -25,51 +25,90 @@ __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_doub
 // multiply by +i
 __device__ __forceinline__ double2 cmuli(double2 a) { return make_double2(-a.y, a.x); }
 
-// Length-R DFT with exp(+2 pi i q s / R), in place.
-__device__ __forceinline__ void butterfly(double2 *v, int R)
+// Length-R DFT with exp(+2 pi i q s / R), in place, for R = 2, 3, 4, 5.
+template <int R>
+__device__ __forceinline__ void butterfly(double2 *v);
+template <>
+__device__ __forceinline__ void butterfly<4>(double2 *v)
 {
-    if (R == 4) {
-        double2 a = v[0], b = v[1], c = v[2], d = v[3];
-        double2 s0 = cadd(a, c), d0 = csub(a, c), s1 = cadd(b, d), d1 = cmuli(csub(b, d));
-        v[0] = cadd(s0, s1);
-        v[1] = cadd(d0, d1);
-        v[2] = csub(s0, s1);
-        v[3] = csub(d0, d1);
-    } else if (R == 2) {
-        double2 a = v[0], b = v[1];
-        v[0] = cadd(a, b);
-        v[1] = csub(a, b);
-    } else if (R == 3) {
-        const double h = 0.86602540378443864676;  // sqrt(3)/2
-        double2 a = v[0], t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
-        double2 m = make_double2(a.x - 0.5 * t1.x, a.y - 0.5 * t1.y);
-        double2 j = make_double2(-h * t2.y, h * t2.x);  // i*h*t2
-        v[0] = cadd(a, t1);
-        v[1] = cadd(m, j);
-        v[2] = csub(m, j);
-    } else {  // R == 5
-        const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
-        const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
-        double2 a = v[0];
-        double2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
-        double2 u1 = csub(v[1], v[4]), u2 = csub(v[2], v[3]);
-        double2 m1 = make_double2(a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y);
-        double2 m2 = make_double2(a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y);
-        // i*(s1*u1 + s2*u2) and i*(s2*u1 - s1*u2)
-        double2 n1 = make_double2(-(s1 * u1.y + s2 * u2.y), s1 * u1.x + s2 * u2.x);
-        double2 n2 = make_double2(-(s2 * u1.y - s1 * u2.y), s2 * u1.x - s1 * u2.x);
-        v[0] = cadd(a, cadd(t1, t2));
-        v[1] = cadd(m1, n1);
-        v[4] = csub(m1, n1);
-        v[2] = cadd(m2, n2);
-        v[3] = csub(m2, n2);
+    double2 a = v[0], b = v[1], c = v[2], d = v[3];
+    double2 s0 = cadd(a, c), d0 = csub(a, c), s1 = cadd(b, d), d1 = cmuli(csub(b, d));
+    v[0] = cadd(s0, s1);
+    v[1] = cadd(d0, d1);
+    v[2] = csub(s0, s1);
+    v[3] = csub(d0, d1);
+}
+template <>
+__device__ __forceinline__ void butterfly<2>(double2 *v)
+{
+    double2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+template <>
+__device__ __forceinline__ void butterfly<3>(double2 *v)
+{
+    const double h = 0.86602540378443864676;  // sqrt(3)/2
+    double2 a = v[0], t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+    double2 m = make_double2(a.x - 0.5 * t1.x, a.y - 0.5 * t1.y);
+    double2 j = make_double2(-h * t2.y, h * t2.x);  // i*h*t2
+    v[0] = cadd(a, t1);
+    v[1] = cadd(m, j);
+    v[2] = csub(m, j);
+}
+template <>
+__device__ __forceinline__ void butterfly<5>(double2 *v)
+{
+    const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+    const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
+    double2 a = v[0];
+    double2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+    double2 u1 = csub(v[1], v[4]), u2 = csub(v[2], v[3]);
+    double2 m1 = make_double2(a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y);
+    double2 m2 = make_double2(a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y);
+    // i*(s1*u1 + s2*u2) and i*(s2*u1 - s1*u2)
+    double2 n1 = make_double2(-(s1 * u1.y + s2 * u2.y), s1 * u1.x + s2 * u2.x);
+    double2 n2 = make_double2(-(s2 * u1.y - s1 * u2.y), s2 * u1.x - s1 * u2.x);
+    v[0] = cadd(a, cadd(t1, t2));
+    v[1] = cadd(m1, n1);
+    v[4] = csub(m1, n1);
+    v[2] = cadd(m2, n2);
+    v[3] = csub(m2, n2);
+}
+
+// One Stockham stage of radix R over nb transforms of length N: reads
+// x[i + t N/R], twiddles by W_{pR}^{t k} (k = i mod p), writes
+// y[(i-k) R + k + t p].
+template <int R>
+__device__ __forceinline__ void fft_stage(const double2 *a, double2 *b, int nb, int N, int p,
+                                          const double2 *__restrict__ tw)
+{
+    const unsigned NR = (unsigned)(N / R);
+    const unsigned tstep = (unsigned)(N / (p * R));
+    const unsigned total = (unsigned)nb * NR;
+    for (unsigned idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        const unsigned bt = idx / NR;
+        const unsigned i = idx - bt * NR;
+        const unsigned k = i % (unsigned)p;
+        const double2 *x = a + bt * (unsigned)N;
+        double2 *y = b + bt * (unsigned)N;
+        double2 v[R];
+#pragma unroll
+        for (int t = 0; t < R; ++t) v[t] = x[i + t * NR];
+        if (k > 0) {
+#pragma unroll
+            for (int t = 1; t < R; ++t) v[t] = cmul(v[t], tw[t * k * tstep]);
+        }
+        butterfly<R>(v);
+        const unsigned j = (i - k) * R + k;
+#pragma unroll
+        for (int t = 0; t < R; ++t) y[j + t * (unsigned)p] = v[t];
     }
 }
 
 // nb transforms of length N stored back to back in a (LDS); b is scratch of
-// the same size.  Returns the buffer holding the result.  Self-sorting
-// Stockham: stage with radix R reads x[i + t N/R], twiddles by
-// W_{pR}^{t k} (k = i mod p), and writes y[(i-k) R + k + t p].
+// the same size.  Returns the buffer holding the result (self-sorting
+// Stockham, radices in fd.radix order).
 __device__ double2 *lds_fft(double2 *a, double2 *b, int nb, const FftDesc &fd,
                             const double2 *__restrict__ tw)
 {
@@ -77,22 +116,11 @@ __device__ double2 *lds_fft(double2 *a, double2 *b, int nb, const FftDesc &fd,
     int p = 1;
     for (int s = 0; s < fd.nst; ++s) {
         const int R = fd.radix[s];
-        const int NR = N / R;
-        const int tstep = N / (p * R);
-        for (int idx = threadIdx.x; idx < nb * NR; idx += blockDim.x) {
-            const int bt = idx / NR;
-            const int i = idx - bt * NR;
-            const int k = i % p;
-            const double2 *x = a + bt * N;
-            double2 *y = b + bt * N;
-            double2 v[5];
-            for (int t = 0; t < R; ++t) {
-                v[t] = x[i + t * NR];
-                if (t > 0 && k > 0) v[t] = cmul(v[t], tw[(t * k * tstep) % N]);
-            }
-            butterfly(v, R);
-            const int j = (i - k) * R + k;
-            for (int t = 0; t < R; ++t) y[j + t * p] = v[t];
+        switch (R) {
+        case 4: fft_stage<4>(a, b, nb, N, p, tw); break;
+        case 2: fft_stage<2>(a, b, nb, N, p, tw); break;
+        case 3: fft_stage<3>(a, b, nb, N, p, tw); break;
+        default: fft_stage<5>(a, b, nb, N, p, tw); break;
         }
         __syncthreads();
         double2 *tmp = a; a = b; b = tmp;
