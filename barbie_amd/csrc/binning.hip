@@ -108,7 +108,7 @@ __global__ __launch_bounds__(64) void k_binning(const int32_t *__restrict__ seg,
         double Biana = 2. * bp.ff * (P0 * P1 + P1 * P2 + P2 * P0);
         Biana = Biana * (1 + bp.beta * m1 * m1) * (1 + bp.beta * m2 * m2) * (1 + bp.beta * m3 * m3);
         double Sv = 0.0;  // sum of the x-split partials, fixed order
-        for (int sp = 0; sp < bp.nsplit; ++sp) Sv += S[sp * bp.split_stride + R.pos];
+        for (int sp = 0; sp < R.nsp; ++sp) Sv += S[(long long)R.pos + 256 * sp];
         // barbie.c:355-357.  The sum is exactly zero iff one of its three
         // fields is identically zero; the fields' zero flags decide it.
         if (!nzflag[R.f1] || !nzflag[R.f2] || !nzflag[R.f3]) {

@@ -78,7 +78,7 @@ template <int NT>
 struct Stager {
     const char *cache;
     const char *split;
-    long long field_bytes;   // bytes of one field
+    long long field_bytes;   // bytes between fields (the padding past a field's points is zero)
     int nunits;              // 8-byte units this chunk: nrows * kRowBytes / 8
     int nrows;
     uint2 pf[kPF];
@@ -154,9 +154,8 @@ struct K4<float> {
 // is sized for.
 template <typename R, int NW, int TWW, int MINB>
 __global__ __launch_bounds__(64 * NW, MINB) void k_contract_k4(
-    const typename K4<R>::C2 *__restrict__ cache, long long npts,
-    const WorkItem *__restrict__ items, int nitems, const TWin *__restrict__ tws, int nsplit,
-    double *__restrict__ out, long long split_stride, double scale)
+    const typename K4<R>::C2 *__restrict__ cache, const WorkItem *__restrict__ items,
+    const WorkUnit *__restrict__ units, const TWin *__restrict__ tws, double *__restrict__ out)
 {
     using T = K4<R>;
     using C2 = typename T::C2;
@@ -167,13 +166,11 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_contract_k4(
     int *row_split = row_slot + kRowCap;
     unsigned char *rows = lds + kTableBytes;
 
-    const int item_id = blockIdx.x / nsplit, split = blockIdx.x - item_id * nsplit;
-    if (item_id >= nitems) return;
-    const WorkItem &it = items[item_id];
+    const WorkUnit U = units[blockIdx.x];
+    const WorkItem &it = items[U.item];
     const int nrows = uni(it.nrows);
     row_tables<64 * NW>(it, row_slot, row_split);
-    const long long nchunks = (npts + CH - 1) / CH;
-    const long long cb = nchunks * split / nsplit, ce = nchunks * (split + 1) / nsplit;
+    const long long cb = U.cb, ce = U.ce;
     const int lane = threadIdx.x & 63, wave = uni(threadIdx.x >> 6);
     const int m = lane & 15, kq = lane >> 4;
 
@@ -199,9 +196,9 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_contract_k4(
         for (int q = 0; q < 4; ++q) accd[j][q] = 0.0;
     }
     Stager<64 * NW> S;
-    S.cache = reinterpret_cast<const char *>(cache);
+    S.cache = reinterpret_cast<const char *>(cache + it.off);
     S.split = S.cache;
-    S.field_bytes = npts * (long long)sizeof(C2);
+    S.field_bytes = it.fstride * (long long)sizeof(C2);
     S.nunits = nrows * (kRowBytes / 8);
     S.nrows = nrows;
     __syncthreads();
@@ -252,12 +249,12 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_contract_k4(
         }
     }
 
-    double *o = out + (long long)split * split_stride;
+    const double scale = it.scale;
 #pragma unroll
     for (int j = 0; j < TWW; ++j) {
         const int ot_idx = TW[j].out;
         if (ot_idx >= 0) {
-            double *ot = o + (long long)ot_idx * 256;
+            double *ot = out + ((long long)ot_idx + U.split) * 256;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const double v = T::kFP64Acc ? (double)acc[j][q] : accd[j][q];
@@ -277,9 +274,9 @@ __device__ __forceinline__ f32x4 mfma_bf16(const bf16x8 &a, const bf16x8 &b, f32
 // hold, per group of 8 points, [Re hi x8][Re lo x8][Im hi x8][Im lo x8] bf16.
 template <int NW>
 __global__ __launch_bounds__(64 * NW, 8 / NW) void k_contract_bf16x3(
-    const float2 *__restrict__ cache, const __bf16 *__restrict__ splitc, long long npts,
-    const WorkItem *__restrict__ items, int nitems, const TWin *__restrict__ tws, int nsplit,
-    double *__restrict__ out, long long split_stride, double scale)
+    const float2 *__restrict__ cache, const __bf16 *__restrict__ splitc,
+    const WorkItem *__restrict__ items, const WorkUnit *__restrict__ units,
+    const TWin *__restrict__ tws, double *__restrict__ out)
 {
     constexpr int TWW = kTWW32;
     extern __shared__ __align__(16) unsigned char lds[];
@@ -287,13 +284,11 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_contract_bf16x3(
     int *row_split = row_slot + kRowCap;
     unsigned char *rows = lds + kTableBytes;
 
-    const int item_id = blockIdx.x / nsplit, split = blockIdx.x - item_id * nsplit;
-    if (item_id >= nitems) return;
-    const WorkItem &it = items[item_id];
+    const WorkUnit U = units[blockIdx.x];
+    const WorkItem &it = items[U.item];
     const int nrows = uni(it.nrows);
     row_tables<64 * NW>(it, row_slot, row_split);
-    const long long nchunks = (npts + 31) / 32;
-    const long long cb = nchunks * split / nsplit, ce = nchunks * (split + 1) / nsplit;
+    const long long cb = U.cb, ce = U.ce;
     const int lane = threadIdx.x & 63, wave = uni(threadIdx.x >> 6);
     const int m = lane & 15, kq = lane >> 4;
 
@@ -316,9 +311,9 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_contract_bf16x3(
         for (int q = 0; q < 4; ++q) accd[j][q] = 0.0;
     }
     Stager<64 * NW> S;
-    S.cache = reinterpret_cast<const char *>(cache);
-    S.split = reinterpret_cast<const char *>(splitc);
-    S.field_bytes = npts * 8;
+    S.cache = reinterpret_cast<const char *>(cache + it.off);
+    S.split = reinterpret_cast<const char *>(splitc + it.off * 4);
+    S.field_bytes = it.fstride * 8;
     S.nunits = nrows * (kRowBytes / 8);
     S.nrows = nrows;
     __syncthreads();
@@ -377,12 +372,12 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_contract_bf16x3(
         }
     }
 
-    double *o = out + (long long)split * split_stride;
+    const double scale = it.scale;
 #pragma unroll
     for (int j = 0; j < TWW; ++j) {
         const int ot_idx = TW[j].out;
         if (ot_idx >= 0) {
-            double *ot = o + (long long)ot_idx * 256;
+            double *ot = out + ((long long)ot_idx + U.split) * 256;
 #pragma unroll
             for (int q = 0; q < 4; ++q) ot[(4 * kq + q) * 16 + m] = accd[j][q] * scale;
         }
@@ -394,52 +389,45 @@ int contract_chunk(CPrec p) { return p == CPrec::F64 ? kRowBytes / 16 : kRowByte
 
 template <int NW>
 static hipError_t launch_nw(CPrec p, int tww, const void *cache, const void *split,
-                            long long npts, const WorkItem *items, int nitems, const TWin *tws,
-                            int nsplit, int max_rows, double *out, long long split_stride,
-                            double scale, hipStream_t s)
+                            const WorkItem *items, const WorkUnit *units, int nunits,
+                            const TWin *tws, double *out, hipStream_t s)
 {
-    dim3 grid((unsigned)((long long)nitems * nsplit));
-    (void)max_rows;
+    dim3 grid((unsigned)nunits);
     const size_t lds = kTableBytes + (size_t)(kRowCap + 1) * kPitch;   // + spare ring
     if (p == CPrec::F64 && tww <= kTWWF64Narrow) {
         auto k = k_contract_k4<double, NW, kTWWF64Narrow, 16 / NW>;
         (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds);
-        hipLaunchKernelGGL(k, grid, dim3(64 * NW), lds, s, (const double2 *)cache, npts, items,
-                           nitems, tws, nsplit, out, split_stride, scale);
+        hipLaunchKernelGGL(k, grid, dim3(64 * NW), lds, s, (const double2 *)cache, items, units,
+                           tws, out);
     } else if (p == CPrec::F64) {
         auto k = k_contract_k4<double, NW, kTWW, 8 / NW>;
         (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds);
-        hipLaunchKernelGGL(k, grid, dim3(64 * NW), lds, s, (const double2 *)cache, npts, items,
-                           nitems, tws, nsplit, out, split_stride, scale);
+        hipLaunchKernelGGL(k, grid, dim3(64 * NW), lds, s, (const double2 *)cache, items, units,
+                           tws, out);
     } else if (p == CPrec::F32) {
         auto k = k_contract_k4<float, NW, kTWWF32, 8 / NW>;
         (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds);
-        hipLaunchKernelGGL(k, grid, dim3(64 * NW), lds, s, (const float2 *)cache, npts, items,
-                           nitems, tws, nsplit, out, split_stride, scale);
+        hipLaunchKernelGGL(k, grid, dim3(64 * NW), lds, s, (const float2 *)cache, items, units,
+                           tws, out);
     } else {
         (void)hipFuncSetAttribute((const void *)k_contract_bf16x3<NW>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL((k_contract_bf16x3<NW>), grid, dim3(64 * NW), lds, s,
-                           (const float2 *)cache, (const __bf16 *)split, npts, items, nitems,
-                           tws, nsplit, out, split_stride, scale);
+                           (const float2 *)cache, (const __bf16 *)split, items, units, tws, out);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_contract(CPrec p, int waves, int tww, const void *cache, const void *split,
-                           long long npts, const WorkItem *items, int nitems, const TWin *tws,
-                           int nsplit, int max_rows, double *out, long long split_stride,
-                           double scale, hipStream_t s)
+                           const WorkItem *items, const WorkUnit *units, int nunits,
+                           const TWin *tws, double *out, hipStream_t s)
 {
-    if (nitems <= 0) return hipSuccess;
-    if (waves == 4)
-        return launch_nw<4>(p, tww, cache, split, npts, items, nitems, tws, nsplit, max_rows,
-                            out, split_stride, scale, s);
-    return launch_nw<8>(p, tww, cache, split, npts, items, nitems, tws, nsplit, max_rows, out,
-                        split_stride, scale, s);
+    if (nunits <= 0) return hipSuccess;
+    if (waves == 4) return launch_nw<4>(p, tww, cache, split, items, units, nunits, tws, out, s);
+    return launch_nw<8>(p, tww, cache, split, items, units, nunits, tws, out, s);
 }
 
 }  // namespace bb

@@ -19,10 +19,19 @@ struct FftDesc {
 struct FieldTask {
     int32_t plane;    // kz plane (>= 0); -1 for the unit field (mask only)
     int32_t ring;
-    int32_t fid;      // destination field slot in the cache
+    int32_t fid;      // destination field slot in the level's cache
     int32_t rowband;  // rows |ia| <= rowband can hold masked modes
     int32_t nlo, nhi; // ring holds ia^2 + ja^2 in [nlo, nhi)
+    int32_t zid;      // nonzero flag of (plane, ring): plane * Nr + ring
 };
+
+// ---------------------------------------------------------------------------
+// Grid levels.  The sum over x of a triple product of fields band-limited to
+// |k_x|, |k_y| <= K1, K2, K3 is the same (up to the factor (NN/NG)^2) on every
+// grid of NG > K1 + K2 + K3 points per side: no sum of three modes can wrap
+// around.  Each contraction runs on the smallest of a few such grids
+// ("levels"); every level holds its own copy of the fields it needs.
+constexpr int kMaxLevels = 16;
 
 // ---------------------------------------------------------------------------
 // Contraction work lists.
@@ -44,7 +53,7 @@ constexpr int kTWW = 12;        // tile windows per wave (slots reserved per wav
 constexpr int kTWW32 = 8;       // tile windows per wave, BF16x3 form
 constexpr int kTWWF32 = 6;      // tile windows per wave, FP32 form
 constexpr int kTWWF64Narrow = 6;  // FP64 form sized for two workgroups per CU
-constexpr int kMaxBlk = 12;     // ring blocks per item
+constexpr int kMaxBlk = 48;     // ring blocks per item
 constexpr int kRowCap = 240;    // staged rings per item (8 waves; 30 per wave)
 inline int rowcap_for(int waves) { return 30 * waves; }
 
@@ -57,11 +66,15 @@ struct TWin {
     int16_t nb;       // valid columns
     int16_t flags;    // TW_*
     int16_t pad;
-    int32_t out;      // output tile index
+    int32_t out;      // first partial tile; x-split s writes tile out + s (-1: unused slot)
 };
 
 struct WorkItem {
-    int32_t blk_slot[kMaxBlk];   // cache slot of the block's first ring
+    long long off;               // first point of the item's level in the cache
+    long long npts;              // points per field of the level (NG^2)
+    long long fstride;           // points between fields (npts rounded up to 32; zero padding)
+    double scale;                // (NN / NG)^2 of the level
+    int32_t blk_slot[kMaxBlk];   // field slot (in the level) of the block's first ring
     int16_t blk_rows[kMaxBlk];   // rings in the block (consecutive slots)
     int16_t blk_split[kMaxBlk];  // 1: staged as bf16 hi/lo B operands
     int16_t nblk, nrows;
@@ -69,20 +82,26 @@ struct WorkItem {
     int8_t cnt[kWaves];          // TWins per wave
 };
 
+// One workgroup of the contraction launch: chunks [cb, ce) of an item's
+// points, written to partial tile `split` of each of its windows.
+struct WorkUnit {
+    int32_t item, split, cb, ce;
+};
+
 // Contraction record used by the binning kernel (bin-sorted order).
 struct BinRec {
     int16_t p1, p2, p3, r1, r2, r3;
+    int16_t nsp;     // partial sums (x-splits), 256 doubles apart
+    int16_t pad;
     int32_t rp;      // ring triple (index into Num)
-    int32_t pos;     // position of the contraction's sum in the tile outputs
-    int32_t f1, f2, f3;  // field slots (zero-field rule)
+    int32_t pos;     // position of the contraction's first partial sum
+    int32_t f1, f2, f3;  // nonzero flags of the three fields (zero-field rule)
 };
 
 struct BinParams {
     int nbins;
     int n_mtbin;
     int NN;
-    int nsplit;              // partial sums per contraction (x-splits)
-    long long split_stride;  // doubles between splits
     double Cunits;
     double ff, beta;
 };
@@ -90,16 +109,19 @@ struct BinParams {
 // Launchers (fft.hip, contract.hip, binning.hip).
 hipError_t launch_gather(const double2 *field, double2 *planes, int NN, int NG, int kmax,
                          int Nmax, hipStream_t s);
-hipError_t launch_rowpass(const double2 *planes, double2 *rowbuf, long long buf_stride,
+// planes: NP x NP per plane (signed modes wrapped), read for the ring modes of
+// each task on the NG = fd.N grid of the level.
+hipError_t launch_rowpass(const double2 *planes, int NP, double2 *rowbuf, long long buf_stride,
                           const FieldTask *tasks, int ntasks, int max_rows, const FftDesc &fd,
                           const double2 *tw, int *nzflag, hipStream_t s);
+// Field fid goes to cache + fid * out_stride (points; a multiple of 8).
 hipError_t launch_colpass_f64(const double2 *rowbuf, long long buf_stride, double2 *cache,
-                              const FieldTask *tasks, int ntasks, const FftDesc &fd,
-                              const double2 *tw, hipStream_t s);
+                              long long out_stride, const FieldTask *tasks, int ntasks,
+                              const FftDesc &fd, const double2 *tw, hipStream_t s);
 // split (may be null): also write bf16 hi/lo operands, 8 bytes per point.
 hipError_t launch_colpass_f32(const double2 *rowbuf, long long buf_stride, float2 *cache,
-                              void *split, const FieldTask *tasks, int ntasks, const FftDesc &fd,
-                              const double2 *tw, hipStream_t s);
+                              void *split, long long out_stride, const FieldTask *tasks,
+                              int ntasks, const FftDesc &fd, const double2 *tw, hipStream_t s);
 
 // Contraction arithmetic.
 enum class CPrec { F64 = 0, F32 = 1, BF16X3 = 2 };
@@ -110,14 +132,14 @@ inline int contract_tww(CPrec p)
 {
     return p == CPrec::F64 ? kTWW : p == CPrec::F32 ? kTWWF32 : kTWW32;
 }
-// cache: field values (FP64 or FP32); split: bf16 hi/lo operands of the same
-// fields (BF16x3 only, rows flagged blk_split are read from it).
+// cache: field values (FP64 or FP32) of all levels; split: bf16 hi/lo
+// operands of the same fields at the same point offsets (BF16x3 only, rows
+// flagged blk_split are read from it).  One workgroup per unit.
 // waves: 4 or 8 waves per workgroup; tww: tile windows per wave (the work
 // lists must be packed for both).
 hipError_t launch_contract(CPrec p, int waves, int tww, const void *cache, const void *split,
-                           long long npts, const WorkItem *items, int nitems, const TWin *tws,
-                           int nsplit, int max_rows, double *out, long long split_stride,
-                           double scale, hipStream_t s);
+                           const WorkItem *items, const WorkUnit *units, int nunits,
+                           const TWin *tws, double *out, hipStream_t s);
 
 hipError_t launch_binning(const int32_t *seg, const BinRec *recs, const double *S,
                           const double *num, const int *nzflag, const double *avg,

@@ -1,7 +1,7 @@
-// Estimator: device memory, work lists and the per-field pipeline
-// (plane gather -> plane-ring cache -> contraction -> binning).
-// Replaces barbie.c:150-373 (cache build, ring triples, hot loop) for one
-// field held in HBM.
+// Estimator: grid levels, device memory, work lists and the per-field
+// pipeline (plane gather -> plane-ring caches per level -> contraction ->
+// binning).  Replaces barbie.c:150-373 (cache build, ring triples, hot loop)
+// for one field held in HBM.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -149,24 +149,73 @@ void pair_blocks(std::vector<CItem> &cs, int pa, int pb, int pc, int flags,
 struct WorkLists {
     std::vector<WorkItem> items;
     std::vector<TWin> tws;
-    int ntw = 0;               // output tiles
-    int max_rows = 0;          // largest staged ring count of an item
-    long long tile_slots = 0;  // ntw * 256
+    std::vector<int> item_work;   // tile windows of the busiest wave of each item
+    int ntw = 0;                  // output tiles
+    int max_rows = 0;             // largest staged ring count of an item
     long long tw_used = 0, tw_slots = 0;   // wave balance: sum cnt, sum waves * max cnt
 };
 
-// Packs pair blocks (already grouped so that neighbours share planes) into
-// work items.  slot(plane, ring) gives the cache slot; split_b: stage the r3
-// rings as bf16 hi/lo operands (BF16x3) instead of raw values.
-template <typename Slot>
-int pack_items(std::vector<PairBlock> &blocks, bool split_b, int tww, int waves, Slot slot,
+}  // namespace
+
+namespace bb {
+
+// One grid level: the NG x NG grid, the fields it holds (per plane the
+// contiguous range of rings in use) and their FFT tasks.
+struct Level {
+    int NG = 0;
+    long long npts = 0;
+    long long stride = 0;              // points between fields: npts rounded up to 32
+    double scale = 1.0;                // (NN / NG)^2
+    long long off = 0;                 // first point of the level in the cache
+    std::vector<int> rlo, rhi, base;   // per plane
+    int nfields = 0;
+    int max_rows = 0;                  // rows of the widest ring band
+    std::vector<FieldTask> tasks;
+
+    int fid(int plane, int ring) const
+    {
+        if (plane < 0 || plane >= (int)rlo.size() || rlo[plane] < 0) return -1;
+        if (ring < rlo[plane] || ring > rhi[plane]) return -1;
+        return base[plane] + ring - rlo[plane];
+    }
+};
+
+struct Layout {
+    CPrec prec = CPrec::F64;
+    int waves = kWaves, tww = kTWW, chunk = 32;
+    int NN = 0, NGtop = 0, kmax = 0;
+    std::vector<Level> lv;
+    std::vector<uint8_t> lvl;          // contraction -> level
+    long long cache_pts = 0;           // points of all levels' fields
+    std::vector<WorkItem> items;
+    std::vector<TWin> tws;
+    std::vector<WorkUnit> units;
+    long long ntiles = 0, part_tiles = 0;
+    std::vector<int32_t> pos;          // contraction -> first partial sum
+    std::vector<int16_t> nsp;          // contraction -> partial sums
+    std::vector<BinRec> recs;          // bin-sorted
+    std::vector<int32_t> seg;
+    BinParams bp{};
+    int max_rows = 0;
+    long long staged_rows = 0, staged_bytes = 0, useful_pts = 0;
+    double wave_balance = 1.0;
+};
+
+}  // namespace bb
+
+namespace {
+
+// Packs pair blocks of one level (already grouped so that neighbours share
+// planes) into work items.  split_b: stage the r3 rings as bf16 hi/lo
+// operands (BF16x3) instead of raw values.  pos[cid] receives tile * 256 +
+// row * 16 + column of each contraction.
+int pack_items(std::vector<PairBlock> &blocks, const Level &L, bool split_b, int tww, int waves,
                WorkLists &W, std::vector<int32_t> &pos)
 {
     const int rowcap = rowcap_for(waves);
     struct Key { int plane, split; bool operator<(const Key &o) const { return plane != o.plane ? plane < o.plane : split < o.split; } };
     struct Range { int lo, hi; };
     size_t a = 0;
-    std::vector<std::pair<long long, size_t>> order;   // (work, item index) for LPT ordering
     while (a < blocks.size()) {
         std::map<Key, Range> keys;
         int load[kWaves] = {0};
@@ -203,11 +252,18 @@ int pack_items(std::vector<PairBlock> &blocks, bool split_b, int tww, int waves,
         // Lay out the item: blocks in key order.
         WorkItem it;
         std::memset(&it, 0, sizeof it);
+        it.off = L.off;
+        it.npts = L.npts;
+        it.fstride = L.stride;
+        it.scale = L.scale;
         std::map<Key, int> row0;
         int nrows = 0;
         for (auto &kv : keys) {
             const int bi = it.nblk++;
-            it.blk_slot[bi] = slot(kv.first.plane, kv.second.lo);
+            const int f0 = L.fid(kv.first.plane, kv.second.lo);
+            if (f0 < 0 || L.fid(kv.first.plane, kv.second.hi) != f0 + kv.second.hi - kv.second.lo)
+                return set_error(BB_E_INVALID, "internal: staged rings missing from the level");
+            it.blk_slot[bi] = f0;
             it.blk_rows[bi] = (int16_t)(kv.second.hi - kv.second.lo + 1);
             it.blk_split[bi] = (int16_t)kv.first.split;
             row0[kv.first] = nrows - kv.second.lo;   // row of ring r = row0 + r
@@ -224,7 +280,7 @@ int pack_items(std::vector<PairBlock> &blocks, bool split_b, int tww, int waves,
             dummy.out = -1;
             W.tws.resize(W.tws.size() + kWaves * kTWW, dummy);
         }
-        long long work = 0;
+        int work = 0;
         for (int w = 0; w < waves; ++w) {
             int j = 0;
             for (size_t bi : wave_blocks[w]) {
@@ -252,52 +308,138 @@ int pack_items(std::vector<PairBlock> &blocks, bool split_b, int tww, int waves,
                 }
             }
             it.cnt[w] = (int8_t)j;
-            work = std::max<long long>(work, j);
+            work = std::max(work, j);
             W.tw_used += j;
         }
         W.tw_slots += (long long)waves * work;
-        order.push_back({work * 1000000LL + nrows, W.items.size()});
+        W.item_work.push_back(work);
         W.items.push_back(it);
         a = b;
     }
-    // Largest items first (longest-processing-time order).
+    return BB_OK;
+}
+
+// Workgroups of the launch: each item's chunks are split into x-ranges so
+// that units carry similar work (or into x_splits ranges when given), and
+// every tile gets one partial tile per x-range.  Rewrites the tile indices
+// of the windows and of pos into partial-tile positions; units are ordered
+// largest first.
+constexpr long long kTargetUnits = 2048;
+
+int make_units(WorkLists &W, int chunk, int x_splits, std::vector<WorkUnit> &units,
+               std::vector<int32_t> &pos, std::vector<int16_t> &nsp, long long &part_tiles)
+{
+    const size_t n = W.items.size();
+    std::vector<int> nch(n), sp(n);
+    long long total = 0;
+    for (size_t i = 0; i < n; ++i) {
+        nch[i] = (int)((W.items[i].npts + chunk - 1) / chunk);
+        total += (long long)W.item_work[i] * nch[i];
+    }
+    const long long unit_cost = std::max<long long>(1, total / kTargetUnits);
+    for (size_t i = 0; i < n; ++i) {
+        const long long cost = (long long)W.item_work[i] * nch[i];
+        int s = x_splits > 0 ? x_splits : (int)std::min<long long>(64, (cost + unit_cost - 1) / unit_cost);
+        sp[i] = std::max(1, std::min(s, nch[i]));
+    }
+    std::vector<int> tile_item(W.ntw, -1);
+    for (size_t i = 0; i < n; ++i) {
+        const WorkItem &it = W.items[i];
+        for (int w = 0; w < kWaves; ++w)
+            for (int j = 0; j < it.cnt[w]; ++j) {
+                const int t = W.tws[(size_t)it.tw0 + w * kTWW + j].out;
+                if (t < 0 || t >= W.ntw) return set_error(BB_E_INVALID, "internal: tile index out of range");
+                tile_item[t] = (int)i;
+            }
+    }
+    std::vector<long long> pbase(W.ntw);
+    long long run = 0;
+    for (int t = 0; t < W.ntw; ++t) {
+        if (tile_item[t] < 0) return set_error(BB_E_INVALID, "internal: tile without a window");
+        pbase[t] = run;
+        run += sp[tile_item[t]];
+    }
+    if (run * 256 > 0x7fffffffLL)
+        return set_error(BB_E_INVALID, "too many partial sums (" + std::to_string(run) +
+                                           " partial tiles); lower x_splits");
+    part_tiles = run;
+    for (TWin &T : W.tws)
+        if (T.out >= 0) T.out = (int32_t)pbase[T.out];
+    nsp.assign(pos.size(), 1);
+    for (size_t c = 0; c < pos.size(); ++c) {
+        const int t = pos[c] / 256, r = pos[c] % 256;
+        pos[c] = (int32_t)(pbase[t] * 256 + r);
+        nsp[c] = (int16_t)sp[tile_item[t]];
+    }
+    units.clear();
+    std::vector<std::pair<long long, size_t>> order;
+    for (size_t i = 0; i < n; ++i)
+        for (int s = 0; s < sp[i]; ++s) {
+            WorkUnit u;
+            u.item = (int32_t)i;
+            u.split = s;
+            u.cb = (int32_t)((long long)nch[i] * s / sp[i]);
+            u.ce = (int32_t)((long long)nch[i] * (s + 1) / sp[i]);
+            order.push_back({(long long)(u.ce - u.cb) * W.item_work[i], units.size()});
+            units.push_back(u);
+        }
     std::stable_sort(order.begin(), order.end(),
                      [](const std::pair<long long, size_t> &x, const std::pair<long long, size_t> &y) {
                          return x.first > y.first;
                      });
-    std::vector<WorkItem> sorted;
-    sorted.reserve(W.items.size());
-    for (auto &o : order) sorted.push_back(W.items[o.second]);
-    W.items.swap(sorted);
-    W.tile_slots = (long long)W.ntw * 256;
+    std::vector<WorkUnit> sorted;
+    sorted.reserve(units.size());
+    for (auto &o : order) sorted.push_back(units[o.second]);
+    units.swap(sorted);
     return BB_OK;
 }
 
-// Checks the invariants the contraction kernels rely on (every LDS row index
-// inside the item's staged rows, every output slot used once).
-int check_lists(const WorkLists &W, const std::vector<int32_t> &pos, size_t ncontr)
+// Checks the invariants the contraction kernels rely on: every staged ring
+// inside its level, every LDS row index inside the item's staged rows, every
+// unit's chunk range inside its item, every partial sum written once.
+int check_lists(const Layout &L, size_t ncontr)
 {
-    std::vector<uint8_t> seen((size_t)std::max<long long>(W.tile_slots, 1), 0);
-    for (size_t c = 0; c < ncontr; ++c) {
-        if (pos[c] < 0 || pos[c] >= W.tile_slots || seen[pos[c]])
-            return set_error(BB_E_INVALID, "internal: contraction output slots are not unique");
-        seen[pos[c]] = 1;
+    const long long nslots = L.part_tiles * 256;
+    std::vector<uint8_t> seen((size_t)std::max<long long>(nslots, 1), 0);
+    for (size_t c = 0; c < ncontr; ++c)
+        for (int s = 0; s < L.nsp[c]; ++s) {
+            const long long q = (long long)L.pos[c] + 256LL * s;
+            if (L.pos[c] < 0 || L.nsp[c] < 1 || q >= nslots || seen[q])
+                return set_error(BB_E_INVALID, "internal: contraction output slots are not unique");
+            seen[q] = 1;
+        }
+    std::vector<int> item_sp(L.items.size(), 0);
+    for (const WorkUnit &u : L.units) {
+        if (u.item < 0 || (size_t)u.item >= L.items.size())
+            return set_error(BB_E_INVALID, "internal: unit without an item");
+        const WorkItem &it = L.items[u.item];
+        const long long nch = (it.npts + L.chunk - 1) / L.chunk;
+        if (u.cb < 0 || u.cb >= u.ce || u.ce > nch)
+            return set_error(BB_E_INVALID, "internal: unit chunk range out of range");
+        item_sp[u.item]++;
     }
-    for (const WorkItem &it : W.items) {
+    for (size_t i = 0; i < L.items.size(); ++i) {
+        const WorkItem &it = L.items[i];
         int rows = 0;
-        for (int b = 0; b < it.nblk; ++b) rows += it.blk_rows[b];
-        if (rows != it.nrows || it.nrows > kRowCap || it.nblk > kMaxBlk)
+        for (int b = 0; b < it.nblk; ++b) {
+            rows += it.blk_rows[b];
+            if (it.blk_slot[b] < 0 || it.fstride < it.npts || it.fstride % 32 != 0 ||
+                it.off + (long long)(it.blk_slot[b] + it.blk_rows[b]) * it.fstride > L.cache_pts)
+                return set_error(BB_E_INVALID, "internal: staged ring outside the cache");
+        }
+        if (rows != it.nrows || it.nrows > kRowCap || it.nblk > kMaxBlk || item_sp[i] < 1)
             return set_error(BB_E_INVALID, "internal: work item row blocks are inconsistent");
         for (int w = 0; w < kWaves; ++w) {
             if (it.cnt[w] < 0 || it.cnt[w] > kTWW)
                 return set_error(BB_E_INVALID, "internal: too many tile windows in a wave");
             for (int j = it.cnt[w]; j < kTWW; ++j)
-                if (W.tws[(size_t)it.tw0 + w * kTWW + j].out != -1)
+                if (L.tws[(size_t)it.tw0 + w * kTWW + j].out != -1)
                     return set_error(BB_E_INVALID, "internal: unused tile window slot is live");
             for (int j = 0; j < it.cnt[w]; ++j) {
-                const TWin &T = W.tws[(size_t)it.tw0 + w * kTWW + j];
+                const TWin &T = L.tws[(size_t)it.tw0 + w * kTWW + j];
                 bool ok = T.b0 >= 0 && T.b0 < it.nrows && T.nb >= 1 && T.nb <= 16 &&
-                          T.out >= 0 && T.out < W.ntw && (j > 0 || (T.flags & TW_NEWA));
+                          T.out >= 0 && (long long)T.out + item_sp[i] <= L.part_tiles &&
+                          (j > 0 || (T.flags & TW_NEWA));
                 for (int m = 0; m < 16; ++m)
                     ok = ok && T.a1[m] >= -1 && T.a1[m] < it.nrows && T.a2[m] >= 0 &&
                          T.a2[m] < it.nrows;
@@ -308,10 +450,291 @@ int check_lists(const WorkLists &W, const std::vector<int32_t> &pos, size_t ncon
     return BB_OK;
 }
 
-int auto_splits(int nitems)
+// Largest |k_x| of any mode of ring r (0 for a ring without modes): the
+// largest ia for which some ja puts ia^2 + ja^2 in [nlo, nhi).
+std::vector<int> ring_kx(const PlanData &d)
 {
-    int s = (1024 + std::max(nitems, 1) - 1) / std::max(nitems, 1);
-    return std::max(1, std::min(s, 64));
+    std::vector<int> K(d.g.Nr, 0);
+    for (int r = 0; r < d.g.Nr; ++r) {
+        const int lo = d.nlo[r], hi = d.nhi[r];
+        for (int ia = hi > 0 ? isqrt_floor(hi - 1) : -1; ia >= 0; --ia) {
+            const int top = hi - 1 - ia * ia;            // ja^2 <= top
+            const int j = isqrt_floor(top);
+            if (j >= 0 && (long long)j * j >= (long long)lo - (long long)ia * ia) {
+                K[r] = ia;
+                break;
+            }
+        }
+    }
+    return K;
+}
+
+// Grid levels.  A contraction of rings (r1, r2, r3) sums exactly on any grid
+// of NG > K1 + K2 + K3 points per side (K_r the largest |k_x| of ring r: no
+// sum of three modes wraps around), and the FFT of a ring on NG points holds
+// all its modes when NG >= 2 K_r + 2; need(c) is the larger bound.  Both
+// hold on the NN grid when NN >= 3 kmax + 1 (kmax over all rings); when NN
+// is smaller the NN-grid sums alias, and only the NN grid reproduces them.
+// Otherwise up to max_levels grids are chosen among the even 2,3,5-smooth
+// sizes to minimise the sum over contractions of NG(c)^2 (dynamic
+// programming over the histogram of needs); lvl[c] is the level of
+// contraction c and NGtop the largest grid (the plane grid).  kmax is the
+// largest |k_x| of any ring a contraction uses.
+void choose_grids(const PlanData &d, bool full_grid, int max_levels, int &kmax, int &NGtop,
+                  std::vector<int> &grids, std::vector<uint8_t> &lvl)
+{
+    const int NN = d.g.NN;
+    const std::vector<int> K = ring_kx(d);
+    int kall = 0;
+    for (int k : K) kall = std::max(kall, k);
+    lvl.assign(d.contr.size(), 0);
+    kmax = kall;
+    NGtop = NN;
+    grids.assign(1, NN);
+    if (full_grid || NN < 3 * kall + 1 || d.contr.empty()) return;
+
+    std::vector<int> need(d.contr.size());
+    int maxneed = 1;
+    kmax = 0;
+    for (size_t c = 0; c < d.contr.size(); ++c) {
+        const Contraction &k = d.contr[c];
+        const int a = K[k.r1], b = K[k.r2], e = K[k.r3];
+        need[c] = std::max(a + b + e + 1, 2 * std::max(a, std::max(b, e)) + 2);
+        maxneed = std::max(maxneed, need[c]);
+        kmax = std::max(kmax, std::max(a, std::max(b, e)));
+    }
+    std::vector<int> cand;
+    for (int n = 8; n <= NN; n += 2)
+        if (smooth235(n)) {
+            cand.push_back(n);
+            if (n >= maxneed) break;
+        }
+    const int top = cand.back();   // NN itself qualifies (even, smooth, >= maxneed)
+    std::vector<long long> cum(top + 2, 0);   // cum[x + 1] = contractions needing <= x
+    for (int n : need) cum[n + 1]++;
+    for (int x = 1; x <= top + 1; ++x) cum[x] += cum[x - 1];
+    auto below = [&](int x) { return cum[std::min(x, top) + 1]; };
+    const int C = (int)cand.size(), T = C - 1;
+    const int Lmax = std::max(1, std::min(max_levels, C));
+    const double inf = 1e300;
+    std::vector<std::vector<double>> F(Lmax + 1, std::vector<double>(C, inf));
+    std::vector<std::vector<int>> from(Lmax + 1, std::vector<int>(C, -1));
+    for (int j = 0; j < C; ++j) F[1][j] = (double)below(cand[j]) * cand[j] * cand[j];
+    for (int l = 2; l <= Lmax; ++l)
+        for (int j = 0; j < C; ++j)
+            for (int i = 0; i < j; ++i) {
+                if (F[l - 1][i] >= inf) continue;
+                const double v = F[l - 1][i] +
+                                 (double)(below(cand[j]) - below(cand[i])) * cand[j] * cand[j];
+                if (v < F[l][j]) {
+                    F[l][j] = v;
+                    from[l][j] = i;
+                }
+            }
+    int best = 1;
+    for (int l = 2; l <= Lmax; ++l)
+        if (F[l][T] < F[best][T]) best = l;
+    std::vector<int> g;
+    for (int l = best, j = T; l >= 1 && j >= 0; j = from[l][j], --l) g.push_back(cand[j]);
+    std::sort(g.begin(), g.end());
+    grids.clear();
+    int prev = 0;
+    for (int n : g) {
+        if (below(n) > below(prev) || n == g.back()) grids.push_back(n);   // skip empty levels
+        prev = n;
+    }
+    NGtop = grids.back();
+    for (size_t c = 0; c < d.contr.size(); ++c) {
+        int l = 0;
+        while (grids[l] < need[c]) ++l;
+        lvl[c] = (uint8_t)l;
+    }
+}
+
+int default_levels(const bb_options &opt)
+{
+    if (opt.grid_levels > 0) return opt.grid_levels;
+    if (const char *ev = std::getenv("BARBIE_GRID_LEVELS")) {
+        const int v = std::atoi(ev);
+        if (v >= 1 && v <= BB_MAX_LEVELS) return v;
+    }
+    return 8;
+}
+
+int layout_waves()
+{
+    if (const char *ev = std::getenv("BARBIE_CONTRACT_WAVES")) {
+        const int w = std::atoi(ev);
+        if (w == 4 || w == 8) return w;
+    }
+    return kWaves;
+}
+
+int layout_tww(CPrec p)
+{
+    int t = contract_tww(p);
+    if (const char *ev = std::getenv("BARBIE_CONTRACT_TWW")) {
+        const int v = std::atoi(ev);
+        if (p == CPrec::F64 && (v == kTWW || v == kTWWF64Narrow)) t = v;
+    }
+    return t;
+}
+
+int build_layout(const PlanData &d, CPrec prec, const bb_options &opt, int waves, int tww,
+                 int max_levels, Layout &L)
+{
+    const int NN = d.g.NN, Nmax = d.g.Nmax, Nr = d.g.Nr;
+    L.prec = prec;
+    L.waves = waves;
+    L.tww = tww;
+    L.chunk = contract_chunk(prec);
+    L.NN = NN;
+    std::vector<int> grids;
+    choose_grids(d, opt.full_grid != 0, max_levels, L.kmax, L.NGtop, grids, L.lvl);
+    const std::vector<uint8_t> &lvl = L.lvl;
+    const std::vector<int> K = ring_kx(d);
+    const size_t elem = prec == CPrec::F64 ? 16 : 8;
+
+    // Fields of every level: per plane the contiguous range of rings in use.
+    const int nl = (int)grids.size();
+    L.lv.assign(nl, Level());
+    {
+        std::vector<std::vector<uint8_t>> used(nl, std::vector<uint8_t>((size_t)Nmax * Nr, 0));
+        for (size_t c = 0; c < d.contr.size(); ++c) {
+            const Contraction &k = d.contr[c];
+            std::vector<uint8_t> &u = used[lvl[c]];
+            u[(size_t)k.p1 * Nr + k.r1] = 1;
+            u[(size_t)std::abs(k.p2) * Nr + k.r2] = 1;
+            u[(size_t)std::abs(k.p3) * Nr + k.r3] = 1;
+        }
+        for (int l = 0; l < nl; ++l) {
+            Level &V = L.lv[l];
+            V.NG = grids[l];
+            V.npts = (long long)V.NG * V.NG;
+            V.stride = (V.npts + 31) / 32 * 32;
+            V.scale = ((double)NN * NN) / ((double)V.NG * V.NG);
+            V.off = L.cache_pts;
+            V.rlo.assign(Nmax, -1);
+            V.rhi.assign(Nmax, -2);
+            V.base.assign(Nmax, 0);
+            for (int p = 0; p < Nmax; ++p)
+                for (int r = 0; r < Nr; ++r)
+                    if (used[l][(size_t)p * Nr + r]) {
+                        if (V.rlo[p] < 0) V.rlo[p] = r;
+                        V.rhi[p] = r;
+                    }
+            for (int p = 0; p < Nmax; ++p) {
+                V.base[p] = V.nfields;
+                if (V.rlo[p] < 0) continue;
+                for (int r = V.rlo[p]; r <= V.rhi[p]; ++r) {
+                    const int band = d.nhi[r] > 0 ? std::min(V.NG / 2, K[r]) : -1;
+                    V.tasks.push_back(FieldTask{p, r, V.nfields, band, d.nlo[r], d.nhi[r], p * Nr + r});
+                    V.max_rows = std::max(V.max_rows, std::min(V.NG, 2 * band + 1));
+                    V.nfields++;
+                }
+            }
+            L.cache_pts += (long long)V.nfields * V.stride;
+        }
+    }
+
+    // Pair blocks per (level, plane group); groups of one level with the same
+    // plane set {p1, |p2|, |p3|} are adjacent so that work items share staged
+    // rings.
+    WorkLists W;
+    L.pos.assign(std::max<size_t>(d.contr.size(), 1), 0);
+    for (int l = 0; l < nl; ++l) {
+        std::map<std::pair<int, int>, std::vector<CItem>> groups;
+        for (size_t c = 0; c < d.contr.size(); ++c) {
+            if (lvl[c] != l) continue;
+            const Contraction &k = d.contr[c];
+            groups[{k.p1, k.p2}].push_back({k.r1, k.r2, k.r3, (int)c});
+        }
+        std::map<std::array<int, 3>, std::vector<std::pair<int, int>>> by_set;
+        for (auto &kv : groups) {
+            const int p1 = kv.first.first, p2 = kv.first.second, p3 = -p1 - p2;
+            std::array<int, 3> key{p1, std::abs(p2), std::abs(p3)};
+            std::sort(key.begin(), key.end());
+            by_set[key].push_back(kv.first);
+        }
+        std::vector<PairBlock> blocks;
+        for (auto &kv : by_set)
+            for (auto &gk : kv.second) {
+                const int p1 = gk.first, p2 = gk.second, p3 = -p1 - p2;
+                const int flags = (p2 < 0 ? TW_CONJ2 : 0) | (p3 < 0 ? TW_CONJ3 : 0);
+                pair_blocks(groups[gk], p1, std::abs(p2), std::abs(p3), flags, blocks);
+            }
+        int rc = pack_items(blocks, L.lv[l], prec == CPrec::BF16X3, tww, waves, W, L.pos);
+        if (rc) return rc;
+    }
+    L.ntiles = W.ntw;
+    L.max_rows = W.max_rows;
+    L.wave_balance = W.tw_slots ? (double)W.tw_used / (double)W.tw_slots : 1.0;
+    for (const WorkItem &it : W.items) {
+        L.staged_rows += it.nrows;
+        L.staged_bytes += (long long)it.nrows * it.npts * (long long)elem;   // padding excluded
+    }
+    for (size_t c = 0; c < d.contr.size(); ++c) L.useful_pts += L.lv[lvl[c]].npts;
+    int rc = make_units(W, L.chunk, opt.x_splits, L.units, L.pos, L.nsp, L.part_tiles);
+    if (rc) return rc;
+    L.items.swap(W.items);
+    L.tws.swap(W.tws);
+    if ((rc = check_lists(L, d.contr.size()))) return rc;
+
+    // Binning records, sorted by bin.
+    L.recs.resize(d.contr.size());
+    L.seg.assign(d.nbins + 1, 0);
+    for (const Contraction &k : d.contr) L.seg[k.bin + 1]++;
+    for (int b = 0; b < d.nbins; ++b) L.seg[b + 1] += L.seg[b];
+    {
+        std::vector<int32_t> at(L.seg.begin(), L.seg.end() - 1);
+        for (size_t c = 0; c < d.contr.size(); ++c) {
+            const Contraction &k = d.contr[c];
+            BinRec R;
+            std::memset(&R, 0, sizeof R);
+            R.p1 = k.p1; R.p2 = k.p2; R.p3 = k.p3;
+            R.r1 = k.r1; R.r2 = k.r2; R.r3 = k.r3;
+            R.nsp = L.nsp[c];
+            R.rp = k.rp;
+            R.pos = L.pos[c];
+            R.f1 = k.p1 * Nr + k.r1;
+            R.f2 = std::abs(k.p2) * Nr + k.r2;
+            R.f3 = std::abs(k.p3) * Nr + k.r3;
+            L.recs[at[k.bin]++] = R;
+        }
+    }
+    L.bp.nbins = d.nbins;
+    L.bp.n_mtbin = d.g.n_mtbin;
+    L.bp.NN = NN;
+    L.bp.Cunits = 2. * M_PI / (NN * d.g.LL);
+    L.bp.ff = d.g.ff;
+    L.bp.beta = d.g.beta;
+    return BB_OK;
+}
+
+// The layout of (plan, precision, options), built once and cached in the plan.
+int get_layout(const bb_plan *p, CPrec prec, const bb_options &opt,
+               std::shared_ptr<const Layout> &out)
+{
+    const int waves = layout_waves(), tww = layout_tww(prec), lev = default_levels(opt);
+    const std::string key = std::to_string((int)prec) + "/" + std::to_string(opt.full_grid != 0) +
+                            "/" + std::to_string(opt.x_splits) + "/" + std::to_string(lev) + "/" +
+                            std::to_string(waves) + "/" + std::to_string(tww);
+    bb_plan *mp = const_cast<bb_plan *>(p);
+    {
+        std::lock_guard<std::mutex> g(mp->mu);
+        auto it = mp->layouts.find(key);
+        if (it != mp->layouts.end()) {
+            out = it->second;
+            return BB_OK;
+        }
+    }
+    auto L = std::make_shared<Layout>();
+    int rc = build_layout(p->d, prec, opt, waves, tww, lev, *L);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g(mp->mu);
+    auto ins = mp->layouts.emplace(key, std::shared_ptr<const Layout>(L));
+    out = ins.first->second;
+    return BB_OK;
 }
 
 }  // namespace
@@ -321,48 +744,32 @@ struct bb_estimator {
     bb_options opt{};
     CPrec cprec = CPrec::F64;
     hipStream_t stream = nullptr;
-    FftDesc fd{};
-    int NN = 0, NG = 0, kmax = 0;
-    double scale = 1.0;            // (NN/NG)^2
-    long long npts = 0;            // NG^2
-    std::vector<int> rlo, rhi, base;  // per plane ring range and first slot
-    int nfields = 0;
+    std::shared_ptr<const Layout> L;
+    std::vector<FftDesc> fd;          // per level
+    std::vector<double2 *> d_tw;      // per level
+    std::vector<int> task0;           // per level: first task in d_tasks
     std::vector<void *> owned;
-    double2 *d_tw = nullptr;
-    double2 *d_field = nullptr;    // staging for bb_estimate (host input)
-    double2 *d_planes = nullptr;
+    double2 *d_field = nullptr;       // staging for bb_estimate (host input)
+    double2 *d_planes = nullptr;      // NGtop x NGtop per plane
     void *d_cache = nullptr;
-    void *d_split = nullptr;       // BF16x3: bf16 hi/lo operands of every field
-    double2 *d_rowbuf = nullptr;   // FP32 caches: FP64 row-pass scratch
-    int rowbuf_fields = 0;
+    void *d_split = nullptr;          // BF16x3: bf16 hi/lo operands of every field
+    double2 *d_rowbuf = nullptr;      // FP32 caches: FP64 row-pass scratch
+    long long rowbuf_pts = 0;
     FieldTask *d_tasks = nullptr;
-    int ntasks = 0;
-    int max_rows = 0;
     WorkItem *d_items = nullptr;
     TWin *d_tws = nullptr;
-    int nitems = 0, nsplit = 1, chunk = 32, max_fields = 0, waves = kWaves, tww = kTWW;
-    double wave_balance = 1.0;
-    long long staged_rows = 0;
-    long long tile_slots = 0;
-    double *d_part = nullptr;      // [nsplit][tile_slots]
-    std::vector<int32_t> pos;      // contraction -> tile position
+    WorkUnit *d_units = nullptr;
+    double *d_part = nullptr;         // partial tiles
     BinRec *d_recs = nullptr;
     int32_t *d_seg = nullptr;
     double *d_num = nullptr;
     std::vector<double> num_host;
-    int *d_nz = nullptr;
+    int *d_nz = nullptr;              // [Nmax * Nr]
     double *d_avg = nullptr;
     double *d_bins = nullptr;
-    BinParams bp{};
     hipEvent_t ev[5] = {};
     float ms[4] = {0, 0, 0, 0};
 
-    int fid(int plane, int ring) const
-    {
-        if (plane < 0 || plane >= (int)rlo.size() || rlo[plane] < 0) return -1;
-        if (ring < rlo[plane] || ring > rhi[plane]) return -1;
-        return base[plane] + ring - rlo[plane];
-    }
     ~bb_estimator()
     {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -381,58 +788,183 @@ int upload(void *dst, const void *src, size_t bytes, hipStream_t s)
     return BB_OK;
 }
 
-// FFT pipeline for a list of tasks into a cache (FP64 in place, or FP32 via
-// the FP64 row-pass scratch).
-int run_fields(bb_estimator *E, const double2 *planes, const FieldTask *d_tasks, int ntasks,
-               int max_rows, void *cache, bool f32, int *nz)
+std::vector<double2> twiddles(int NG)
 {
-    const long long NG2 = E->npts;
-    if (!f32) {
-        HIPCHK(launch_rowpass(planes, (double2 *)cache, NG2, d_tasks, ntasks, max_rows, E->fd,
-                              E->d_tw, nz, E->stream));
-        HIPCHK(launch_colpass_f64((double2 *)cache, NG2, (double2 *)cache, d_tasks, ntasks, E->fd,
-                                  E->d_tw, E->stream));
+    // exp(+2 pi i t / NG), FP64
+    std::vector<double2> tw(NG);
+    for (int t = 0; t < NG; ++t) {
+        long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)t / NG;
+        tw[t] = make_double2((double)cosl(a), (double)sinl(a));
+    }
+    return tw;
+}
+
+// FFTs of one level's fields into its part of the cache: FP64 in place, or
+// FP32 (plus bf16 operands) through the FP64 row-pass scratch.
+int run_level(bb_estimator *E, int l, const double2 *planes, int NP, int *nz)
+{
+    const Level &V = E->L->lv[l];
+    const int nt = (int)V.tasks.size();
+    if (nt == 0) return BB_OK;
+    const FieldTask *t = E->d_tasks + E->task0[l];
+    if (E->cprec == CPrec::F64) {
+        double2 *c = (double2 *)E->d_cache + V.off;
+        HIPCHK(launch_rowpass(planes, NP, c, V.stride, t, nt, V.max_rows, E->fd[l], E->d_tw[l],
+                              nz, E->stream));
+        HIPCHK(launch_colpass_f64(c, V.stride, c, V.stride, t, nt, E->fd[l], E->d_tw[l],
+                                  E->stream));
         return BB_OK;
     }
-    for (int t0 = 0; t0 < ntasks; t0 += E->rowbuf_fields) {
-        int nb = std::min(E->rowbuf_fields, ntasks - t0);
-        HIPCHK(launch_rowpass(planes, E->d_rowbuf, -NG2, d_tasks + t0, nb, max_rows, E->fd,
-                              E->d_tw, nz, E->stream));
-        HIPCHK(launch_colpass_f32(E->d_rowbuf, -NG2, (float2 *)cache, E->d_split, d_tasks + t0,
-                                  nb, E->fd, E->d_tw, E->stream));
+    float2 *c = (float2 *)E->d_cache + V.off;
+    void *sp = E->d_split ? (void *)((char *)E->d_split + V.off * 8) : nullptr;
+    const int per = (int)std::max<long long>(1, E->rowbuf_pts / V.npts);
+    for (int t0 = 0; t0 < nt; t0 += per) {
+        const int nb = std::min(per, nt - t0);
+        HIPCHK(launch_rowpass(planes, NP, E->d_rowbuf, -V.npts, t + t0, nb, V.max_rows, E->fd[l],
+                              E->d_tw[l], nz, E->stream));
+        HIPCHK(launch_colpass_f32(E->d_rowbuf, -V.npts, c, sp, V.stride, t + t0, nb, E->fd[l],
+                                  E->d_tw[l], E->stream));
     }
     return BB_OK;
 }
 
-int choose_grid(bb_estimator *E, const PlanData &d)
+// Triangle counts Num on the GPU (calCR, funcs.c:328-424): unit fields
+// I_r = IFFT2(mask_r) in FP64 on the top grid, then Num = sum_x I1 I2 I3 for
+// every ring triple with the FP64 contraction kernel.  Cached in the plan.
+int compute_num(bb_estimator *E)
 {
-    const int NN = d.g.NN;
-    int kmax = 0;
-    for (int r = 0; r < d.g.Nr; ++r)
-        if (d.nhi[r] > 0) kmax = std::max(kmax, isqrt_floor(d.nhi[r] - 1));
-    E->kmax = kmax;
-    const int need = 3 * kmax + 1;
-    E->NG = NN;
-    if (!E->opt.full_grid && NN >= need) {
-        for (int n = need; n <= NN; ++n)
-            if ((n % 2 == 0) && smooth235(n)) {
-                E->NG = n;
-                break;
-            }
+    bb_plan *mp = const_cast<bb_plan *>(E->plan);
+    {
+        std::lock_guard<std::mutex> g(mp->mu);
+        if (mp->have_num) {
+            E->num_host = mp->num;
+            return BB_OK;
+        }
     }
-    E->scale = ((double)NN * NN) / ((double)E->NG * E->NG);
-    E->npts = (long long)E->NG * E->NG;
+    const PlanData &d = E->plan->d;
+    const int Nr = d.g.Nr, NN = d.g.NN, NG = E->L->NGtop;
+    const int rcount = (int)(d.tri.size() / 3);
+    const std::vector<int> K = ring_kx(d);
+    Level U;
+    U.NG = NG;
+    U.npts = (long long)NG * NG;
+    U.stride = (U.npts + 31) / 32 * 32;
+    U.scale = ((double)NN * NN) / ((double)NG * NG);
+    U.rlo.assign(1, 0);
+    U.rhi.assign(1, Nr - 1);
+    U.base.assign(1, 0);
+    U.nfields = Nr;
+    for (int r = 0; r < Nr; ++r) {
+        const int band = d.nhi[r] > 0 ? std::min(NG / 2, K[r]) : -1;
+        U.tasks.push_back(FieldTask{-1, r, r, band, d.nlo[r], d.nhi[r], 0});
+        U.max_rows = std::max(U.max_rows, std::min(NG, 2 * band + 1));
+    }
+    Layout L;
+    L.chunk = contract_chunk(CPrec::F64);
+    L.cache_pts = (long long)Nr * U.stride;
+    std::vector<CItem> cs;
+    for (int rp = 0; rp < rcount; ++rp)
+        cs.push_back({d.tri[3 * rp], d.tri[3 * rp + 1], d.tri[3 * rp + 2], rp});
+    WorkLists W;
+    L.pos.assign(std::max(rcount, 1), 0);
+    std::vector<PairBlock> blocks;
+    if (rcount > 0) pair_blocks(cs, 0, 0, 0, 0, blocks);
+    int rc = pack_items(blocks, U, false, contract_tww(CPrec::F64), kWaves, W, L.pos);
+    if (!rc) rc = make_units(W, L.chunk, 0, L.units, L.pos, L.nsp, L.part_tiles);
+    if (rc) return rc;
+    L.items.swap(W.items);
+    L.tws.swap(W.tws);
+    if ((rc = check_lists(L, rcount))) return rc;
+    FftDesc fd;
+    if (!make_fft_desc(NG, fd)) return set_error(BB_E_INVALID, "unsupported FFT length");
+    const std::vector<double2> tw = twiddles(NG);
+    const long long uslots = std::max<long long>(L.part_tiles, 1) * 256;
+    std::vector<void *> tmp;
+    FieldTask *d_ut;
+    double2 *d_unit, *d_utw;
+    WorkItem *d_uitems;
+    WorkUnit *d_uunits;
+    TWin *d_utws;
+    double *d_upart;
+    rc = dev_alloc(&d_ut, Nr, tmp);
+    if (!rc) rc = dev_alloc(&d_utw, NG, tmp);
+    if (!rc) rc = dev_alloc(&d_unit, (size_t)L.cache_pts, tmp);
+    if (!rc && hipMemsetAsync(d_unit, 0, sizeof(double2) * L.cache_pts, E->stream) != hipSuccess)
+        rc = set_error(BB_E_DEVICE, "unit-field clear failed");
+    if (!rc) rc = dev_alloc(&d_uitems, L.items.size(), tmp);
+    if (!rc) rc = dev_alloc(&d_uunits, L.units.size(), tmp);
+    if (!rc) rc = dev_alloc(&d_utws, L.tws.size(), tmp);
+    if (!rc) rc = dev_alloc(&d_upart, (size_t)uslots, tmp);
+    if (!rc) rc = upload(d_ut, U.tasks.data(), sizeof(FieldTask) * Nr, E->stream);
+    if (!rc) rc = upload(d_utw, tw.data(), sizeof(double2) * NG, E->stream);
+    if (!rc && !L.items.empty())
+        rc = upload(d_uitems, L.items.data(), sizeof(WorkItem) * L.items.size(), E->stream);
+    if (!rc && !L.units.empty())
+        rc = upload(d_uunits, L.units.data(), sizeof(WorkUnit) * L.units.size(), E->stream);
+    if (!rc && !L.tws.empty())
+        rc = upload(d_utws, L.tws.data(), sizeof(TWin) * L.tws.size(), E->stream);
+    if (!rc) {
+        hipError_t e = launch_rowpass(nullptr, NG, d_unit, U.stride, d_ut, Nr, U.max_rows, fd,
+                                      d_utw, nullptr, E->stream);
+        if (e == hipSuccess)
+            e = launch_colpass_f64(d_unit, U.stride, d_unit, U.stride, d_ut, Nr, fd, d_utw,
+                                   E->stream);
+        if (e == hipSuccess && rcount > 0)
+            e = launch_contract(CPrec::F64, kWaves, contract_tww(CPrec::F64), d_unit, nullptr,
+                                d_uitems, d_uunits, (int)L.units.size(), d_utws, d_upart, E->stream);
+        if (e != hipSuccess) rc = set_error(BB_E_DEVICE, hipGetErrorString(e));
+    }
+    std::vector<double> part((size_t)uslots);
+    if (!rc && hipMemcpyAsync(part.data(), d_upart, sizeof(double) * part.size(),
+                              hipMemcpyDeviceToHost, E->stream) != hipSuccess)
+        rc = set_error(BB_E_DEVICE, "triangle-count copy failed");
+    if (!rc && hipStreamSynchronize(E->stream) != hipSuccess)
+        rc = set_error(BB_E_DEVICE, "triangle-count kernels failed");
+    for (void *q : tmp) (void)hipFree(q);
+    if (rc) return rc;
+    E->num_host.assign(std::max(rcount, 1), 0.0);
+    for (int rp = 0; rp < rcount; ++rp) {
+        double v = 0.0;
+        for (int s = 0; s < L.nsp[rp]; ++s) v += part[(size_t)L.pos[rp] + 256 * (size_t)s];
+        E->num_host[rp] = v;
+    }
+    std::lock_guard<std::mutex> g(mp->mu);
+    mp->num = E->num_host;
+    mp->have_num = true;
     return BB_OK;
+}
+
+// Device bytes held by an estimator of layout L (steady state).
+size_t layout_bytes(const PlanData &d, const Layout &L)
+{
+    const size_t NN = d.g.NN, Nr = d.g.Nr, Nmax = d.g.Nmax;
+    const size_t top2 = (size_t)L.NGtop * L.NGtop;
+    size_t b = NN * NN * (NN / 2 + 1) * 16;                  // bb_estimate staging
+    b += Nmax * top2 * 16;                                   // gathered planes
+    b += (size_t)L.cache_pts * (L.prec == CPrec::F64 ? 16 : 8);
+    if (L.prec == CPrec::BF16X3) b += (size_t)L.cache_pts * 8;
+    if (L.prec != CPrec::F64) b += 128 * top2 * 16;          // row-pass scratch
+    size_t tw = 0, nt = 0;
+    for (const Level &V : L.lv) {
+        tw += V.NG;
+        nt += V.tasks.size();
+    }
+    b += tw * 16 + nt * sizeof(FieldTask);
+    b += L.items.size() * sizeof(WorkItem) + L.tws.size() * sizeof(TWin) +
+         L.units.size() * sizeof(WorkUnit);
+    b += (size_t)std::max<long long>(L.part_tiles, 1) * 256 * 8;
+    b += L.recs.size() * sizeof(BinRec) + L.seg.size() * 4;
+    b += d.tri.size() / 3 * 8 + Nmax * Nr * 4 + Nr * 8 + (size_t)3 * d.nbins * BB_NPOLES * 8;
+    return b;
 }
 
 int estimator_init(bb_estimator *E)
 {
     const PlanData &d = E->plan->d;
-    const int NN = d.g.NN, Nmax = d.g.Nmax, Nr = d.g.Nr;
-    E->NN = NN;
-    choose_grid(E, d);
-    const int NG = E->NG;
-    if (!make_fft_desc(NG, E->fd)) return set_error(BB_E_INVALID, "unsupported FFT length");
+    const int Nmax = d.g.Nmax, Nr = d.g.Nr;
+    int rc = get_layout(E->plan, E->cprec, E->opt, E->L);
+    if (rc) return rc;
+    const Layout &L = *E->L;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
         (void)hipGetLastError();
@@ -440,223 +972,70 @@ int estimator_init(bb_estimator *E)
     }
     HIPCHK(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
     for (auto &e : E->ev) HIPCHK(hipEventCreate(&e));
-    int rc;
-    const bool f32 = (E->cprec != CPrec::F64);
+    if ((rc = compute_num(E))) return rc;
+    if ((rc = dev_alloc(&E->d_num, E->num_host.size(), E->owned))) return rc;
+    if ((rc = upload(E->d_num, E->num_host.data(), sizeof(double) * E->num_host.size(), E->stream)))
+        return rc;
 
-    // Twiddles exp(+2 pi i t / NG), FP64.
-    std::vector<double2> tw(NG);
-    for (int t = 0; t < NG; ++t) {
-        long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)t / NG;
-        tw[t] = make_double2((double)cosl(a), (double)sinl(a));
-    }
-    if ((rc = dev_alloc(&E->d_tw, NG, E->owned))) return rc;
-    if ((rc = upload(E->d_tw, tw.data(), sizeof(double2) * NG, E->stream))) return rc;
-
-    std::vector<int> rowband(Nr);
-    for (int r = 0; r < Nr; ++r)
-        rowband[r] = d.nhi[r] > 0 ? std::min(NG / 2, isqrt_floor(d.nhi[r] - 1)) : -1;
-
-    // --- Triangle counts Num on the GPU (calCR, funcs.c:328-424) -----------
-    // Unit fields I_r = IFFT2(mask_r) in FP64, then Num = sum_x I1 I2 I3 for
-    // every ring triple with the FP64 contraction kernel.
-    {
-        const int rcount = (int)(d.tri.size() / 3);
-        std::vector<FieldTask> ut(Nr);
-        int mr = 0;
-        for (int r = 0; r < Nr; ++r) {
-            ut[r] = FieldTask{-1, r, r, rowband[r], d.nlo[r], d.nhi[r]};
-            mr = std::max(mr, std::min(NG, 2 * rowband[r] + 1));
-        }
-        std::vector<CItem> cs;
-        for (int rp = 0; rp < rcount; ++rp)
-            cs.push_back({d.tri[3 * rp], d.tri[3 * rp + 1], d.tri[3 * rp + 2], rp});
-        WorkLists W;
-        std::vector<int32_t> upos(std::max(rcount, 1), 0);
-        std::vector<PairBlock> blocks;
-        if (rcount > 0) pair_blocks(cs, 0, 0, 0, 0, blocks);
-        rc = pack_items(blocks, false, contract_tww(CPrec::F64), kWaves,
-                        [](int, int r) { return r; }, W, upos);
-        if (!rc) rc = check_lists(W, upos, rcount);
-        if (rc) return rc;
-        const int usplit = auto_splits((int)W.items.size());
-        const long long uslots = std::max<long long>(W.tile_slots, 1);
-        std::vector<void *> tmp;
-        FieldTask *d_ut;
-        double2 *d_unit;
-        WorkItem *d_uitems;
-        TWin *d_utw;
-        double *d_upart;
-        rc = dev_alloc(&d_ut, Nr, tmp);
-        if (!rc) rc = dev_alloc(&d_unit, (size_t)Nr * E->npts, tmp);
-        if (!rc) rc = dev_alloc(&d_uitems, W.items.size(), tmp);
-        if (!rc) rc = dev_alloc(&d_utw, W.tws.size(), tmp);
-        if (!rc) rc = dev_alloc(&d_upart, (size_t)usplit * uslots, tmp);
-        if (!rc) rc = upload(d_ut, ut.data(), sizeof(FieldTask) * Nr, E->stream);
-        if (!rc && !W.items.empty())
-            rc = upload(d_uitems, W.items.data(), sizeof(WorkItem) * W.items.size(), E->stream);
-        if (!rc && !W.tws.empty())
-            rc = upload(d_utw, W.tws.data(), sizeof(TWin) * W.tws.size(), E->stream);
-        if (!rc) rc = run_fields(E, nullptr, d_ut, Nr, mr, d_unit, false, nullptr);
-        if (!rc && rcount > 0) {
-            hipError_t e = launch_contract(CPrec::F64, kWaves, contract_tww(CPrec::F64), d_unit,
-                                           nullptr, E->npts, d_uitems,
-                                           (int)W.items.size(), d_utw, usplit, W.max_rows,
-                                           d_upart, uslots, E->scale, E->stream);
-            if (e != hipSuccess) rc = set_error(BB_E_DEVICE, hipGetErrorString(e));
-        }
-        std::vector<double> part((size_t)usplit * uslots);
-        if (!rc && hipMemcpyAsync(part.data(), d_upart, sizeof(double) * part.size(),
-                                  hipMemcpyDeviceToHost, E->stream) != hipSuccess)
-            rc = set_error(BB_E_DEVICE, "triangle-count copy failed");
-        if (!rc && hipStreamSynchronize(E->stream) != hipSuccess)
-            rc = set_error(BB_E_DEVICE, "triangle-count kernels failed");
-        for (void *p : tmp) (void)hipFree(p);
-        if (rc) return rc;
-        E->num_host.assign(std::max(rcount, 1), 0.0);
-        for (int rp = 0; rp < rcount; ++rp) {
-            double v = 0.0;
-            for (int s = 0; s < usplit; ++s) v += part[(size_t)s * uslots + upos[rp]];
-            E->num_host[rp] = v;
-        }
-        if ((rc = dev_alloc(&E->d_num, E->num_host.size(), E->owned))) return rc;
-        if ((rc = upload(E->d_num, E->num_host.data(), sizeof(double) * E->num_host.size(),
-                         E->stream)))
-            return rc;
-    }
-
-    // --- Field layout: per plane the contiguous range of rings in use ------
-    E->rlo.assign(Nmax, -1);
-    E->rhi.assign(Nmax, -2);
-    E->base.assign(Nmax, 0);
-    for (int p = 0; p < Nmax; ++p)
-        for (int r = 0; r < Nr; ++r)
-            if (d.field_used[(size_t)p * Nr + r]) {
-                if (E->rlo[p] < 0) E->rlo[p] = r;
-                E->rhi[p] = r;
-            }
-    E->nfields = 0;
+    // Per level: FFT plan, twiddles, tasks.
+    const int nl = (int)L.lv.size();
+    E->fd.resize(nl);
+    E->d_tw.resize(nl);
+    E->task0.resize(nl);
     std::vector<FieldTask> tasks;
-    for (int p = 0; p < Nmax; ++p) {
-        E->base[p] = E->nfields;
-        if (E->rlo[p] < 0) continue;
-        for (int r = E->rlo[p]; r <= E->rhi[p]; ++r) {
-            tasks.push_back(FieldTask{p, r, E->nfields, rowband[r], d.nlo[r], d.nhi[r]});
-            E->max_rows = std::max(E->max_rows, std::min(NG, 2 * rowband[r] + 1));
-            E->nfields++;
-        }
+    for (int l = 0; l < nl; ++l) {
+        const Level &V = L.lv[l];
+        if (!make_fft_desc(V.NG, E->fd[l])) return set_error(BB_E_INVALID, "unsupported FFT length");
+        const std::vector<double2> tw = twiddles(V.NG);
+        if ((rc = dev_alloc(&E->d_tw[l], V.NG, E->owned))) return rc;
+        if ((rc = upload(E->d_tw[l], tw.data(), sizeof(double2) * V.NG, E->stream))) return rc;
+        E->task0[l] = (int)tasks.size();
+        tasks.insert(tasks.end(), V.tasks.begin(), V.tasks.end());
     }
-    E->ntasks = (int)tasks.size();
-
-    // --- Contraction work lists ------------------------------------------------
-    // Pair blocks per plane group; groups with the same plane set
-    // {p1, |p2|, |p3|} are adjacent so that work items share staged rings.
-    std::map<std::pair<int, int>, std::vector<CItem>> groups;
-    for (size_t c = 0; c < d.contr.size(); ++c) {
-        const Contraction &k = d.contr[c];
-        groups[{k.p1, k.p2}].push_back({k.r1, k.r2, k.r3, (int)c});
-    }
-    std::map<std::array<int, 3>, std::vector<std::pair<int, int>>> by_set;
-    for (auto &kv : groups) {
-        const int p1 = kv.first.first, p2 = kv.first.second, p3 = -p1 - p2;
-        std::array<int, 3> key{p1, std::abs(p2), std::abs(p3)};
-        std::sort(key.begin(), key.end());
-        by_set[key].push_back(kv.first);
-    }
-    std::vector<PairBlock> blocks;
-    for (auto &kv : by_set)
-        for (auto &gk : kv.second) {
-            const int p1 = gk.first, p2 = gk.second, p3 = -p1 - p2;
-            const int flags = (p2 < 0 ? TW_CONJ2 : 0) | (p3 < 0 ? TW_CONJ3 : 0);
-            pair_blocks(groups[gk], p1, std::abs(p2), std::abs(p3), flags, blocks);
-        }
-    WorkLists W;
-    E->pos.assign(std::max<size_t>(d.contr.size(), 1), 0);
-    E->waves = kWaves;
-    if (const char *ev = std::getenv("BARBIE_CONTRACT_WAVES")) {
-        const int w = std::atoi(ev);
-        if (w == 4 || w == 8) E->waves = w;
-    }
-    E->tww = contract_tww(E->cprec);
-    if (const char *ev = std::getenv("BARBIE_CONTRACT_TWW")) {
-        const int t = std::atoi(ev);
-        if (E->cprec == CPrec::F64 && (t == kTWW || t == kTWWF64Narrow)) E->tww = t;
-    }
-    rc = pack_items(blocks, E->cprec == CPrec::BF16X3, E->tww, E->waves,
-                    [&](int plane, int ring) { return E->fid(plane, ring); }, W, E->pos);
-    E->wave_balance = W.tw_slots ? (double)W.tw_used / (double)W.tw_slots : 1.0;
-    if (!rc) rc = check_lists(W, E->pos, d.contr.size());
-    if (rc) return rc;
-    E->nitems = (int)W.items.size();
-    E->max_fields = std::max(W.max_rows, 1);
-    for (const WorkItem &it : W.items) E->staged_rows += it.nrows;
-    E->tile_slots = std::max<long long>(W.tile_slots, 1);
-    E->nsplit = E->opt.x_splits > 0 ? E->opt.x_splits : auto_splits(E->nitems);
-    E->chunk = contract_chunk(E->cprec);
-
-    // --- Binning records, sorted by bin --------------------------------------
-    std::vector<BinRec> recs(d.contr.size());
-    std::vector<int32_t> seg(d.nbins + 1, 0);
-    for (const Contraction &k : d.contr) seg[k.bin + 1]++;
-    for (int b = 0; b < d.nbins; ++b) seg[b + 1] += seg[b];
-    {
-        std::vector<int32_t> at(seg.begin(), seg.end() - 1);
-        for (size_t c = 0; c < d.contr.size(); ++c) {
-            const Contraction &k = d.contr[c];
-            BinRec R;
-            R.p1 = k.p1; R.p2 = k.p2; R.p3 = k.p3;
-            R.r1 = k.r1; R.r2 = k.r2; R.r3 = k.r3;
-            R.rp = k.rp;
-            R.pos = E->pos[c];
-            R.f1 = E->fid(k.p1, k.r1);
-            R.f2 = E->fid(std::abs(k.p2), k.r2);
-            R.f3 = E->fid(std::abs(k.p3), k.r3);
-            recs[at[k.bin]++] = R;
-        }
-    }
-    E->bp.nbins = d.nbins;
-    E->bp.n_mtbin = d.g.n_mtbin;
-    E->bp.NN = NN;
-    E->bp.nsplit = E->nsplit;
-    E->bp.split_stride = E->tile_slots;
-    E->bp.Cunits = 2. * M_PI / (NN * d.g.LL);
-    E->bp.ff = d.g.ff;
-    E->bp.beta = d.g.beta;
 
     // --- Device buffers --------------------------------------------------------
-    if ((rc = dev_alloc(&E->d_planes, (size_t)Nmax * E->npts, E->owned))) return rc;
-    const size_t nf = (size_t)std::max(E->nfields, 1);
-    if (f32) {
+    const size_t top2 = (size_t)L.NGtop * L.NGtop;
+    const size_t cpts = (size_t)std::max<long long>(L.cache_pts, 1);
+    if ((rc = dev_alloc(&E->d_planes, (size_t)Nmax * top2, E->owned))) return rc;
+    if (E->cprec != CPrec::F64) {
         float2 *c;
-        if ((rc = dev_alloc(&c, nf * E->npts, E->owned))) return rc;
+        if ((rc = dev_alloc(&c, cpts, E->owned))) return rc;
         E->d_cache = c;
-        E->rowbuf_fields = std::max(1, std::min(E->ntasks, 128));
         if (E->cprec == CPrec::BF16X3) {
             float2 *sp;   // 8 bytes per point, like the FP32 values
-            if ((rc = dev_alloc(&sp, nf * E->npts, E->owned))) return rc;
+            if ((rc = dev_alloc(&sp, cpts, E->owned))) return rc;
             E->d_split = sp;
         }
-        if ((rc = dev_alloc(&E->d_rowbuf, (size_t)E->rowbuf_fields * E->npts, E->owned))) return rc;
+        E->rowbuf_pts = (long long)std::max<size_t>(1, std::min<size_t>(tasks.size(), 128)) * top2;
+        if ((rc = dev_alloc(&E->d_rowbuf, (size_t)E->rowbuf_pts, E->owned))) return rc;
     } else {
         double2 *c;
-        if ((rc = dev_alloc(&c, nf * E->npts, E->owned))) return rc;
+        if ((rc = dev_alloc(&c, cpts, E->owned))) return rc;
         E->d_cache = c;
     }
+    const size_t nslots = (size_t)std::max<long long>(L.part_tiles, 1) * 256;
     if ((rc = dev_alloc(&E->d_tasks, tasks.size(), E->owned))) return rc;
-    if ((rc = dev_alloc(&E->d_items, W.items.size(), E->owned))) return rc;
-    if ((rc = dev_alloc(&E->d_tws, W.tws.size(), E->owned))) return rc;
-    if ((rc = dev_alloc(&E->d_part, (size_t)E->nsplit * E->tile_slots, E->owned))) return rc;
-    if ((rc = dev_alloc(&E->d_recs, recs.size(), E->owned))) return rc;
-    if ((rc = dev_alloc(&E->d_seg, seg.size(), E->owned))) return rc;
-    if ((rc = dev_alloc(&E->d_nz, nf, E->owned))) return rc;
+    if ((rc = dev_alloc(&E->d_items, L.items.size(), E->owned))) return rc;
+    if ((rc = dev_alloc(&E->d_tws, L.tws.size(), E->owned))) return rc;
+    if ((rc = dev_alloc(&E->d_units, L.units.size(), E->owned))) return rc;
+    if ((rc = dev_alloc(&E->d_part, nslots, E->owned))) return rc;
+    if ((rc = dev_alloc(&E->d_recs, L.recs.size(), E->owned))) return rc;
+    if ((rc = dev_alloc(&E->d_seg, L.seg.size(), E->owned))) return rc;
+    if ((rc = dev_alloc(&E->d_nz, (size_t)Nmax * Nr, E->owned))) return rc;
     if ((rc = dev_alloc(&E->d_avg, Nr, E->owned))) return rc;
     if ((rc = dev_alloc(&E->d_bins, (size_t)3 * d.nbins * BB_NPOLES, E->owned))) return rc;
     if (!tasks.empty() && (rc = upload(E->d_tasks, tasks.data(), sizeof(FieldTask) * tasks.size(), E->stream))) return rc;
-    if (!W.items.empty() && (rc = upload(E->d_items, W.items.data(), sizeof(WorkItem) * W.items.size(), E->stream))) return rc;
-    if (!W.tws.empty() && (rc = upload(E->d_tws, W.tws.data(), sizeof(TWin) * W.tws.size(), E->stream))) return rc;
-    if (!recs.empty() && (rc = upload(E->d_recs, recs.data(), sizeof(BinRec) * recs.size(), E->stream))) return rc;
-    if ((rc = upload(E->d_seg, seg.data(), sizeof(int32_t) * seg.size(), E->stream))) return rc;
+    if (!L.items.empty() && (rc = upload(E->d_items, L.items.data(), sizeof(WorkItem) * L.items.size(), E->stream))) return rc;
+    if (!L.tws.empty() && (rc = upload(E->d_tws, L.tws.data(), sizeof(TWin) * L.tws.size(), E->stream))) return rc;
+    if (!L.units.empty() && (rc = upload(E->d_units, L.units.data(), sizeof(WorkUnit) * L.units.size(), E->stream))) return rc;
+    if (!L.recs.empty() && (rc = upload(E->d_recs, L.recs.data(), sizeof(BinRec) * L.recs.size(), E->stream))) return rc;
+    if ((rc = upload(E->d_seg, L.seg.data(), sizeof(int32_t) * L.seg.size(), E->stream))) return rc;
     if ((rc = upload(E->d_avg, d.avg.data(), sizeof(double) * Nr, E->stream))) return rc;
-    HIPCHK(hipMemsetAsync(E->d_part, 0, sizeof(double) * E->nsplit * E->tile_slots, E->stream));
+    HIPCHK(hipMemsetAsync(E->d_part, 0, sizeof(double) * nslots, E->stream));
+    // Padding points between fields are read by the contraction and never
+    // written by the transforms: zero them (and everything else) once.
+    HIPCHK(hipMemsetAsync(E->d_cache, 0, cpts * (E->cprec == CPrec::F64 ? 16 : 8), E->stream));
+    if (E->d_split) HIPCHK(hipMemsetAsync(E->d_split, 0, cpts * 8, E->stream));
     HIPCHK(hipStreamSynchronize(E->stream));
     return BB_OK;
 }
@@ -665,22 +1044,24 @@ int estimate_device(bb_estimator *E, const double *d_field_in, double *d_bs, dou
                     double *d_biana)
 {
     const PlanData &d = E->plan->d;
-    const bool f32 = (E->cprec != CPrec::F64);
+    const Layout &L = *E->L;
     hipStream_t s = E->stream;
     HIPCHK(hipEventRecord(E->ev[0], s));
-    const int nf = std::max(E->nfields, 1);
-    hipLaunchKernelGGL(k_fill_int, dim3((nf + 255) / 256), dim3(256), 0, s, E->d_nz, nf, 0);
+    const int nz = d.g.Nmax * d.g.Nr;
+    hipLaunchKernelGGL(k_fill_int, dim3((nz + 255) / 256), dim3(256), 0, s, E->d_nz, nz, 0);
     HIPCHK(hipGetLastError());
-    HIPCHK(launch_gather((const double2 *)d_field_in, E->d_planes, E->NN, E->NG, E->kmax,
+    HIPCHK(launch_gather((const double2 *)d_field_in, E->d_planes, L.NN, L.NGtop, L.kmax,
                          d.g.Nmax, s));
     HIPCHK(hipEventRecord(E->ev[1], s));
-    int rc = run_fields(E, E->d_planes, E->d_tasks, E->ntasks, E->max_rows, E->d_cache, f32, E->d_nz);
-    if (rc) return rc;
+    for (int l = 0; l < (int)L.lv.size(); ++l) {
+        int rc = run_level(E, l, E->d_planes, L.NGtop, E->d_nz);
+        if (rc) return rc;
+    }
     HIPCHK(hipEventRecord(E->ev[2], s));
-    HIPCHK(launch_contract(E->cprec, E->waves, E->tww, E->d_cache, E->d_split, E->npts, E->d_items, E->nitems, E->d_tws,
-                           E->nsplit, E->max_fields, E->d_part, E->tile_slots, E->scale, s));
+    HIPCHK(launch_contract(E->cprec, L.waves, L.tww, E->d_cache, E->d_split, E->d_items,
+                           E->d_units, (int)L.units.size(), E->d_tws, E->d_part, s));
     HIPCHK(hipEventRecord(E->ev[3], s));
-    HIPCHK(launch_binning(E->d_seg, E->d_recs, E->d_part, E->d_num, E->d_nz, E->d_avg, E->bp,
+    HIPCHK(launch_binning(E->d_seg, E->d_recs, E->d_part, E->d_num, E->d_nz, E->d_avg, L.bp,
                           d_bs, d_norm, d_biana, s));
     HIPCHK(hipEventRecord(E->ev[4], s));
     HIPCHK(hipStreamSynchronize(s));
@@ -688,40 +1069,76 @@ int estimate_device(bb_estimator *E, const double *d_field_in, double *d_bs, dou
     return BB_OK;
 }
 
-size_t cache_fields(const PlanData &d)
+int precision_of(bb_precision p, CPrec &cp)
 {
-    size_t nf = 0;
-    for (int pl = 0; pl < d.g.Nmax; ++pl) {
-        int lo = -1, hi = -2;
-        for (int r = 0; r < d.g.Nr; ++r)
-            if (d.field_used[(size_t)pl * d.g.Nr + r]) {
-                if (lo < 0) lo = r;
-                hi = r;
-            }
-        if (lo >= 0) nf += (size_t)(hi - lo + 1);
+    switch (p) {
+    case BB_FP64: cp = CPrec::F64; return BB_OK;
+    case BB_FP32: cp = CPrec::F32; return BB_OK;
+    case BB_BF16X3: cp = CPrec::BF16X3; return BB_OK;
+    default: return set_error(BB_E_INVALID, "unknown precision");
     }
-    return nf;
+}
+
+int check_options(const bb_options &o)
+{
+    if (o.x_splits < 0 || o.x_splits > 1024)
+        return set_error(BB_E_INVALID, "x_splits must be in [0, 1024]");
+    if (o.grid_levels < 0 || o.grid_levels > BB_MAX_LEVELS)
+        return set_error(BB_E_INVALID, "grid_levels must be in [0, BB_MAX_LEVELS]");
+    return BB_OK;
 }
 
 }  // namespace
 
 extern "C" {
 
+int bb_estimator_footprint_opts(const bb_plan *p, const bb_options *opt, size_t *bytes)
+{
+    if (!p || !opt || !bytes) return set_error(BB_E_INVALID, "null argument");
+    CPrec cp;
+    int rc = precision_of(opt->precision, cp);
+    if (!rc) rc = check_options(*opt);
+    if (rc) return rc;
+    std::shared_ptr<const Layout> L;
+    try {
+        rc = get_layout(p, cp, *opt, L);
+    } catch (const std::bad_alloc &) {
+        rc = set_error(BB_E_NOMEM, "out of host memory");
+    }
+    if (rc) return rc;
+    *bytes = layout_bytes(p->d, *L);
+    return BB_OK;
+}
+
+int bb_plan_grid_levels(const bb_plan *p, const bb_options *opt, int *nlevels, int *grids,
+                        int *clevel)
+{
+    if (!p || !opt || !nlevels) return set_error(BB_E_INVALID, "null argument");
+    CPrec cp;
+    int rc = precision_of(opt->precision, cp);
+    if (!rc) rc = check_options(*opt);
+    if (rc) return rc;
+    std::shared_ptr<const Layout> L;
+    try {
+        rc = get_layout(p, cp, *opt, L);
+    } catch (const std::bad_alloc &) {
+        rc = set_error(BB_E_NOMEM, "out of host memory");
+    }
+    if (rc) return rc;
+    *nlevels = (int)L->lv.size();
+    if (grids)
+        for (size_t l = 0; l < L->lv.size(); ++l) grids[l] = L->lv[l].NG;
+    if (clevel)
+        for (size_t c = 0; c < L->lvl.size(); ++c) clevel[c] = L->lvl[c];
+    return BB_OK;
+}
+
 int bb_estimator_footprint(const bb_plan *p, bb_precision prec, size_t *bytes)
 {
-    if (!p || !bytes) return set_error(BB_E_INVALID, "null argument");
-    const PlanData &d = p->d;
-    // Upper bound: full-size grid.
-    const size_t NN = d.g.NN, NN2 = NN * NN, Nr = d.g.Nr, Nmax = d.g.Nmax;
-    const size_t nf = cache_fields(d);
-    size_t b = NN * NN * (NN / 2 + 1) * 16 + Nmax * NN2 * 16;
-    b += nf * NN2 * (prec == BB_FP64 ? 16 : 8);
-    if (prec == BB_BF16X3) b += nf * NN2 * 8;   // bf16 hi/lo operand copy
-    if (prec != BB_FP64) b += std::min<size_t>(nf, 128) * NN2 * 16;
-    b += d.contr.size() * (sizeof(BinRec) + 8 * 3 * 2);
-    b += Nr * NN2 * 16;  // transient unit fields (triangle counts)
-    *bytes = b;
-    return BB_OK;
+    bb_options o;
+    std::memset(&o, 0, sizeof o);
+    o.precision = prec;
+    return bb_estimator_footprint_opts(p, &o, bytes);
 }
 
 int bb_estimator_create_opts(const bb_plan *p, const bb_options *opt, bb_estimator **out)
@@ -729,20 +1146,14 @@ int bb_estimator_create_opts(const bb_plan *p, const bb_options *opt, bb_estimat
     if (!p || !opt || !out) return set_error(BB_E_INVALID, "null argument");
     *out = nullptr;
     CPrec cp;
-    switch (opt->precision) {
-    case BB_FP64: cp = CPrec::F64; break;
-    case BB_FP32: cp = CPrec::F32; break;
-    case BB_BF16X3: cp = CPrec::BF16X3; break;
-    default: return set_error(BB_E_INVALID, "unknown precision");
-    }
-    if (opt->x_splits < 0 || opt->x_splits > 1024)
-        return set_error(BB_E_INVALID, "x_splits must be in [0, 1024]");
+    int rc = precision_of(opt->precision, cp);
+    if (!rc) rc = check_options(*opt);
+    if (rc) return rc;
     bb_estimator *E = new (std::nothrow) bb_estimator;
     if (!E) return set_error(BB_E_NOMEM, "out of host memory");
     E->plan = p;
     E->opt = *opt;
     E->cprec = cp;
-    int rc;
     try {
         rc = estimator_init(E);
     } catch (const std::bad_alloc &) {
@@ -769,7 +1180,16 @@ void bb_estimator_destroy(bb_estimator *e) { delete e; }
 int bb_estimator_grid(const bb_estimator *e, int *NG)
 {
     if (!e || !NG) return set_error(BB_E_INVALID, "null argument");
-    *NG = e->NG;
+    *NG = e->L->NGtop;
+    return BB_OK;
+}
+
+int bb_estimator_levels(const bb_estimator *e, int *nlevels, int *grids)
+{
+    if (!e || !nlevels) return set_error(BB_E_INVALID, "null argument");
+    *nlevels = (int)e->L->lv.size();
+    if (grids)
+        for (size_t l = 0; l < e->L->lv.size() && l < BB_MAX_LEVELS; ++l) grids[l] = e->L->lv[l].NG;
     return BB_OK;
 }
 
@@ -804,12 +1224,13 @@ int bb_estimate(bb_estimator *e, const double *field, double *bs, double *norm, 
 int bb_estimator_contraction_sums(bb_estimator *e, double *S)
 {
     if (!e || !S) return set_error(BB_E_INVALID, "null argument");
-    std::vector<double> part((size_t)e->nsplit * e->tile_slots);
+    const Layout &L = *e->L;
+    std::vector<double> part((size_t)std::max<long long>(L.part_tiles, 1) * 256);
     HIPCHK(hipMemcpy(part.data(), e->d_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost));
     const size_t n = e->plan->d.contr.size();
     for (size_t c = 0; c < n; ++c) {
         double v = 0.0;
-        for (int s = 0; s < e->nsplit; ++s) v += part[(size_t)s * e->tile_slots + e->pos[c]];
+        for (int s = 0; s < L.nsp[c]; ++s) v += part[(size_t)L.pos[c] + 256 * (size_t)s];
         S[c] = v;
     }
     return BB_OK;
@@ -823,19 +1244,23 @@ int bb_estimator_triangle_counts(bb_estimator *e, double *num)
     return BB_OK;
 }
 
-int bb_estimator_field(bb_estimator *e, int plane, int ring, double *out)
+int bb_estimator_field(bb_estimator *e, int level, int plane, int ring, double *out)
 {
     if (!e || !out) return set_error(BB_E_INVALID, "null argument");
-    const int f = e->fid(plane, ring);
-    if (f < 0) return set_error(BB_E_INVALID, "field not held in the cache (unused by the plan)");
-    const size_t n = (size_t)e->npts;
+    const Layout &L = *e->L;
+    if (level < 0 || level >= (int)L.lv.size()) return set_error(BB_E_INVALID, "no such level");
+    const Level &V = L.lv[level];
+    const int f = V.fid(plane, ring);
+    if (f < 0) return set_error(BB_E_INVALID, "field not held at this level (unused by the plan)");
+    const size_t n = (size_t)V.npts;
+    const size_t at = (size_t)V.off + (size_t)f * (size_t)V.stride;
     if (e->cprec == CPrec::F64) {
-        HIPCHK(hipMemcpy(out, (const double2 *)e->d_cache + (size_t)f * n, n * sizeof(double2),
+        HIPCHK(hipMemcpy(out, (const double2 *)e->d_cache + at, n * sizeof(double2),
                          hipMemcpyDeviceToHost));
     } else {
         std::vector<float2> tmp(n);
-        HIPCHK(hipMemcpy(tmp.data(), (const float2 *)e->d_cache + (size_t)f * n,
-                         n * sizeof(float2), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(tmp.data(), (const float2 *)e->d_cache + at, n * sizeof(float2),
+                         hipMemcpyDeviceToHost));
         for (size_t k = 0; k < n; ++k) {
             out[2 * k] = tmp[k].x;
             out[2 * k + 1] = tmp[k].y;
@@ -844,19 +1269,46 @@ int bb_estimator_field(bb_estimator *e, int plane, int ring, double *out)
     return BB_OK;
 }
 
+static void worklist_of(const Layout &L, long long *o)
+{
+    o[0] = (long long)L.items.size();
+    o[1] = (long long)L.units.size();
+    o[2] = L.ntiles;
+    o[3] = L.max_rows;
+    o[4] = L.staged_rows;
+    o[5] = L.staged_bytes;
+    o[6] = L.NGtop;
+    o[7] = L.chunk;
+    o[8] = L.waves;
+    o[9] = (long long)std::llround(L.wave_balance * 1000.0);
+    o[10] = (long long)L.lv.size();
+    o[11] = L.part_tiles;
+    o[12] = L.useful_pts;
+    o[13] = (long long)L.items.size() * L.waves * L.tww;
+}
+
 int bb_estimator_worklist(const bb_estimator *e, long long *o)
 {
     if (!e || !o) return set_error(BB_E_INVALID, "null argument");
-    o[0] = e->nitems;
-    o[1] = e->nsplit;
-    o[2] = e->tile_slots / 256;
-    o[3] = e->max_fields;
-    o[4] = e->staged_rows;
-    o[5] = e->staged_rows * e->npts * (e->cprec == CPrec::F64 ? 16 : 8);
-    o[6] = e->NG;
-    o[7] = e->chunk;
-    o[8] = e->waves;
-    o[9] = (long long)std::llround(e->wave_balance * 1000.0);
+    worklist_of(*e->L, o);
+    return BB_OK;
+}
+
+int bb_plan_worklist(const bb_plan *p, const bb_options *opt, long long *o)
+{
+    if (!p || !opt || !o) return set_error(BB_E_INVALID, "null argument");
+    CPrec cp;
+    int rc = precision_of(opt->precision, cp);
+    if (!rc) rc = check_options(*opt);
+    if (rc) return rc;
+    std::shared_ptr<const Layout> L;
+    try {
+        rc = get_layout(p, cp, *opt, L);
+    } catch (const std::bad_alloc &) {
+        rc = set_error(BB_E_NOMEM, "out of host memory");
+    }
+    if (rc) return rc;
+    worklist_of(*L, o);
     return BB_OK;
 }
 

@@ -177,8 +177,10 @@ constexpr int kRowsPerBlock = 4;
 
 // Row pass.  grid = (ceil(max_rows / kRowsPerBlock), ntasks).
 // Output row i of task t goes to rowbuf[slot * buf_stride + i * N], with
-// slot = task.fid when use_fid, else t.
-__global__ __launch_bounds__(256) void k_rowpass(const double2 *__restrict__ planes,
+// slot = task.fid when use_fid, else t.  The plane values come from the
+// NP x NP plane grid (NP >= N): a mode (ia, ja) inside the ring satisfies
+// |ia|, |ja| < N/2 and sits at (ia mod NP, ja mod NP) there.
+__global__ __launch_bounds__(256) void k_rowpass(const double2 *__restrict__ planes, int NP,
                                                  double2 *__restrict__ rowbuf,
                                                  long long buf_stride, int use_fid,
                                                  const FieldTask *__restrict__ tasks, FftDesc fd,
@@ -207,13 +209,14 @@ __global__ __launch_bounds__(256) void k_rowpass(const double2 *__restrict__ pla
             if (T.plane < 0) {
                 v = make_double2(1.0, 0.0);
             } else if (!(T.plane == 0 && i == 0 && j == 0)) {
-                v = planes[((long long)T.plane * N + i) * N + j];
+                const int ip = ia < 0 ? ia + NP : ia, jp = ja < 0 ? ja + NP : ja;
+                v = planes[((long long)T.plane * NP + ip) * NP + jp];
             }
         }
         nz |= (v.x != 0.0 || v.y != 0.0);
         a[bt * N + j] = v;
     }
-    if (nzflag && __any(nz) && (threadIdx.x & 63) == 0) atomicOr(&nzflag[T.fid], 1);
+    if (nzflag && __any(nz) && (threadIdx.x & 63) == 0) atomicOr(&nzflag[T.zid], 1);
     __syncthreads();
     double2 *res = lds_fft(a, b, nb, fd, tw);
     const long long slot = use_fid ? T.fid : blockIdx.y;
@@ -225,7 +228,7 @@ __global__ __launch_bounds__(256) void k_rowpass(const double2 *__restrict__ pla
     }
 }
 
-hipError_t launch_rowpass(const double2 *planes, double2 *rowbuf, long long buf_stride,
+hipError_t launch_rowpass(const double2 *planes, int NP, double2 *rowbuf, long long buf_stride,
                           const FieldTask *tasks, int ntasks, int max_rows, const FftDesc &fd,
                           const double2 *tw, int *nzflag, hipStream_t s)
 {
@@ -237,8 +240,8 @@ hipError_t launch_rowpass(const double2 *planes, double2 *rowbuf, long long buf_
     size_t lds = 2 * kRowsPerBlock * fd.N * sizeof(double2);
     (void)hipFuncSetAttribute((const void *)k_rowpass, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds);
-    hipLaunchKernelGGL(k_rowpass, grid, dim3(256), lds, s, planes, rowbuf, stride, use_fid, tasks,
-                       fd, tw, nzflag);
+    hipLaunchKernelGGL(k_rowpass, grid, dim3(256), lds, s, planes, NP, rowbuf, stride, use_fid,
+                       tasks, fd, tw, nzflag);
     return hipGetLastError();
 }
 
@@ -256,16 +259,19 @@ __device__ __forceinline__ float2 to_out<float2>(double2 v)
 
 // Column pass.  grid = (ceil(N / kColsPerBlock), ntasks).  Reads the row-pass
 // output of task t (slot as in the row pass), transforms along i and writes
-// cache[fid][i][j].  Rows outside the ring's band were never written by the
-// row pass and are taken as zero.  With `split` set (FP32 caches), the field
-// is also written as bf16 hi/lo operands: for every 8 consecutive points,
-// 32 bf16 values [Re hi x8][Re lo x8][Im hi x8][Im lo x8], with
-// hi = bf16(v) and lo = bf16(v - hi) of the FP32 value v.
+// point i*N+j of field fid at cache + fid * out_stride.  Rows outside the
+// ring's band were never written by the row pass and are taken as zero.
+// With `split` set (FP32 caches), the field is also written as bf16 hi/lo
+// operands: for every 8 consecutive points, 32 bf16 values
+// [Re hi x8][Re lo x8][Im hi x8][Im lo x8], with hi = bf16(v) and
+// lo = bf16(v - hi) of the FP32 value v; out_stride is a multiple of 8, so
+// no group of 8 straddles two fields (points past N*N stay as they are).
 template <typename OUT2>
 __global__ __launch_bounds__(256) void k_colpass(const double2 *__restrict__ rowbuf,
                                                  long long buf_stride, int use_fid,
                                                  OUT2 *__restrict__ cache,
                                                  __bf16 *__restrict__ split,
+                                                 long long out_stride,
                                                  const FieldTask *__restrict__ tasks, FftDesc fd,
                                                  const double2 *__restrict__ tw)
 {
@@ -288,8 +294,7 @@ __global__ __launch_bounds__(256) void k_colpass(const double2 *__restrict__ row
     }
     __syncthreads();
     double2 *res = lds_fft(a, b, kColsPerBlock, fd, tw);
-    const long long npts = (long long)N * N;
-    OUT2 *dst = cache + (long long)T.fid * npts;
+    OUT2 *dst = cache + (long long)T.fid * out_stride;
     for (int idx = threadIdx.x; idx < kColsPerBlock * N; idx += blockDim.x) {
         const int i = idx / kColsPerBlock, c = idx - i * kColsPerBlock;
         if (j0 + c >= N) continue;
@@ -299,7 +304,7 @@ __global__ __launch_bounds__(256) void k_colpass(const double2 *__restrict__ row
         if (split) {
             const float vr = (float)o.x, vi = (float)o.y;
             const __bf16 hr = (__bf16)vr, hi = (__bf16)vi;
-            __bf16 *g = split + (long long)T.fid * npts * 4 + (x >> 3) * 32 + (x & 7);
+            __bf16 *g = split + (long long)T.fid * out_stride * 4 + (x >> 3) * 32 + (x & 7);
             g[0] = hr;
             g[8] = (__bf16)(vr - (float)hr);
             g[16] = hi;
@@ -310,9 +315,10 @@ __global__ __launch_bounds__(256) void k_colpass(const double2 *__restrict__ row
 
 template <typename OUT2>
 static hipError_t launch_colpass(const double2 *rowbuf, long long buf_stride, OUT2 *cache,
-                                 __bf16 *split, const FieldTask *tasks, int ntasks,
-                                 const FftDesc &fd, const double2 *tw, hipStream_t s)
+                                 __bf16 *split, long long out_stride, const FieldTask *tasks,
+                                 int ntasks, const FftDesc &fd, const double2 *tw, hipStream_t s)
 {
+    if (out_stride < (long long)fd.N * fd.N || out_stride % 8 != 0) return hipErrorInvalidValue;
     int use_fid = buf_stride > 0 ? 1 : 0;
     long long stride = buf_stride > 0 ? buf_stride : -buf_stride;
     dim3 grid((fd.N + kColsPerBlock - 1) / kColsPerBlock, ntasks);
@@ -320,23 +326,24 @@ static hipError_t launch_colpass(const double2 *rowbuf, long long buf_stride, OU
     (void)hipFuncSetAttribute((const void *)k_colpass<OUT2>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(k_colpass<OUT2>, grid, dim3(256), lds, s, rowbuf, stride, use_fid, cache,
-                       split, tasks, fd, tw);
+                       split, out_stride, tasks, fd, tw);
     return hipGetLastError();
 }
 
 hipError_t launch_colpass_f64(const double2 *rowbuf, long long buf_stride, double2 *cache,
-                              const FieldTask *tasks, int ntasks, const FftDesc &fd,
-                              const double2 *tw, hipStream_t s)
+                              long long out_stride, const FieldTask *tasks, int ntasks,
+                              const FftDesc &fd, const double2 *tw, hipStream_t s)
 {
-    return launch_colpass<double2>(rowbuf, buf_stride, cache, nullptr, tasks, ntasks, fd, tw, s);
+    return launch_colpass<double2>(rowbuf, buf_stride, cache, nullptr, out_stride, tasks, ntasks,
+                                   fd, tw, s);
 }
 
 hipError_t launch_colpass_f32(const double2 *rowbuf, long long buf_stride, float2 *cache,
-                              void *split, const FieldTask *tasks, int ntasks, const FftDesc &fd,
-                              const double2 *tw, hipStream_t s)
+                              void *split, long long out_stride, const FieldTask *tasks,
+                              int ntasks, const FftDesc &fd, const double2 *tw, hipStream_t s)
 {
-    return launch_colpass<float2>(rowbuf, buf_stride, cache, (__bf16 *)split, tasks, ntasks, fd,
-                                  tw, s);
+    return launch_colpass<float2>(rowbuf, buf_stride, cache, (__bf16 *)split, out_stride, tasks,
+                                  ntasks, fd, tw, s);
 }
 
 }  // namespace bb

@@ -1,6 +1,9 @@
 // Host-side geometry plan of the estimator (internal to libbarbie_amd).
 #pragma once
 #include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -30,8 +33,17 @@ struct PlanData {
 // Sets the thread's error message and returns code.
 int set_error(int code, const std::string &msg);
 
+// Device-independent estimator layout (grid levels, field slots, work lists,
+// binning records); built once per (plan, precision, options) and shared by
+// every estimator of that key.
+struct Layout;
+
 }  // namespace bb
 
 struct bb_plan {
     bb::PlanData d;
+    std::mutex mu;   // guards the caches below
+    std::map<std::string, std::shared_ptr<const bb::Layout>> layouts;
+    std::vector<double> num;   // triangle counts (calCR), once computed
+    bool have_num = false;
 };

@@ -26,12 +26,17 @@ RINGS_REFERENCE, RINGS_UNIFORM, RINGS_EXPLICIT = 0, 1, 2
 EXPORTED = [
     "bb_last_error", "bb_version", "bb_geometry_reference", "bb_plan_create",
     "bb_plan_destroy", "bb_plan_stats_get", "bb_plan_ring_edges", "bb_plan_triples",
-    "bb_plan_contractions", "bb_estimator_create", "bb_estimator_create_opts",
-    "bb_estimator_destroy", "bb_estimator_grid",
-    "bb_estimator_footprint", "bb_estimate", "bb_estimate_device",
+    "bb_plan_contractions", "bb_plan_grid_levels", "bb_plan_worklist", "bb_estimator_create", "bb_estimator_create_opts",
+    "bb_estimator_destroy", "bb_estimator_grid", "bb_estimator_levels",
+    "bb_estimator_footprint", "bb_estimator_footprint_opts", "bb_estimate", "bb_estimate_device",
     "bb_estimator_contraction_sums", "bb_estimator_triangle_counts", "bb_estimator_field",
     "bb_estimator_timings", "bb_estimator_worklist", "bb_write_outputs",
 ]
+
+# bb_estimator_worklist / bb_plan_worklist entries, in order.
+WORKLIST_KEYS = ("items", "units", "tiles", "max_rings", "staged_rings", "staged_bytes", "grid",
+                 "chunk", "waves", "wave_balance_x1000", "levels", "partial_tiles",
+                 "points_summed", "slots")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -56,7 +61,7 @@ class BBPlanStats(ctypes.Structure):
 
 class BBOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("full_grid", ctypes.c_int),
-                ("x_splits", ctypes.c_int)]
+                ("x_splits", ctypes.c_int), ("grid_levels", ctypes.c_int)]
 
 
 class BarbieError(RuntimeError):
@@ -99,7 +104,15 @@ def lib() -> ctypes.CDLL:
         L.bb_estimate_device.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 4
         L.bb_estimator_contraction_sums.argtypes = [ctypes.c_void_p, _dp]
         L.bb_estimator_triangle_counts.argtypes = [ctypes.c_void_p, _dp]
-        L.bb_estimator_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+        L.bb_estimator_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, _dp]
+        L.bb_estimator_levels.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), _ip]
+        L.bb_estimator_footprint_opts.argtypes = [ctypes.c_void_p, ctypes.POINTER(BBOptions),
+                                                  ctypes.POINTER(ctypes.c_size_t)]
+        L.bb_plan_worklist.argtypes = [ctypes.c_void_p, ctypes.POINTER(BBOptions),
+                                       ctypes.POINTER(ctypes.c_longlong)]
+        L.bb_plan_grid_levels.argtypes = [ctypes.c_void_p, ctypes.POINTER(BBOptions),
+                                          ctypes.POINTER(ctypes.c_int), _ip, _ip]
         L.bb_estimator_timings.argtypes = [ctypes.c_void_p, _dp]
         L.bb_estimator_worklist.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)]
         L.bb_write_outputs.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_char_p,
@@ -197,10 +210,33 @@ class Plan:
         _check(lib().bb_plan_contractions(self.handle, out.ctypes.data_as(_ip)))
         return out
 
-    def footprint(self, precision: str) -> int:
+    def footprint(self, precision: str, full_grid: bool = False, x_splits: int = 0,
+                  grid_levels: int = 0) -> int:
+        """Device bytes an Estimator with these options holds."""
         b = ctypes.c_size_t()
-        _check(lib().bb_estimator_footprint(self.handle, PRECISIONS[precision], ctypes.byref(b)))
+        o = BBOptions(PRECISIONS[precision], int(full_grid), int(x_splits), int(grid_levels))
+        _check(lib().bb_estimator_footprint_opts(self.handle, ctypes.byref(o), ctypes.byref(b)))
         return b.value
+
+    def grid_levels(self, precision: str = "fp64", full_grid: bool = False,
+                    grid_levels: int = 0):
+        """(grids, level of each contraction) an Estimator with these options
+        uses; host only."""
+        n = ctypes.c_int()
+        grids = (ctypes.c_int * 16)()
+        cl = np.zeros(self.stats["ncontr"], dtype=np.int32)
+        o = BBOptions(PRECISIONS[precision], int(full_grid), 0, int(grid_levels))
+        _check(lib().bb_plan_grid_levels(self.handle, ctypes.byref(o), ctypes.byref(n), grids,
+                                         cl.ctypes.data_as(_ip)))
+        return [grids[i] for i in range(n.value)], cl
+
+    def worklist(self, precision: str = "fp64", full_grid: bool = False, x_splits: int = 0,
+                 grid_levels: int = 0) -> dict:
+        """Work-list figures of an Estimator with these options (host only)."""
+        o = (ctypes.c_longlong * len(WORKLIST_KEYS))()
+        opt = BBOptions(PRECISIONS[precision], int(full_grid), int(x_splits), int(grid_levels))
+        _check(lib().bb_plan_worklist(self.handle, ctypes.byref(opt), o))
+        return dict(zip(WORKLIST_KEYS, list(o)))
 
     def write_outputs(self, bs, norm, biana, outdir: str, NRi: int):
         bs, norm, biana = (np.ascontiguousarray(a, dtype=np.float64) for a in (bs, norm, biana))
@@ -212,16 +248,20 @@ class Estimator:
     """Device estimator (bb_estimator) for one plan and precision."""
 
     def __init__(self, plan: Plan, precision: str = "fp64", full_grid: bool = False,
-                 x_splits: int = 0):
+                 x_splits: int = 0, grid_levels: int = 0):
         self.plan = plan
         self.precision = precision
         h = ctypes.c_void_p()
-        o = BBOptions(PRECISIONS[precision], int(full_grid), int(x_splits))
+        o = BBOptions(PRECISIONS[precision], int(full_grid), int(x_splits), int(grid_levels))
         _check(lib().bb_estimator_create_opts(plan.handle, ctypes.byref(o), ctypes.byref(h)))
         self.handle = h
         ng = ctypes.c_int()
         _check(lib().bb_estimator_grid(h, ctypes.byref(ng)))
         self.grid = ng.value
+        n = ctypes.c_int()
+        grids = (ctypes.c_int * 16)()
+        _check(lib().bb_estimator_levels(h, ctypes.byref(n), grids))
+        self.levels = [grids[i] for i in range(n.value)]
 
     def close(self):
         h = getattr(self, "handle", None)
@@ -259,19 +299,19 @@ class Estimator:
         _check(lib().bb_estimator_triangle_counts(self.handle, _p(n)))
         return n
 
-    def field(self, plane: int, ring: int) -> np.ndarray:
-        """Plane-ring field D_{plane,ring} on the estimator's grid (NG x NG)."""
-        NG = self.grid
+    def field(self, level: int, plane: int, ring: int) -> np.ndarray:
+        """Plane-ring field D_{plane,ring} on the grid of a level
+        (NG x NG, NG = levels[level])."""
+        NG = self.levels[level]
         out = np.zeros((NG, NG), dtype=np.complex128)
-        _check(lib().bb_estimator_field(self.handle, plane, ring, _p(out.view(np.float64))))
+        _check(lib().bb_estimator_field(self.handle, level, plane, ring,
+                                        _p(out.view(np.float64))))
         return out
 
     def worklist(self) -> dict:
-        o = (ctypes.c_longlong * 10)()
+        o = (ctypes.c_longlong * len(WORKLIST_KEYS))()
         _check(lib().bb_estimator_worklist(self.handle, o))
-        keys = ("items", "x_splits", "tiles", "max_rings", "staged_rings", "staged_bytes",
-                "grid", "chunk", "waves", "wave_balance_x1000")
-        return dict(zip(keys, list(o)))
+        return dict(zip(WORKLIST_KEYS, list(o)))
 
     def timings(self) -> np.ndarray:
         t = np.zeros(4)

@@ -52,7 +52,7 @@ def geometry(NN):
 
 def time_path(B, torch, plan, precision, d_field, steps, warmup, full_grid=False):
     est = B.Estimator(plan, precision, full_grid=full_grid)
-    grid = est.grid
+    grid = est.levels
     wl = est.worklist()
     nb = plan.geom.nbins * 16
     out = torch.zeros(3 * nb, dtype=torch.float64, device="cuda")
@@ -129,10 +129,9 @@ def main():
         res["fp64"] = r64[:3]
         wl64 = r64[4]
 
-    # useful contraction flops actually performed: Re(P C) = 2 FMAs per point
-    # per contraction on the grid the sums run on (NG^2 points)
-    npts = grid * grid
-    w_contr = 4.0 * npts * s["ncontr"]
+    # algorithmic contraction flops: Re(P C) = 2 FMAs per point per
+    # contraction, summed over the points of each contraction's grid level
+    w_contr = 4.0 * wl["points_summed"]
     t_contr = st[2] * 1e-3
     achieved = w_contr / t_contr
     peak = PEAK[a.precision]
@@ -153,15 +152,15 @@ def main():
         "config": {"workload": f"estimate NN={g.NN} Nmax={g.Nmax} Nr={g.Nr} UL_k1={g.UL_k1} "
                                f"LL_k1={g.LL_k1} BW_k1={g.BW_k1} n_mtbin={g.n_mtbin}",
                    "ncontr": s["ncontr"], "fields_used": s["nfields_used"],
-                   "precision": a.precision, "grid": grid},
+                   "precision": a.precision, "grid_levels": grid},
         "stages_ms": {"gather": st[0], "fft_cache": st[1], "contraction": st[2],
                       "binning": st[3]},
         "worklist": wl,
         "roofline": {"bound": "mfma", "kernel": "contraction",
                      "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,
-                     "work": f"4*NG^2*ncontr = {w_contr:.4g} flop per launch (NG={grid}); "
-                             "bf16x3 peak taken as 2.5 PF/3"},
+                     "work": f"4 * sum_c NG(c)^2 = {w_contr:.4g} flop per launch (grid levels "
+                             f"{grid}); bf16x3 peak taken as 2.5 PF/3"},
     }
     if "fp64" in res:
         ms64, st64, bins64 = res["fp64"]

@@ -56,12 +56,20 @@ typedef enum {
 /* Execution options of an estimator (bb_estimator_create_opts). */
 typedef struct bb_options {
     bb_precision precision;
-    int full_grid;  /* 0: run the plane-ring transforms and triple sums on the
-                       smallest 2,3,5-smooth grid NG >= 3*kmax+1 (kmax = largest
-                       |k_x| of any ring mode) when NG < NN; the sums are then
-                       exactly (NN/NG)^2 times the NN-grid sums, because no
-                       closed triangle aliases on either grid.  1: always use NN. */
+    int full_grid;  /* 1: run the plane-ring transforms and triple sums on the
+                       NN grid, as the reference does.  0: when NN >= 3*kmax+1
+                       (kmax = largest |k_x| of any ring mode, so that no sum
+                       of three ring modes wraps around on the NN grid), run
+                       each contraction on a smaller grid instead: a
+                       contraction of rings r1, r2, r3 sums exactly (times
+                       (NN/NG)^2) on any grid NG > K1 + K2 + K3 (K_r = largest
+                       |k_x| of ring r) that also holds each ring's modes
+                       (NG >= 2 K_r + 2).  Below 3*kmax+1 the NN grid is used. */
     int x_splits;   /* partial sums per contraction over the points (0: automatic) */
+    int grid_levels; /* with full_grid 0: the largest number of distinct grids
+                       (0: automatic, 1: one grid).  The grids are chosen
+                       among the even 2,3,5-smooth sizes to minimise the
+                       summed points over all contractions. */
 } bb_options;
 
 /* Geometry and binning of one estimate.  Replaces the constants hard-coded at
@@ -126,10 +134,22 @@ int bb_plan_contractions(const bb_plan *p, int *rec);
 int bb_estimator_create(const bb_plan *p, bb_precision prec, bb_estimator **out);
 int bb_estimator_create_opts(const bb_plan *p, const bb_options *opt, bb_estimator **out);
 void bb_estimator_destroy(bb_estimator *e);
-/* Grid the estimator's transforms and sums run on (NG; NN with full_grid). */
+/* Grid levels an estimator with these options uses (host only, no GPU):
+ * *nlevels grids in ascending size, and for each contraction (plan order)
+ * the index of its level (clevel may be NULL). */
+int bb_plan_grid_levels(const bb_plan *p, const bb_options *opt, int *nlevels, int *grids,
+                        int *clevel);
+/* Largest grid the estimator's transforms and sums run on (NG; NN with
+ * full_grid). */
 int bb_estimator_grid(const bb_estimator *e, int *NG);
-/* Device bytes held by an estimator of this plan and precision. */
+/* Grid levels in ascending size: *nlevels and up to BB_MAX_LEVELS grids. */
+#define BB_MAX_LEVELS 16
+int bb_estimator_levels(const bb_estimator *e, int *nlevels, int *grids);
+/* Device bytes an estimator of this plan and precision holds (default
+ * options); the layout it measures is cached in the plan and reused by
+ * bb_estimator_create. */
 int bb_estimator_footprint(const bb_plan *p, bb_precision prec, size_t *bytes);
+int bb_estimator_footprint_opts(const bb_plan *p, const bb_options *opt, size_t *bytes);
 
 /* One estimate (barbie.c:176-373): field_r2c is the FFTW in-place r2c buffer
  * [NN][NN][NN+2] doubles in host memory (the contents of
@@ -147,19 +167,28 @@ int bb_estimate_device(bb_estimator *e, const double *d_field_r2c,
 /* Diagnostics of the last estimate: per-contraction triple-product sums in
  * the reference's enumeration order (ncontr doubles, scaled to the NN grid),
  * the GPU triangle counts Num per ring triple (rcount doubles, NN grid), and
- * one plane-ring field D_{plane,ring} on the estimator's grid (NG*NG complex,
- * interleaved) converted to FP64. */
+ * one plane-ring field D_{plane,ring} on the grid of a level (NG*NG complex,
+ * interleaved, NG = grids[level]) converted to FP64; BB_E_INVALID when the
+ * level holds no such field. */
 int bb_estimator_contraction_sums(bb_estimator *e, double *S);
 int bb_estimator_triangle_counts(bb_estimator *e, double *num);
-int bb_estimator_field(bb_estimator *e, int plane, int ring, double *out);
-/* Contraction work lists: out[0] work items, [1] x-splits, [2] 16x16 output
- * tiles, [3] largest staged ring count of an item, [4] sum of staged rings
- * over items, [5] bytes the contraction reads from the cache per estimate,
- * [6] grid NG, [7] points per LDS chunk, [8] waves per workgroup, [9] wave
- * balance x1000 (tile windows / (waves x busiest wave), summed over items).
- * The environment variable BARBIE_CONTRACT_WAVES (4 or 8) picks the
- * workgroup width when the estimator is created. */
-int bb_estimator_worklist(const bb_estimator *e, long long *out10);
+int bb_estimator_field(bb_estimator *e, int level, int plane, int ring, double *out);
+/* Contraction work lists: out[0] work items, [1] workgroups (units) of the
+ * launch, [2] 16x16 output tiles, [3] largest staged ring count of an item,
+ * [4] sum of staged rings over items, [5] bytes the contraction reads from
+ * the cache per estimate, [6] largest grid NG, [7] points per LDS chunk,
+ * [8] waves per workgroup, [9] wave balance x1000 (tile windows / (waves x
+ * busiest wave), summed over items), [10] grid levels, [11] partial tiles
+ * (output tiles x their x-splits), [12] sum over contractions of NG^2 of
+ * their level (points summed).  The environment variables
+ * BARBIE_CONTRACT_WAVES (4 or 8) and BARBIE_GRID_LEVELS (overrides
+ * grid_levels) are read when the estimator is created. */
+#define BB_WORKLIST_N 14
+int bb_estimator_worklist(const bb_estimator *e, long long *out);
+/* The same work-list figures for an estimator with these options, computed
+ * on the host without a device ([13]: tile window slots executed, i.e.
+ * items x waves x windows per wave). */
+int bb_plan_worklist(const bb_plan *p, const bb_options *opt, long long *out);
 /* Per-stage device times of the last estimate in milliseconds:
  * [0] upload/gather, [1] plane-ring FFTs, [2] contraction, [3] binning. */
 int bb_estimator_timings(bb_estimator *e, double *ms4);

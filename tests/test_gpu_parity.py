@@ -39,15 +39,22 @@ def rel_err(a, b):
     return np.max(np.abs(a - b)) / scale if scale > 0 else np.max(np.abs(a))
 
 
-@pytest.mark.parametrize("full_grid", [False, True])
+# Grid modes: the NN grid of the reference ("full"), one alias-free grid for
+# every contraction ("one_grid") and the default grid levels ("levels").
+MODES = {"full": dict(full_grid=True), "one_grid": dict(grid_levels=1), "levels": {}}
+
+
+@pytest.mark.parametrize("mode", list(MODES))
 @pytest.mark.parametrize("gf", [tiny, small, radix5, medium, config1])
-def test_fp64_matches_oracle(gf, full_grid):
+def test_fp64_matches_oracle(gf, mode):
     o = oracle_run(gf, NRi=1)
     g = o["g"]
     plan = B.Plan(g)
-    est = B.Estimator(plan, "fp64", full_grid=full_grid)
-    if full_grid:
-        assert est.grid == g.NN
+    est = B.Estimator(plan, "fp64", **MODES[mode])
+    if mode == "full":
+        assert est.grid == g.NN and est.levels == [g.NN]
+    if mode == "one_grid":
+        assert len(est.levels) == 1
     bs, nm, bi = est.estimate(o["field"])
     # triangle counts Num (calCR) within 1e-12
     assert rel_err(est.triangle_counts(), o["rd"][:, 3]) <= 1e-12
@@ -81,8 +88,61 @@ def test_fp64_plane_ring_fields_match_oracle(gf):
     used |= {(abs(int(p)), int(r)) for p, r in zip(pc[:, 2], pc[:, 3])}
     for p, r in sorted(used)[:: max(1, len(used) // 12)]:
         ref = O.df(o["field"], g.NN, p, lo[r], hi[r])
-        got = est.field(p, r)
+        got = est.field(0, p, r)
         assert np.max(np.abs(got - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref))), (p, r)
+
+
+def resample(d_nn, NG):
+    """The trigonometric polynomial sampled on the NN grid, resampled on NG
+    points per side (numpy FFT): its modes must lie within |k| < NG/2."""
+    NN = d_nn.shape[0]
+    A = np.fft.fft2(d_nn) / NN ** 2
+    k = np.arange(NN)
+    s = np.where(k > NN // 2, k - NN, k)
+    keep = np.abs(s) < NG // 2
+    out = np.zeros((NG, NG), dtype=np.complex128)
+    ii = np.nonzero(keep)[0]
+    out[np.ix_(s[ii] % NG, s[ii] % NG)] = A[np.ix_(ii, ii)]
+    tol = 1e-12 * np.max(np.abs(A))
+    assert np.all(np.abs(A[~keep, :]) <= tol) and np.all(np.abs(A[:, ~keep]) <= tol)
+    return np.fft.ifft2(out) * NG ** 2
+
+
+@pytest.mark.parametrize("gf", [medium, config1])
+def test_fp64_level_fields_match_oracle(gf):
+    """Every level holds its fields on its own grid: the oracle's NN-grid
+    field resampled onto that grid, within 1e-12."""
+    o = oracle_run(gf, NRi=1)
+    g = o["g"]
+    lo, hi, _ = o["edges"]
+    plan = B.Plan(g)
+    est = B.Estimator(plan, "fp64")
+    est.estimate(o["field"])
+    grids, cl = plan.grid_levels("fp64")
+    assert grids == est.levels and len(grids) > 1
+    pc = plan.contractions()
+    for lv, NG in enumerate(grids):
+        sel = pc[cl == lv]
+        used = {(int(p), int(r)) for p, r in zip(sel[:, 0], sel[:, 1])}
+        used |= {(abs(int(p)), int(r)) for p, r in zip(sel[:, 4], sel[:, 5])}
+        for p, r in sorted(used)[:: max(1, len(used) // 6)]:
+            ref = resample(O.df(o["field"], g.NN, p, lo[r], hi[r]), NG)
+            got = est.field(lv, p, r)
+            assert np.max(np.abs(got - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref))), (lv, p, r)
+
+
+def test_levels_match_one_grid():
+    """Grid levels and a single grid give the same sums (to FP64 rounding)."""
+    g = config1()
+    f = B.make_field(g.NN, NRi=3)
+    plan = B.Plan(g)
+    a = B.Estimator(plan, "fp64", grid_levels=1)
+    b = B.Estimator(plan, "fp64")
+    assert len(b.levels) > 1
+    ra, rb = a.estimate(f), b.estimate(f)
+    for x, y in zip(ra, rb):
+        assert rel_err(y, x) <= 1e-12
+    assert rel_err(b.contraction_sums(), a.contraction_sums()) <= 1e-12
 
 
 def test_zero_signal_field():
