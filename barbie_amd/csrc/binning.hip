@@ -3,9 +3,10 @@
 // comparison), :343-363 (bins, zero-signal rule, accumulation) and cal_coeff
 // (funcs.c:708-745).
 //
-// Contractions arrive sorted by bin; one wavefront owns one bin, each lane
-// accumulates a strided subset of the bin's contractions in registers and
-// the wave reduces its 48 sums in a fixed tree order (deterministic).
+// Contractions arrive sorted by bin; one workgroup of four waves owns one
+// bin, each thread accumulates a strided subset of the bin's contractions in
+// registers, each wave reduces its 48 sums in a fixed tree order and the
+// four wave sums are added in wave order (deterministic).
 #include <hip/hip_runtime.h>
 
 #include "device.hpp"
@@ -64,7 +65,9 @@ __device__ __forceinline__ void cal_coeff(double px, double pz, double *bs, doub
     norm[15] = om2 * om2 * om2;
 }
 
-__global__ __launch_bounds__(64) void k_binning(const int32_t *__restrict__ seg,
+constexpr int kBinThreads = 256;   // four waves per bin
+
+__global__ __launch_bounds__(kBinThreads) void k_binning(const int32_t *__restrict__ seg,
                                                 const BinRec *__restrict__ recs,
                                                 const double *__restrict__ S,
                                                 const double *__restrict__ num,
@@ -75,12 +78,12 @@ __global__ __launch_bounds__(64) void k_binning(const int32_t *__restrict__ seg,
                                                 double *__restrict__ out_biana)
 {
     const int b = blockIdx.x;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     double abs_[16], anm[16], abi[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) abs_[k] = anm[k] = abi[k] = 0.0;
 
-    for (int c = seg[b] + lane; c < seg[b + 1]; c += 64) {
+    for (int c = seg[b] + tid; c < seg[b + 1]; c += kBinThreads) {
         const BinRec R = recs[c];
         const double a1 = avg[R.r1], a2 = avg[R.r2], a3 = avg[R.r3];
         const double p1 = R.p1, p2 = R.p2, p3 = R.p3;
@@ -124,6 +127,8 @@ __global__ __launch_bounds__(64) void k_binning(const int32_t *__restrict__ seg,
             anm[k] += Num * nm[k];
         }
     }
+    // Wave sums (butterfly), then the four waves' sums in wave order.
+    __shared__ double part[kBinThreads / 64][48];
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
         double v0 = abs_[k], v1 = anm[k], v2 = abi[k];
@@ -134,10 +139,18 @@ __global__ __launch_bounds__(64) void k_binning(const int32_t *__restrict__ seg,
             v2 += __shfl_xor(v2, off, 64);
         }
         if (lane == 0) {
-            if (out_bs) out_bs[(long long)b * 16 + k] = v0;
-            if (out_norm) out_norm[(long long)b * 16 + k] = v1;
-            if (out_biana) out_biana[(long long)b * 16 + k] = v2;
+            part[wave][k] = v0;
+            part[wave][16 + k] = v1;
+            part[wave][32 + k] = v2;
         }
+    }
+    __syncthreads();
+    if (tid < 48) {
+        double v = part[0][tid];
+        for (int w = 1; w < kBinThreads / 64; ++w) v += part[w][tid];
+        const int k = tid & 15;
+        double *o = tid < 16 ? out_bs : tid < 32 ? out_norm : out_biana;
+        if (o) o[(long long)b * 16 + k] = v;
     }
 }
 
@@ -146,8 +159,8 @@ hipError_t launch_binning(const int32_t *seg, const BinRec *recs, const double *
                           const BinParams &bp, double *bins_bs, double *bins_norm,
                           double *bins_biana, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_binning, dim3(bp.nbins), dim3(64), 0, s, seg, recs, S, num, nzflag, avg,
-                       bp, bins_bs, bins_norm, bins_biana);
+    hipLaunchKernelGGL(k_binning, dim3(bp.nbins), dim3(kBinThreads), 0, s, seg, recs, S, num,
+                       nzflag, avg, bp, bins_bs, bins_norm, bins_biana);
     return hipGetLastError();
 }
 

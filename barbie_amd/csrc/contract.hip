@@ -270,6 +270,54 @@ __device__ __forceinline__ f32x4 mfma_bf16(const bf16x8 &a, const bf16x8 &b, f32
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// hi = bf16(x) (round to nearest even), lo = bf16(x - hi), eight values.
+// Each pair is converted once; the FP32 value of hi is read back from the
+// packed bits (low half << 16, high half masked).
+__device__ __forceinline__ void split8(const f32x8 &x, bf16x8 &hi, bf16x8 &lo)
+{
+    u32x4 h, l;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const f32x2 v = {x[2 * q], x[2 * q + 1]};
+        const unsigned hb = __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
+        const f32x2 r = {v.x - __uint_as_float(hb << 16), v.y - __uint_as_float(hb & 0xffff0000u)};
+        h[q] = hb;
+        l[q] = __builtin_bit_cast(unsigned, __builtin_convertvector(r, bf16x2));
+    }
+    hi = __builtin_bit_cast(bf16x8, h);
+    lo = __builtin_bit_cast(bf16x8, l);
+}
+
+// Pair products of one tile's rows over the chunk's 8 points of this lane,
+// split into bf16 hi/lo A operands: Pr = Re(D1 D~2), Ai = -Im(D1 D~2), or
+// +Im when the third plane is negative.  C2 / C3: conjugations as template
+// parameters, so that the signs become instruction modifiers.
+template <bool C2, bool C3>
+__device__ __forceinline__ void pair_operands(const unsigned char *rows, int oa1, int oa2,
+                                              bf16x8 &arh, bf16x8 &arl, bf16x8 &aih,
+                                              bf16x8 &ail)
+{
+    const float4 *r1p = reinterpret_cast<const float4 *>(rows + oa1);
+    const float4 *r2p = reinterpret_cast<const float4 *>(rows + oa2);
+    f32x8 pr, ai;
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {   // two points per float4
+        const float4 a = r1p[h], b = r2p[h];
+        const float e0 = C2 ? -b.y : b.y, e1 = C2 ? -b.w : b.w;
+        pr[2 * h] = a.x * b.x - a.y * e0;
+        pr[2 * h + 1] = a.z * b.z - a.w * e1;
+        const float i0 = a.x * e0 + a.y * b.x, i1 = a.z * e1 + a.w * b.z;
+        ai[2 * h] = C3 ? i0 : -i0;
+        ai[2 * h + 1] = C3 ? i1 : -i1;
+    }
+    split8(pr, arh, arl);
+    split8(ai, aih, ail);
+}
+
 // A staged ring of 32 points: raw rows hold float2 per point; split rows
 // hold, per group of 8 points, [Re hi x8][Re lo x8][Im hi x8][Im lo x8] bf16.
 template <int NW>
@@ -293,6 +341,7 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_contract_bf16x3(
     const int m = lane & 15, kq = lane >> 4;
 
     const TWin *TW = tws + it.tw0 + wave * kTWW;
+    const int cnt = uni(it.cnt[wave]);
     int oa1[TWW], oa2[TWW], ob[TWW], fl[TWW];
 #pragma unroll
     for (int j = 0; j < TWW; ++j) {
@@ -328,6 +377,7 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_contract_bf16x3(
         bf16x8 arh = {}, arl = {}, aih = {}, ail = {};
 #pragma unroll
         for (int j = 0; j < TWW; ++j) {
+            if (j >= cnt) break;   // uniform: the wave's remaining slots are empty
             // B first, so its latency overlaps the pair products.
             const unsigned char *bp = rows + ob[j];
             const bf16x8 brh = *reinterpret_cast<const bf16x8 *>(bp);
@@ -335,24 +385,12 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_contract_bf16x3(
             const bf16x8 bih = *reinterpret_cast<const bf16x8 *>(bp + 32);
             const bf16x8 bil = *reinterpret_cast<const bf16x8 *>(bp + 48);
             if (fl[j] & TW_NEWA) {
-                const float4 *r1p = reinterpret_cast<const float4 *>(rows + oa1[j]);
-                const float4 *r2p = reinterpret_cast<const float4 *>(rows + oa2[j]);
-                const unsigned m2 = (fl[j] & TW_CONJ2) ? 0x80000000u : 0u;
-                const unsigned m3 = (fl[j] & TW_CONJ3) ? 0u : 0x80000000u;
-                f32x8 pr, ai;
-#pragma unroll
-                for (int h = 0; h < 4; ++h) {   // two points per float4
-                    const float4 a = r1p[h], b = r2p[h];
-                    const float e0 = flip(b.y, m2), e1 = flip(b.w, m2);
-                    pr[2 * h] = a.x * b.x - a.y * e0;
-                    pr[2 * h + 1] = a.z * b.z - a.w * e1;
-                    ai[2 * h] = flip(a.x * e0 + a.y * b.x, m3);
-                    ai[2 * h + 1] = flip(a.z * e1 + a.w * b.z, m3);
+                switch (fl[j] & (TW_CONJ2 | TW_CONJ3)) {
+                case 0: pair_operands<false, false>(rows, oa1[j], oa2[j], arh, arl, aih, ail); break;
+                case TW_CONJ2: pair_operands<true, false>(rows, oa1[j], oa2[j], arh, arl, aih, ail); break;
+                case TW_CONJ3: pair_operands<false, true>(rows, oa1[j], oa2[j], arh, arl, aih, ail); break;
+                default: pair_operands<true, true>(rows, oa1[j], oa2[j], arh, arl, aih, ail); break;
                 }
-                arh = __builtin_convertvector(pr, bf16x8);
-                arl = __builtin_convertvector(pr - __builtin_convertvector(arh, f32x8), bf16x8);
-                aih = __builtin_convertvector(ai, bf16x8);
-                ail = __builtin_convertvector(ai - __builtin_convertvector(aih, f32x8), bf16x8);
             }
             acc[j] = mfma_bf16(arh, brh, acc[j]);
             acc[j] = mfma_bf16(aih, bih, acc[j]);

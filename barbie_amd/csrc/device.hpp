@@ -3,16 +3,28 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <vector>
 
 namespace bb {
 
 constexpr int kMaxStages = 16;
 
 // Factorisation of an FFT length into radices 4, 2, 3, 5 (Stockham order).
+// Exact division for 0 <= n < 2^20 and 1 <= d < 2^12 (the index ranges of
+// the FFT passes): n / d = (n * ceil(2^40 / d)) >> 40.
+inline unsigned long long udiv_magic(unsigned d) { return ((1ULL << 40) + d - 1) / d; }
+__device__ __forceinline__ unsigned udiv(unsigned n, unsigned long long m)
+{
+    return (unsigned)(((unsigned long long)n * m) >> 40);
+}
+
 struct FftDesc {
     int N;
     int nst;
     int radix[kMaxStages];
+    unsigned long long mN;                  // udiv_magic(N)
+    unsigned long long mNR[kMaxStages];     // udiv_magic(N / radix[s])
+    unsigned long long mP[kMaxStages];      // udiv_magic(radix[0] * .. * radix[s-1])
 };
 
 // One plane-ring field to build: D_{plane,ring}(x) = IFFT2(mask_ring * plane).
@@ -23,7 +35,22 @@ struct FieldTask {
     int32_t rowband;  // rows |ia| <= rowband can hold masked modes
     int32_t nlo, nhi; // ring holds ia^2 + ja^2 in [nlo, nhi)
     int32_t zid;      // nonzero flag of (plane, ring): plane * Nr + ring
+    int32_t level;    // grid level (index into the LevelDesc table)
 };
+
+// FFT geometry of one grid level.
+struct LevelDesc {
+    FftDesc fd;
+    int tw_off;          // first twiddle of the level in the twiddle table
+    int rpb, cpb;        // transforms per block of the row and column passes (cpb: 2^k)
+    int cpb_log2;
+    int max_rows;        // widest row band of the level's fields
+    long long off;       // first point of the level in the cache
+    long long stride;    // points between fields (multiple of 8)
+};
+// Transforms per block of the row and column passes for length N.
+int fft_rows_per_block(int N);
+int fft_cols_per_block(int N);
 
 // ---------------------------------------------------------------------------
 // Grid levels.  The sum over x of a triple product of fields band-limited to
@@ -109,19 +136,26 @@ struct BinParams {
 // Launchers (fft.hip, contract.hip, binning.hip).
 hipError_t launch_gather(const double2 *field, double2 *planes, int NN, int NG, int kmax,
                          int Nmax, hipStream_t s);
-// planes: NP x NP per plane (signed modes wrapped), read for the ring modes of
-// each task on the NG = fd.N grid of the level.
-hipError_t launch_rowpass(const double2 *planes, int NP, double2 *rowbuf, long long buf_stride,
-                          const FieldTask *tasks, int ntasks, int max_rows, const FftDesc &fd,
-                          const double2 *tw, int *nzflag, hipStream_t s);
-// Field fid goes to cache + fid * out_stride (points; a multiple of 8).
-hipError_t launch_colpass_f64(const double2 *rowbuf, long long buf_stride, double2 *cache,
-                              long long out_stride, const FieldTask *tasks, int ntasks,
-                              const FftDesc &fd, const double2 *tw, hipStream_t s);
-// split (may be null): also write bf16 hi/lo operands, 8 bytes per point.
-hipError_t launch_colpass_f32(const double2 *rowbuf, long long buf_stride, float2 *cache,
-                              void *split, long long out_stride, const FieldTask *tasks,
-                              int ntasks, const FftDesc &fd, const double2 *tw, hipStream_t s);
+// Blocks of the row and column passes: {task, batch index} pairs in device
+// memory, built by fft_blocks on the host.
+struct FftBlocks {
+    const int2 *row = nullptr;
+    const int2 *col = nullptr;
+    int nrow = 0, ncol = 0;
+};
+void fft_blocks(const FieldTask *tasks, int ntasks, const LevelDesc *h_lv, std::vector<int2> &row,
+                std::vector<int2> &col);
+// Row and column passes of every task (fields of all levels), writing the
+// FP64 cache, or the FP32 cache plus (split != null) the bf16 operands.
+// planes: NP x NP per plane (signed modes wrapped), NP >= every level's N;
+// d_lv / h_lv: the same LevelDesc table in device and host memory.
+hipError_t launch_fft_f64(const double2 *planes, int NP, double2 *cache, const FieldTask *tasks,
+                          const FftBlocks &fb, const LevelDesc *d_lv, const LevelDesc *h_lv,
+                          int nlev, const double2 *tw, int *nzflag, hipStream_t s);
+hipError_t launch_fft_f32(const double2 *planes, int NP, float2 *cache, void *split,
+                          const FieldTask *tasks, const FftBlocks &fb, const LevelDesc *d_lv,
+                          const LevelDesc *h_lv, int nlev, const double2 *tw, int *nzflag,
+                          hipStream_t s);
 
 // Contraction arithmetic.
 enum class CPrec { F64 = 0, F32 = 1, BF16X3 = 2 };

@@ -43,7 +43,15 @@ bool make_fft_desc(int N, FftDesc &fd)
     while (n % 2 == 0 && fd.nst < kMaxStages) { fd.radix[fd.nst++] = 2; n /= 2; }
     while (n % 3 == 0 && fd.nst < kMaxStages) { fd.radix[fd.nst++] = 3; n /= 3; }
     while (n % 5 == 0 && fd.nst < kMaxStages) { fd.radix[fd.nst++] = 5; n /= 5; }
-    return n == 1;
+    if (n != 1 || N >= 4096) return false;
+    fd.mN = udiv_magic((unsigned)N);
+    int p = 1;
+    for (int s = 0; s < fd.nst; ++s) {
+        fd.mNR[s] = udiv_magic((unsigned)(N / fd.radix[s]));
+        fd.mP[s] = udiv_magic((unsigned)p);
+        p *= fd.radix[s];
+    }
+    return true;
 }
 
 bool smooth235(int n)
@@ -628,7 +636,7 @@ int build_layout(const PlanData &d, CPrec prec, const bb_options &opt, int waves
                 if (V.rlo[p] < 0) continue;
                 for (int r = V.rlo[p]; r <= V.rhi[p]; ++r) {
                     const int band = d.nhi[r] > 0 ? std::min(V.NG / 2, K[r]) : -1;
-                    V.tasks.push_back(FieldTask{p, r, V.nfields, band, d.nlo[r], d.nhi[r], p * Nr + r});
+                    V.tasks.push_back(FieldTask{p, r, V.nfields, band, d.nlo[r], d.nhi[r], p * Nr + r, l});
                     V.max_rows = std::max(V.max_rows, std::min(V.NG, 2 * band + 1));
                     V.nfields++;
                 }
@@ -745,16 +753,15 @@ struct bb_estimator {
     CPrec cprec = CPrec::F64;
     hipStream_t stream = nullptr;
     std::shared_ptr<const Layout> L;
-    std::vector<FftDesc> fd;          // per level
-    std::vector<double2 *> d_tw;      // per level
-    std::vector<int> task0;           // per level: first task in d_tasks
+    std::vector<LevelDesc> lvd;       // FFT geometry per level (host copy)
+    LevelDesc *d_lvd = nullptr;
+    double2 *d_tw = nullptr;          // twiddles of every level
+    FftBlocks fb;                     // blocks of the row and column passes
     std::vector<void *> owned;
     double2 *d_field = nullptr;       // staging for bb_estimate (host input)
     double2 *d_planes = nullptr;      // NGtop x NGtop per plane
     void *d_cache = nullptr;
     void *d_split = nullptr;          // BF16x3: bf16 hi/lo operands of every field
-    double2 *d_rowbuf = nullptr;      // FP32 caches: FP64 row-pass scratch
-    long long rowbuf_pts = 0;
     FieldTask *d_tasks = nullptr;
     WorkItem *d_items = nullptr;
     TWin *d_tws = nullptr;
@@ -799,31 +806,34 @@ std::vector<double2> twiddles(int NG)
     return tw;
 }
 
-// FFTs of one level's fields into its part of the cache: FP64 in place, or
-// FP32 (plus bf16 operands) through the FP64 row-pass scratch.
-int run_level(bb_estimator *E, int l, const double2 *planes, int NP, int *nz)
+// FFT geometry of a level: radices, twiddle offset, batch sizes, cache
+// placement.
+int level_desc(const Level &V, int tw_off, LevelDesc &D)
 {
-    const Level &V = E->L->lv[l];
-    const int nt = (int)V.tasks.size();
-    if (nt == 0) return BB_OK;
-    const FieldTask *t = E->d_tasks + E->task0[l];
+    std::memset(&D, 0, sizeof D);
+    if (!make_fft_desc(V.NG, D.fd)) return set_error(BB_E_INVALID, "unsupported FFT length");
+    D.tw_off = tw_off;
+    D.rpb = fft_rows_per_block(V.NG);
+    D.cpb = fft_cols_per_block(V.NG);
+    while ((1 << D.cpb_log2) < D.cpb) ++D.cpb_log2;
+    if ((1 << D.cpb_log2) != D.cpb) return set_error(BB_E_INVALID, "internal: column batch");
+    D.max_rows = V.max_rows;
+    D.off = V.off;
+    D.stride = V.stride;
+    return BB_OK;
+}
+
+// Plane-ring fields of every level into the cache: FP64, or FP32 plus the
+// bf16 operands.
+int run_fields(bb_estimator *E, const double2 *planes, int NP, int *nz)
+{
+    const int nl = (int)E->lvd.size();
     if (E->cprec == CPrec::F64) {
-        double2 *c = (double2 *)E->d_cache + V.off;
-        HIPCHK(launch_rowpass(planes, NP, c, V.stride, t, nt, V.max_rows, E->fd[l], E->d_tw[l],
-                              nz, E->stream));
-        HIPCHK(launch_colpass_f64(c, V.stride, c, V.stride, t, nt, E->fd[l], E->d_tw[l],
-                                  E->stream));
-        return BB_OK;
-    }
-    float2 *c = (float2 *)E->d_cache + V.off;
-    void *sp = E->d_split ? (void *)((char *)E->d_split + V.off * 8) : nullptr;
-    const int per = (int)std::max<long long>(1, E->rowbuf_pts / V.npts);
-    for (int t0 = 0; t0 < nt; t0 += per) {
-        const int nb = std::min(per, nt - t0);
-        HIPCHK(launch_rowpass(planes, NP, E->d_rowbuf, -V.npts, t + t0, nb, V.max_rows, E->fd[l],
-                              E->d_tw[l], nz, E->stream));
-        HIPCHK(launch_colpass_f32(E->d_rowbuf, -V.npts, c, sp, V.stride, t + t0, nb, E->fd[l],
-                                  E->d_tw[l], E->stream));
+        HIPCHK(launch_fft_f64(planes, NP, (double2 *)E->d_cache, E->d_tasks, E->fb, E->d_lvd,
+                              E->lvd.data(), nl, E->d_tw, nz, E->stream));
+    } else {
+        HIPCHK(launch_fft_f32(planes, NP, (float2 *)E->d_cache, E->d_split, E->d_tasks, E->fb,
+                              E->d_lvd, E->lvd.data(), nl, E->d_tw, nz, E->stream));
     }
     return BB_OK;
 }
@@ -856,7 +866,7 @@ int compute_num(bb_estimator *E)
     U.nfields = Nr;
     for (int r = 0; r < Nr; ++r) {
         const int band = d.nhi[r] > 0 ? std::min(NG / 2, K[r]) : -1;
-        U.tasks.push_back(FieldTask{-1, r, r, band, d.nlo[r], d.nhi[r], 0});
+        U.tasks.push_back(FieldTask{-1, r, r, band, d.nlo[r], d.nhi[r], 0, 0});
         U.max_rows = std::max(U.max_rows, std::min(NG, 2 * band + 1));
     }
     Layout L;
@@ -875,18 +885,26 @@ int compute_num(bb_estimator *E)
     L.items.swap(W.items);
     L.tws.swap(W.tws);
     if ((rc = check_lists(L, rcount))) return rc;
-    FftDesc fd;
-    if (!make_fft_desc(NG, fd)) return set_error(BB_E_INVALID, "unsupported FFT length");
+    LevelDesc ud;
+    if ((rc = level_desc(U, 0, ud))) return rc;
     const std::vector<double2> tw = twiddles(NG);
     const long long uslots = std::max<long long>(L.part_tiles, 1) * 256;
     std::vector<void *> tmp;
     FieldTask *d_ut;
+    LevelDesc *d_ud;
+    int2 *d_ublk;
+    std::vector<int2> urow, ucol;
+    fft_blocks(U.tasks.data(), Nr, &ud, urow, ucol);
+    urow.insert(urow.end(), ucol.begin(), ucol.end());
     double2 *d_unit, *d_utw;
     WorkItem *d_uitems;
     WorkUnit *d_uunits;
     TWin *d_utws;
     double *d_upart;
     rc = dev_alloc(&d_ut, Nr, tmp);
+    if (!rc) rc = dev_alloc(&d_ud, 1, tmp);
+    if (!rc) rc = dev_alloc(&d_ublk, urow.size(), tmp);
+    if (!rc) rc = upload(d_ublk, urow.data(), sizeof(int2) * urow.size(), E->stream);
     if (!rc) rc = dev_alloc(&d_utw, NG, tmp);
     if (!rc) rc = dev_alloc(&d_unit, (size_t)L.cache_pts, tmp);
     if (!rc && hipMemsetAsync(d_unit, 0, sizeof(double2) * L.cache_pts, E->stream) != hipSuccess)
@@ -896,6 +914,7 @@ int compute_num(bb_estimator *E)
     if (!rc) rc = dev_alloc(&d_utws, L.tws.size(), tmp);
     if (!rc) rc = dev_alloc(&d_upart, (size_t)uslots, tmp);
     if (!rc) rc = upload(d_ut, U.tasks.data(), sizeof(FieldTask) * Nr, E->stream);
+    if (!rc) rc = upload(d_ud, &ud, sizeof(LevelDesc), E->stream);
     if (!rc) rc = upload(d_utw, tw.data(), sizeof(double2) * NG, E->stream);
     if (!rc && !L.items.empty())
         rc = upload(d_uitems, L.items.data(), sizeof(WorkItem) * L.items.size(), E->stream);
@@ -904,11 +923,13 @@ int compute_num(bb_estimator *E)
     if (!rc && !L.tws.empty())
         rc = upload(d_utws, L.tws.data(), sizeof(TWin) * L.tws.size(), E->stream);
     if (!rc) {
-        hipError_t e = launch_rowpass(nullptr, NG, d_unit, U.stride, d_ut, Nr, U.max_rows, fd,
-                                      d_utw, nullptr, E->stream);
-        if (e == hipSuccess)
-            e = launch_colpass_f64(d_unit, U.stride, d_unit, U.stride, d_ut, Nr, fd, d_utw,
-                                   E->stream);
+        FftBlocks ub;
+        ub.row = d_ublk;
+        ub.nrow = (int)(urow.size() - ucol.size());
+        ub.col = d_ublk + ub.nrow;
+        ub.ncol = (int)ucol.size();
+        hipError_t e = launch_fft_f64(nullptr, NG, d_unit, d_ut, ub, d_ud, &ud, 1, d_utw, nullptr,
+                                      E->stream);
         if (e == hipSuccess && rcount > 0)
             e = launch_contract(CPrec::F64, kWaves, contract_tww(CPrec::F64), d_unit, nullptr,
                                 d_uitems, d_uunits, (int)L.units.size(), d_utws, d_upart, E->stream);
@@ -943,13 +964,16 @@ size_t layout_bytes(const PlanData &d, const Layout &L)
     b += Nmax * top2 * 16;                                   // gathered planes
     b += (size_t)L.cache_pts * (L.prec == CPrec::F64 ? 16 : 8);
     if (L.prec == CPrec::BF16X3) b += (size_t)L.cache_pts * 8;
-    if (L.prec != CPrec::F64) b += 128 * top2 * 16;          // row-pass scratch
-    size_t tw = 0, nt = 0;
+    size_t tw = 0, nt = 0, nblk = 0;
     for (const Level &V : L.lv) {
         tw += V.NG;
         nt += V.tasks.size();
+        const int rpb = fft_rows_per_block(V.NG), cpb = fft_cols_per_block(V.NG);
+        for (const FieldTask &T : V.tasks)
+            nblk += (std::min(V.NG, 2 * T.rowband + 1) + rpb - 1) / rpb + (V.NG + cpb - 1) / cpb;
     }
-    b += tw * 16 + nt * sizeof(FieldTask);
+    b += nblk * sizeof(int2);
+    b += tw * 16 + nt * sizeof(FieldTask) + L.lv.size() * sizeof(LevelDesc);
     b += L.items.size() * sizeof(WorkItem) + L.tws.size() * sizeof(TWin) +
          L.units.size() * sizeof(WorkUnit);
     b += (size_t)std::max<long long>(L.part_tiles, 1) * 256 * 8;
@@ -977,21 +1001,35 @@ int estimator_init(bb_estimator *E)
     if ((rc = upload(E->d_num, E->num_host.data(), sizeof(double) * E->num_host.size(), E->stream)))
         return rc;
 
-    // Per level: FFT plan, twiddles, tasks.
+    // Per level: FFT geometry and twiddles; the tasks of every level.
     const int nl = (int)L.lv.size();
-    E->fd.resize(nl);
-    E->d_tw.resize(nl);
-    E->task0.resize(nl);
+    E->lvd.resize(nl);
     std::vector<FieldTask> tasks;
+    std::vector<double2> tw_all;
     for (int l = 0; l < nl; ++l) {
         const Level &V = L.lv[l];
-        if (!make_fft_desc(V.NG, E->fd[l])) return set_error(BB_E_INVALID, "unsupported FFT length");
+        if ((rc = level_desc(V, (int)tw_all.size(), E->lvd[l]))) return rc;
         const std::vector<double2> tw = twiddles(V.NG);
-        if ((rc = dev_alloc(&E->d_tw[l], V.NG, E->owned))) return rc;
-        if ((rc = upload(E->d_tw[l], tw.data(), sizeof(double2) * V.NG, E->stream))) return rc;
-        E->task0[l] = (int)tasks.size();
+        tw_all.insert(tw_all.end(), tw.begin(), tw.end());
         tasks.insert(tasks.end(), V.tasks.begin(), V.tasks.end());
     }
+    {
+        std::vector<int2> row, col;
+        fft_blocks(tasks.data(), (int)tasks.size(), E->lvd.data(), row, col);
+        int2 *d;
+        row.insert(row.end(), col.begin(), col.end());
+        if ((rc = dev_alloc(&d, row.size(), E->owned))) return rc;
+        if (!row.empty() && (rc = upload(d, row.data(), sizeof(int2) * row.size(), E->stream)))
+            return rc;
+        E->fb.row = d;
+        E->fb.nrow = (int)(row.size() - col.size());
+        E->fb.col = d + E->fb.nrow;
+        E->fb.ncol = (int)col.size();
+    }
+    if ((rc = dev_alloc(&E->d_tw, tw_all.size(), E->owned))) return rc;
+    if ((rc = upload(E->d_tw, tw_all.data(), sizeof(double2) * tw_all.size(), E->stream))) return rc;
+    if ((rc = dev_alloc(&E->d_lvd, nl, E->owned))) return rc;
+    if ((rc = upload(E->d_lvd, E->lvd.data(), sizeof(LevelDesc) * nl, E->stream))) return rc;
 
     // --- Device buffers --------------------------------------------------------
     const size_t top2 = (size_t)L.NGtop * L.NGtop;
@@ -1006,8 +1044,6 @@ int estimator_init(bb_estimator *E)
             if ((rc = dev_alloc(&sp, cpts, E->owned))) return rc;
             E->d_split = sp;
         }
-        E->rowbuf_pts = (long long)std::max<size_t>(1, std::min<size_t>(tasks.size(), 128)) * top2;
-        if ((rc = dev_alloc(&E->d_rowbuf, (size_t)E->rowbuf_pts, E->owned))) return rc;
     } else {
         double2 *c;
         if ((rc = dev_alloc(&c, cpts, E->owned))) return rc;
@@ -1053,10 +1089,8 @@ int estimate_device(bb_estimator *E, const double *d_field_in, double *d_bs, dou
     HIPCHK(launch_gather((const double2 *)d_field_in, E->d_planes, L.NN, L.NGtop, L.kmax,
                          d.g.Nmax, s));
     HIPCHK(hipEventRecord(E->ev[1], s));
-    for (int l = 0; l < (int)L.lv.size(); ++l) {
-        int rc = run_level(E, l, E->d_planes, L.NGtop, E->d_nz);
-        if (rc) return rc;
-    }
+    int rc = run_fields(E, E->d_planes, L.NGtop, E->d_nz);
+    if (rc) return rc;
     HIPCHK(hipEventRecord(E->ev[2], s));
     HIPCHK(launch_contract(E->cprec, L.waves, L.tww, E->d_cache, E->d_split, E->d_items,
                            E->d_units, (int)L.units.size(), E->d_tws, E->d_part, s));

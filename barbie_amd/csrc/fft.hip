@@ -5,12 +5,17 @@
 // D_{p,r}(x) = sum_{k in ring r} Delta(kx, ky, kz=p) exp(+2 pi i k.x / NN),
 // unnormalised (FFTW_BACKWARD), with the DC mode of plane 0 zeroed.
 //
-// Two passes per field, each a batch of 1-D Stockham FFTs in LDS (FP64):
+// Two passes per field, each a batch of 1-D Stockham FFTs in LDS (FP64),
+// one launch each for the fields of every grid level:
 //  - row pass: for each row i that can hold masked modes, gather the plane
-//    row, apply the ring mask (integer radius bounds), transform along j;
-//  - column pass: transform along i (rows outside the ring's band are zero),
-//    write the field to the cache (FP64, or rounded to FP32).
+//    row, apply the ring mask (integer radius bounds), transform along j,
+//    write the row into the field's cache slot;
+//  - column pass: transform along i in place (rows outside the ring's band
+//    are zero), write the field to the cache (FP64, or rounded to FP32).
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
 
 #include "device.hpp"
 
@@ -81,15 +86,16 @@ __device__ __forceinline__ void butterfly<5>(double2 *v)
 // y[(i-k) R + k + t p].
 template <int R>
 __device__ __forceinline__ void fft_stage(const double2 *a, double2 *b, int nb, int N, int p,
-                                          const double2 *__restrict__ tw)
+                                          unsigned long long mNR, unsigned long long mP,
+                                          const double2 *tw)
 {
     const unsigned NR = (unsigned)(N / R);
     const unsigned tstep = (unsigned)(N / (p * R));
     const unsigned total = (unsigned)nb * NR;
     for (unsigned idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const unsigned bt = idx / NR;
+        const unsigned bt = udiv(idx, mNR);
         const unsigned i = idx - bt * NR;
-        const unsigned k = i % (unsigned)p;
+        const unsigned k = i - udiv(i, mP) * (unsigned)p;
         const double2 *x = a + bt * (unsigned)N;
         double2 *y = b + bt * (unsigned)N;
         double2 v[R];
@@ -107,20 +113,20 @@ __device__ __forceinline__ void fft_stage(const double2 *a, double2 *b, int nb, 
 }
 
 // nb transforms of length N stored back to back in a (LDS); b is scratch of
-// the same size.  Returns the buffer holding the result (self-sorting
-// Stockham, radices in fd.radix order).
-__device__ double2 *lds_fft(double2 *a, double2 *b, int nb, const FftDesc &fd,
-                            const double2 *__restrict__ tw)
+// the same size, tw the N twiddles (LDS).  Returns the buffer holding the
+// result (self-sorting Stockham, radices in fd.radix order).
+__device__ double2 *lds_fft(double2 *a, double2 *b, int nb, const FftDesc &fd, const double2 *tw)
 {
     const int N = fd.N;
     int p = 1;
     for (int s = 0; s < fd.nst; ++s) {
         const int R = fd.radix[s];
+        const unsigned long long mNR = fd.mNR[s], mP = fd.mP[s];
         switch (R) {
-        case 4: fft_stage<4>(a, b, nb, N, p, tw); break;
-        case 2: fft_stage<2>(a, b, nb, N, p, tw); break;
-        case 3: fft_stage<3>(a, b, nb, N, p, tw); break;
-        default: fft_stage<5>(a, b, nb, N, p, tw); break;
+        case 4: fft_stage<4>(a, b, nb, N, p, mNR, mP, tw); break;
+        case 2: fft_stage<2>(a, b, nb, N, p, mNR, mP, tw); break;
+        case 3: fft_stage<3>(a, b, nb, N, p, mNR, mP, tw); break;
+        default: fft_stage<5>(a, b, nb, N, p, mNR, mP, tw); break;
         }
         __syncthreads();
         double2 *tmp = a; a = b; b = tmp;
@@ -173,31 +179,50 @@ hipError_t launch_gather(const double2 *field, double2 *planes, int NN, int NG, 
     return hipGetLastError();
 }
 
-constexpr int kRowsPerBlock = 4;
+// Precision of the cache the passes write: FP64, or FP32 (the column pass
+// still transforms in FP64; the row-pass output is rounded to FP32 once).
+template <typename C2>
+__device__ __forceinline__ C2 store_as(double2 v);
+template <>
+__device__ __forceinline__ double2 store_as<double2>(double2 v) { return v; }
+template <>
+__device__ __forceinline__ float2 store_as<float2>(double2 v)
+{
+    return make_float2((float)v.x, (float)v.y);
+}
+__device__ __forceinline__ double2 load_as(double2 v) { return v; }
+__device__ __forceinline__ double2 load_as(float2 v) { return make_double2(v.x, v.y); }
 
-// Row pass.  grid = (ceil(max_rows / kRowsPerBlock), ntasks).
-// Output row i of task t goes to rowbuf[slot * buf_stride + i * N], with
-// slot = task.fid when use_fid, else t.  The plane values come from the
+// Row pass over the tasks of every level, one block per entry of `blk`:
+// block {t, x} transforms rows [x * rpb, x * rpb + rpb) of task t's row band.
+// Row i of field fid is written in place of the level's cache, at
+// cache + off + fid * stride + i * N.  The plane values come from the
 // NP x NP plane grid (NP >= N): a mode (ia, ja) inside the ring satisfies
 // |ia|, |ja| < N/2 and sits at (ia mod NP, ja mod NP) there.
+template <typename C2>
 __global__ __launch_bounds__(256) void k_rowpass(const double2 *__restrict__ planes, int NP,
-                                                 double2 *__restrict__ rowbuf,
-                                                 long long buf_stride, int use_fid,
-                                                 const FieldTask *__restrict__ tasks, FftDesc fd,
-                                                 const double2 *__restrict__ tw,
+                                                 C2 *__restrict__ cache,
+                                                 const FieldTask *__restrict__ tasks,
+                                                 const int2 *__restrict__ blk,
+                                                 const LevelDesc *__restrict__ lv,
+                                                 const double2 *__restrict__ tw_all,
                                                  int *__restrict__ nzflag)
 {
     extern __shared__ double2 lds[];
-    const int N = fd.N;
-    const FieldTask T = tasks[blockIdx.y];
+    const int2 B = blk[blockIdx.x];
+    const FieldTask T = tasks[B.x];
+    const LevelDesc &L = lv[T.level];
+    const int N = L.fd.N, rpb = L.rpb;
     const int nrows = min(N, 2 * T.rowband + 1);
-    const int q0 = blockIdx.x * kRowsPerBlock;
+    const int q0 = B.y * rpb;
     if (q0 >= nrows) return;
-    const int nb = min(kRowsPerBlock, nrows - q0);
-    double2 *a = lds, *b = lds + kRowsPerBlock * N;
+    const int nb = min(rpb, nrows - q0);
+    double2 *twl = lds, *a = lds + N, *b = a + rpb * N;
+    for (int t = threadIdx.x; t < N; t += blockDim.x) twl[t] = tw_all[L.tw_off + t];
+    const unsigned long long mN = L.fd.mN;
     int nz = 0;
     for (int idx = threadIdx.x; idx < nb * N; idx += blockDim.x) {
-        const int bt = idx / N, j = idx - bt * N;
+        const int bt = (int)udiv((unsigned)idx, mN), j = idx - bt * N;
         const int q = q0 + bt;
         const int i = (q <= T.rowband) ? q : N - nrows + q;
         const int ia = (i > N / 2) ? (i - N) : i;
@@ -218,93 +243,67 @@ __global__ __launch_bounds__(256) void k_rowpass(const double2 *__restrict__ pla
     }
     if (nzflag && __any(nz) && (threadIdx.x & 63) == 0) atomicOr(&nzflag[T.zid], 1);
     __syncthreads();
-    double2 *res = lds_fft(a, b, nb, fd, tw);
-    const long long slot = use_fid ? T.fid : blockIdx.y;
+    double2 *res = lds_fft(a, b, nb, L.fd, twl);
+    C2 *dst = cache + L.off + (long long)T.fid * L.stride;
     for (int idx = threadIdx.x; idx < nb * N; idx += blockDim.x) {
-        const int bt = idx / N, j = idx - bt * N;
+        const int bt = (int)udiv((unsigned)idx, mN), j = idx - bt * N;
         const int q = q0 + bt;
         const int i = (q <= T.rowband) ? q : N - nrows + q;
-        rowbuf[slot * buf_stride + (long long)i * N + j] = res[bt * N + j];
+        dst[(long long)i * N + j] = store_as<C2>(res[bt * N + j]);
     }
 }
 
-hipError_t launch_rowpass(const double2 *planes, int NP, double2 *rowbuf, long long buf_stride,
-                          const FieldTask *tasks, int ntasks, int max_rows, const FftDesc &fd,
-                          const double2 *tw, int *nzflag, hipStream_t s)
-{
-    // rowbuf == nullptr is not allowed; use_fid is encoded by the caller via
-    // buf_stride sign: negative stride means slot = task index.
-    int use_fid = buf_stride > 0 ? 1 : 0;
-    long long stride = buf_stride > 0 ? buf_stride : -buf_stride;
-    dim3 grid((max_rows + kRowsPerBlock - 1) / kRowsPerBlock, ntasks);
-    size_t lds = 2 * kRowsPerBlock * fd.N * sizeof(double2);
-    (void)hipFuncSetAttribute((const void *)k_rowpass, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds);
-    hipLaunchKernelGGL(k_rowpass, grid, dim3(256), lds, s, planes, NP, rowbuf, stride, use_fid,
-                       tasks, fd, tw, nzflag);
-    return hipGetLastError();
-}
-
-constexpr int kColsPerBlock = 8;
-
-template <typename OUT2>
-__device__ __forceinline__ OUT2 to_out(double2 v);
-template <>
-__device__ __forceinline__ double2 to_out<double2>(double2 v) { return v; }
-template <>
-__device__ __forceinline__ float2 to_out<float2>(double2 v)
-{
-    return make_float2((float)v.x, (float)v.y);
-}
-
-// Column pass.  grid = (ceil(N / kColsPerBlock), ntasks).  Reads the row-pass
-// output of task t (slot as in the row pass), transforms along i and writes
-// point i*N+j of field fid at cache + fid * out_stride.  Rows outside the
-// ring's band were never written by the row pass and are taken as zero.
-// With `split` set (FP32 caches), the field is also written as bf16 hi/lo
-// operands: for every 8 consecutive points, 32 bf16 values
-// [Re hi x8][Re lo x8][Im hi x8][Im lo x8], with hi = bf16(v) and
-// lo = bf16(v - hi) of the FP32 value v; out_stride is a multiple of 8, so
-// no group of 8 straddles two fields (points past N*N stay as they are).
-template <typename OUT2>
-__global__ __launch_bounds__(256) void k_colpass(const double2 *__restrict__ rowbuf,
-                                                 long long buf_stride, int use_fid,
-                                                 OUT2 *__restrict__ cache,
+// Column pass over the tasks of every level, in place, one block per entry
+// of `blk`: block {t, x} transforms columns [x * cpb, x * cpb + cpb) of task
+// t's field along i.  Rows outside the
+// ring's band were not written by the row pass and are taken as zero.  With
+// `split` set (FP32 caches), the field is also written as bf16 hi/lo
+// operands at the same point offsets: for every 8 consecutive points, 32
+// bf16 values [Re hi x8][Re lo x8][Im hi x8][Im lo x8], with hi = bf16(v)
+// and lo = bf16(v - hi) of the FP32 value v.  Field strides are multiples of
+// 8, so no group of 8 straddles two fields.
+template <typename C2>
+__global__ __launch_bounds__(256) void k_colpass(C2 *__restrict__ cache,
                                                  __bf16 *__restrict__ split,
-                                                 long long out_stride,
-                                                 const FieldTask *__restrict__ tasks, FftDesc fd,
-                                                 const double2 *__restrict__ tw)
+                                                 const FieldTask *__restrict__ tasks,
+                                                 const int2 *__restrict__ blk,
+                                                 const LevelDesc *__restrict__ lv,
+                                                 const double2 *__restrict__ tw_all)
 {
     extern __shared__ double2 lds[];
-    const int N = fd.N;
-    const FieldTask T = tasks[blockIdx.y];
+    const int2 B = blk[blockIdx.x];
+    const FieldTask T = tasks[B.x];
+    const LevelDesc &L = lv[T.level];
+    const int N = L.fd.N, cpb = L.cpb;
+    const int j0 = B.y * cpb;
+    if (j0 >= N) return;
     const int nrows = min(N, 2 * T.rowband + 1);
-    const int j0 = blockIdx.x * kColsPerBlock;
-    const long long slot = use_fid ? T.fid : blockIdx.y;
-    const double2 *src = rowbuf + slot * buf_stride;
-    double2 *a = lds, *b = lds + kColsPerBlock * N;
+    const long long base = L.off + (long long)T.fid * L.stride;
+    C2 *fld = cache + base;
+    double2 *twl = lds, *a = lds + N, *b = a + cpb * N;
+    for (int t = threadIdx.x; t < N; t += blockDim.x) twl[t] = tw_all[L.tw_off + t];
+    const int sh = L.cpb_log2;
     // layout a[c * N + i]
-    for (int idx = threadIdx.x; idx < kColsPerBlock * N; idx += blockDim.x) {
-        const int i = idx / kColsPerBlock, c = idx - i * kColsPerBlock;
+    for (int idx = threadIdx.x; idx < cpb * N; idx += blockDim.x) {
+        const int i = idx >> sh, c = idx & (cpb - 1);
         const int ia = (i > N / 2) ? (i - N) : i;
         const bool inband = (ia <= T.rowband && -ia <= T.rowband) || nrows == N;
         double2 v = make_double2(0.0, 0.0);
-        if (inband && j0 + c < N) v = src[(long long)i * N + j0 + c];
+        if (inband && j0 + c < N) v = load_as(fld[(long long)i * N + j0 + c]);
         a[c * N + i] = v;
     }
     __syncthreads();
-    double2 *res = lds_fft(a, b, kColsPerBlock, fd, tw);
-    OUT2 *dst = cache + (long long)T.fid * out_stride;
-    for (int idx = threadIdx.x; idx < kColsPerBlock * N; idx += blockDim.x) {
-        const int i = idx / kColsPerBlock, c = idx - i * kColsPerBlock;
+    double2 *res = lds_fft(a, b, cpb, L.fd, twl);
+    for (int idx = threadIdx.x; idx < cpb * N; idx += blockDim.x) {
+        const int i = idx >> sh, c = idx & (cpb - 1);
         if (j0 + c >= N) continue;
         const long long x = (long long)i * N + j0 + c;
-        const OUT2 o = to_out<OUT2>(res[c * N + i]);
-        dst[x] = o;
+        const C2 o = store_as<C2>(res[c * N + i]);
+        fld[x] = o;
         if (split) {
             const float vr = (float)o.x, vi = (float)o.y;
             const __bf16 hr = (__bf16)vr, hi = (__bf16)vi;
-            __bf16 *g = split + (long long)T.fid * out_stride * 4 + (x >> 3) * 32 + (x & 7);
+            __bf16 *g = split + base * 4 + (x >> 3) * 32 + (x & 7);
             g[0] = hr;
             g[8] = (__bf16)(vr - (float)hr);
             g[16] = hi;
@@ -313,37 +312,75 @@ __global__ __launch_bounds__(256) void k_colpass(const double2 *__restrict__ row
     }
 }
 
-template <typename OUT2>
-static hipError_t launch_colpass(const double2 *rowbuf, long long buf_stride, OUT2 *cache,
-                                 __bf16 *split, long long out_stride, const FieldTask *tasks,
-                                 int ntasks, const FftDesc &fd, const double2 *tw, hipStream_t s)
+// Transforms per block: about 1024 points of work for 256 threads, within
+// the 160 KiB of LDS up to N = 600 ((2 b + 1) N double2 values).
+int fft_rows_per_block(int N) { return N >= 256 ? 4 : N >= 128 ? 8 : N >= 64 ? 16 : 32; }
+int fft_cols_per_block(int N) { return N >= 512 ? 4 : N >= 128 ? 8 : N >= 64 ? 16 : 32; }
+
+template <typename C2>
+static hipError_t launch_fft(const double2 *planes, int NP, C2 *cache, __bf16 *split,
+                             const FieldTask *tasks, const FftBlocks &fb, const LevelDesc *d_lv,
+                             const LevelDesc *h_lv, int nlev, const double2 *tw, int *nzflag,
+                             hipStream_t s)
 {
-    if (out_stride < (long long)fd.N * fd.N || out_stride % 8 != 0) return hipErrorInvalidValue;
-    int use_fid = buf_stride > 0 ? 1 : 0;
-    long long stride = buf_stride > 0 ? buf_stride : -buf_stride;
-    dim3 grid((fd.N + kColsPerBlock - 1) / kColsPerBlock, ntasks);
-    size_t lds = 2 * kColsPerBlock * fd.N * sizeof(double2);
-    (void)hipFuncSetAttribute((const void *)k_colpass<OUT2>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_colpass<OUT2>, grid, dim3(256), lds, s, rowbuf, stride, use_fid, cache,
-                       split, out_stride, tasks, fd, tw);
+    size_t lds_r = 0, lds_c = 0;
+    for (int l = 0; l < nlev; ++l) {
+        const LevelDesc &L = h_lv[l];
+        const int N = L.fd.N;
+        if (L.stride < (long long)N * N || L.stride % 8 != 0 || L.rpb < 1 || L.cpb < 1)
+            return hipErrorInvalidValue;
+        if (L.cpb != (1 << L.cpb_log2) || N >= 4096 || (size_t)L.rpb * N >= (1u << 20) ||
+            (size_t)L.cpb * N >= (1u << 20))
+            return hipErrorInvalidValue;
+        lds_r = std::max(lds_r, (2 * (size_t)L.rpb + 1) * N * sizeof(double2));
+        lds_c = std::max(lds_c, (2 * (size_t)L.cpb + 1) * N * sizeof(double2));
+    }
+    if (fb.nrow > 0) {
+        (void)hipFuncSetAttribute((const void *)k_rowpass<C2>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_r);
+        hipLaunchKernelGGL(k_rowpass<C2>, dim3(fb.nrow), dim3(256), lds_r, s, planes, NP, cache,
+                           tasks, fb.row, d_lv, tw, nzflag);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (fb.ncol > 0) {
+        (void)hipFuncSetAttribute((const void *)k_colpass<C2>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_c);
+        hipLaunchKernelGGL(k_colpass<C2>, dim3(fb.ncol), dim3(256), lds_c, s, cache, split, tasks,
+                           fb.col, d_lv, tw);
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_colpass_f64(const double2 *rowbuf, long long buf_stride, double2 *cache,
-                              long long out_stride, const FieldTask *tasks, int ntasks,
-                              const FftDesc &fd, const double2 *tw, hipStream_t s)
+void fft_blocks(const FieldTask *tasks, int ntasks, const LevelDesc *h_lv, std::vector<int2> &row,
+                std::vector<int2> &col)
 {
-    return launch_colpass<double2>(rowbuf, buf_stride, cache, nullptr, out_stride, tasks, ntasks,
-                                   fd, tw, s);
+    row.clear();
+    col.clear();
+    for (int t = 0; t < ntasks; ++t) {
+        const LevelDesc &L = h_lv[tasks[t].level];
+        const int N = L.fd.N;
+        const int nrows = std::min(N, 2 * tasks[t].rowband + 1);
+        for (int x = 0; x * L.rpb < nrows; ++x) row.push_back(make_int2(t, x));
+        for (int x = 0; x * L.cpb < N; ++x) col.push_back(make_int2(t, x));
+    }
 }
 
-hipError_t launch_colpass_f32(const double2 *rowbuf, long long buf_stride, float2 *cache,
-                              void *split, long long out_stride, const FieldTask *tasks,
-                              int ntasks, const FftDesc &fd, const double2 *tw, hipStream_t s)
+hipError_t launch_fft_f64(const double2 *planes, int NP, double2 *cache, const FieldTask *tasks,
+                          const FftBlocks &fb, const LevelDesc *d_lv, const LevelDesc *h_lv,
+                          int nlev, const double2 *tw, int *nzflag, hipStream_t s)
 {
-    return launch_colpass<float2>(rowbuf, buf_stride, cache, (__bf16 *)split, out_stride, tasks,
-                                  ntasks, fd, tw, s);
+    return launch_fft<double2>(planes, NP, cache, nullptr, tasks, fb, d_lv, h_lv, nlev, tw,
+                               nzflag, s);
+}
+
+hipError_t launch_fft_f32(const double2 *planes, int NP, float2 *cache, void *split,
+                          const FieldTask *tasks, const FftBlocks &fb, const LevelDesc *d_lv,
+                          const LevelDesc *h_lv, int nlev, const double2 *tw, int *nzflag,
+                          hipStream_t s)
+{
+    return launch_fft<float2>(planes, NP, cache, (__bf16 *)split, tasks, fb, d_lv, h_lv, nlev, tw,
+                              nzflag, s);
 }
 
 }  // namespace bb
