@@ -75,9 +75,16 @@ __global__ __launch_bounds__(kBinThreads) void k_binning(const int32_t *__restri
                                                 const double *__restrict__ avg, BinParams bp,
                                                 double *__restrict__ out_bs,
                                                 double *__restrict__ out_norm,
-                                                double *__restrict__ out_biana)
+                                                double *__restrict__ out_biana,
+                                                BatchStrides st)
 {
     const int b = blockIdx.x;
+    const int fb = blockIdx.y;   // field of the batch
+    S += fb * st.part;
+    nzflag += fb * st.nz;
+    if (out_bs) out_bs += fb * st.bins;
+    if (out_norm) out_norm += fb * st.bins;
+    if (out_biana) out_biana += fb * st.bins;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     double abs_[16], anm[16], abi[16];
 #pragma unroll
@@ -157,10 +164,12 @@ __global__ __launch_bounds__(kBinThreads) void k_binning(const int32_t *__restri
 hipError_t launch_binning(const int32_t *seg, const BinRec *recs, const double *S,
                           const double *num, const int *nzflag, const double *avg,
                           const BinParams &bp, double *bins_bs, double *bins_norm,
-                          double *bins_biana, hipStream_t s)
+                          double *bins_biana, const BatchStrides &st, int nf, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_binning, dim3(bp.nbins), dim3(kBinThreads), 0, s, seg, recs, S, num,
-                       nzflag, avg, bp, bins_bs, bins_norm, bins_biana);
+    if (bp.nbins <= 0) return hipSuccess;
+    if (nf < 1 || nf > 65535) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_binning, dim3(bp.nbins, nf), dim3(kBinThreads), 0, s, seg, recs, S, num,
+                       nzflag, avg, bp, bins_bs, bins_norm, bins_biana, st);
     return hipGetLastError();
 }
 

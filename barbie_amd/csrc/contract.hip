@@ -155,8 +155,11 @@ struct K4<float> {
 template <typename R, int NW, int TWW, int MINB>
 __global__ __launch_bounds__(64 * NW, MINB) void k_contract_k4(
     const typename K4<R>::C2 *__restrict__ cache, const WorkItem *__restrict__ items,
-    const WorkUnit *__restrict__ units, const TWin *__restrict__ tws, double *__restrict__ out)
+    const WorkUnit *__restrict__ units, const TWin *__restrict__ tws, double *__restrict__ out,
+    BatchStrides bs)
 {
+    cache += blockIdx.y * bs.cache;   // field of the batch
+    out += blockIdx.y * bs.part;
     using T = K4<R>;
     using C2 = typename T::C2;
     using Acc = typename T::Acc;
@@ -324,8 +327,11 @@ template <int NW>
 __global__ __launch_bounds__(64 * NW, 8 / NW) void k_contract_bf16x3(
     const float2 *__restrict__ cache, const __bf16 *__restrict__ splitc,
     const WorkItem *__restrict__ items, const WorkUnit *__restrict__ units,
-    const TWin *__restrict__ tws, double *__restrict__ out)
+    const TWin *__restrict__ tws, double *__restrict__ out, BatchStrides bs)
 {
+    cache += blockIdx.y * bs.cache;   // field of the batch
+    splitc += blockIdx.y * bs.cache * 4;
+    out += blockIdx.y * bs.part;
     constexpr int TWW = kTWW32;
     extern __shared__ __align__(16) unsigned char lds[];
     int *row_slot = reinterpret_cast<int *>(lds);
@@ -428,44 +434,49 @@ int contract_chunk(CPrec p) { return p == CPrec::F64 ? kRowBytes / 16 : kRowByte
 template <int NW>
 static hipError_t launch_nw(CPrec p, int tww, const void *cache, const void *split,
                             const WorkItem *items, const WorkUnit *units, int nunits,
-                            const TWin *tws, double *out, hipStream_t s)
+                            const TWin *tws, double *out, const BatchStrides &bs, int nf,
+                            hipStream_t s)
 {
-    dim3 grid((unsigned)nunits);
+    dim3 grid((unsigned)nunits, (unsigned)nf);
     const size_t lds = kTableBytes + (size_t)(kRowCap + 1) * kPitch;   // + spare ring
     if (p == CPrec::F64 && tww <= kTWWF64Narrow) {
         auto k = k_contract_k4<double, NW, kTWWF64Narrow, 16 / NW>;
         (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds);
         hipLaunchKernelGGL(k, grid, dim3(64 * NW), lds, s, (const double2 *)cache, items, units,
-                           tws, out);
+                           tws, out, bs);
     } else if (p == CPrec::F64) {
         auto k = k_contract_k4<double, NW, kTWW, 8 / NW>;
         (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds);
         hipLaunchKernelGGL(k, grid, dim3(64 * NW), lds, s, (const double2 *)cache, items, units,
-                           tws, out);
+                           tws, out, bs);
     } else if (p == CPrec::F32) {
         auto k = k_contract_k4<float, NW, kTWWF32, 8 / NW>;
         (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds);
         hipLaunchKernelGGL(k, grid, dim3(64 * NW), lds, s, (const float2 *)cache, items, units,
-                           tws, out);
+                           tws, out, bs);
     } else {
         (void)hipFuncSetAttribute((const void *)k_contract_bf16x3<NW>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL((k_contract_bf16x3<NW>), grid, dim3(64 * NW), lds, s,
-                           (const float2 *)cache, (const __bf16 *)split, items, units, tws, out);
+                           (const float2 *)cache, (const __bf16 *)split, items, units, tws, out,
+                           bs);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_contract(CPrec p, int waves, int tww, const void *cache, const void *split,
                            const WorkItem *items, const WorkUnit *units, int nunits,
-                           const TWin *tws, double *out, hipStream_t s)
+                           const TWin *tws, double *out, const BatchStrides &bs, int nf,
+                           hipStream_t s)
 {
     if (nunits <= 0) return hipSuccess;
-    if (waves == 4) return launch_nw<4>(p, tww, cache, split, items, units, nunits, tws, out, s);
-    return launch_nw<8>(p, tww, cache, split, items, units, nunits, tws, out, s);
+    if (nf < 1 || nf > 65535) return hipErrorInvalidValue;
+    if (waves == 4)
+        return launch_nw<4>(p, tww, cache, split, items, units, nunits, tws, out, bs, nf, s);
+    return launch_nw<8>(p, tww, cache, split, items, units, nunits, tws, out, bs, nf, s);
 }
 
 }  // namespace bb

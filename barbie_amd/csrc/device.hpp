@@ -134,8 +134,19 @@ struct BinParams {
 };
 
 // Launchers (fft.hip, contract.hip, binning.hip).
-hipError_t launch_gather(const double2 *field, double2 *planes, int NN, int NG, int kmax,
-                         int Nmax, hipStream_t s);
+// nf fields of NN * NN * (NN/2+1) complex values each, back to back.
+hipError_t launch_gather(const double2 *fields, double2 *planes, int NN, int NG, int kmax,
+                         int Nmax, int nf, hipStream_t s);
+// Per-field strides of a batched launch (blockIdx.y, or blockIdx.z for the
+// gather, selects the field): elements between consecutive fields' buffers.
+struct BatchStrides {
+    long long planes = 0;   // double2: Nmax * NG * NG
+    long long cache = 0;    // cache elements (double2 or float2; bf16 operands x4)
+    long long nz = 0;       // nonzero flags
+    long long part = 0;     // partial sums (doubles)
+    long long bins = 0;     // bin outputs (doubles)
+};
+
 // Blocks of the row and column passes: {task, batch index} pairs in device
 // memory, built by fft_blocks on the host.
 struct FftBlocks {
@@ -151,11 +162,12 @@ void fft_blocks(const FieldTask *tasks, int ntasks, const LevelDesc *h_lv, std::
 // d_lv / h_lv: the same LevelDesc table in device and host memory.
 hipError_t launch_fft_f64(const double2 *planes, int NP, double2 *cache, const FieldTask *tasks,
                           const FftBlocks &fb, const LevelDesc *d_lv, const LevelDesc *h_lv,
-                          int nlev, const double2 *tw, int *nzflag, hipStream_t s);
+                          int nlev, const double2 *tw, int *nzflag, const BatchStrides &bs,
+                          int nf, hipStream_t s);
 hipError_t launch_fft_f32(const double2 *planes, int NP, float2 *cache, void *split,
                           const FieldTask *tasks, const FftBlocks &fb, const LevelDesc *d_lv,
                           const LevelDesc *h_lv, int nlev, const double2 *tw, int *nzflag,
-                          hipStream_t s);
+                          const BatchStrides &bs, int nf, hipStream_t s);
 
 // Contraction arithmetic.
 enum class CPrec { F64 = 0, F32 = 1, BF16X3 = 2 };
@@ -173,11 +185,12 @@ inline int contract_tww(CPrec p)
 // lists must be packed for both).
 hipError_t launch_contract(CPrec p, int waves, int tww, const void *cache, const void *split,
                            const WorkItem *items, const WorkUnit *units, int nunits,
-                           const TWin *tws, double *out, hipStream_t s);
+                           const TWin *tws, double *out, const BatchStrides &bs, int nf,
+                           hipStream_t s);
 
 hipError_t launch_binning(const int32_t *seg, const BinRec *recs, const double *S,
                           const double *num, const int *nzflag, const double *avg,
                           const BinParams &bp, double *bins_bs, double *bins_norm,
-                          double *bins_biana, hipStream_t s);
+                          double *bins_biana, const BatchStrides &bs, int nf, hipStream_t s);
 
 }  // namespace bb

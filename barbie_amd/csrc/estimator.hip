@@ -758,7 +758,11 @@ struct bb_estimator {
     double2 *d_tw = nullptr;          // twiddles of every level
     FftBlocks fb;                     // blocks of the row and column passes
     std::vector<void *> owned;
+    int batch = 1;                    // fields per launch held in device memory
+    BatchStrides bst;                 // per-field buffer strides
+    bool pending = false;             // an enqueued batch not yet waited for
     double2 *d_field = nullptr;       // staging for bb_estimate (host input)
+    int field_cap = 0;                // fields the staging holds
     double2 *d_planes = nullptr;      // NGtop x NGtop per plane
     void *d_cache = nullptr;
     void *d_split = nullptr;          // BF16x3: bf16 hi/lo operands of every field
@@ -825,15 +829,16 @@ int level_desc(const Level &V, int tw_off, LevelDesc &D)
 
 // Plane-ring fields of every level into the cache: FP64, or FP32 plus the
 // bf16 operands.
-int run_fields(bb_estimator *E, const double2 *planes, int NP, int *nz)
+int run_fields(bb_estimator *E, const double2 *planes, int NP, int *nz, int nf)
 {
     const int nl = (int)E->lvd.size();
     if (E->cprec == CPrec::F64) {
         HIPCHK(launch_fft_f64(planes, NP, (double2 *)E->d_cache, E->d_tasks, E->fb, E->d_lvd,
-                              E->lvd.data(), nl, E->d_tw, nz, E->stream));
+                              E->lvd.data(), nl, E->d_tw, nz, E->bst, nf, E->stream));
     } else {
         HIPCHK(launch_fft_f32(planes, NP, (float2 *)E->d_cache, E->d_split, E->d_tasks, E->fb,
-                              E->d_lvd, E->lvd.data(), nl, E->d_tw, nz, E->stream));
+                              E->d_lvd, E->lvd.data(), nl, E->d_tw, nz, E->bst, nf,
+                              E->stream));
     }
     return BB_OK;
 }
@@ -928,11 +933,13 @@ int compute_num(bb_estimator *E)
         ub.nrow = (int)(urow.size() - ucol.size());
         ub.col = d_ublk + ub.nrow;
         ub.ncol = (int)ucol.size();
+        const BatchStrides one;
         hipError_t e = launch_fft_f64(nullptr, NG, d_unit, d_ut, ub, d_ud, &ud, 1, d_utw, nullptr,
-                                      E->stream);
+                                      one, 1, E->stream);
         if (e == hipSuccess && rcount > 0)
             e = launch_contract(CPrec::F64, kWaves, contract_tww(CPrec::F64), d_unit, nullptr,
-                                d_uitems, d_uunits, (int)L.units.size(), d_utws, d_upart, E->stream);
+                                d_uitems, d_uunits, (int)L.units.size(), d_utws, d_upart, one, 1,
+                                E->stream);
         if (e != hipSuccess) rc = set_error(BB_E_DEVICE, hipGetErrorString(e));
     }
     std::vector<double> part((size_t)uslots);
@@ -956,14 +963,20 @@ int compute_num(bb_estimator *E)
 }
 
 // Device bytes held by an estimator of layout L (steady state).
-size_t layout_bytes(const PlanData &d, const Layout &L)
+// Device bytes held by an estimator of layout L with F fields per batch:
+// per-field buffers times F, shared work lists once, and the staging of one
+// host field (bb_estimate; bb_estimate_batch grows it to the batch).
+size_t layout_bytes(const PlanData &d, const Layout &L, size_t F)
 {
     const size_t NN = d.g.NN, Nr = d.g.Nr, Nmax = d.g.Nmax;
     const size_t top2 = (size_t)L.NGtop * L.NGtop;
-    size_t b = NN * NN * (NN / 2 + 1) * 16;                  // bb_estimate staging
-    b += Nmax * top2 * 16;                                   // gathered planes
-    b += (size_t)L.cache_pts * (L.prec == CPrec::F64 ? 16 : 8);
-    if (L.prec == CPrec::BF16X3) b += (size_t)L.cache_pts * 8;
+    size_t b = NN * NN * (NN / 2 + 1) * 16;                  // host-field staging
+    size_t per = Nmax * top2 * 16;                           // gathered planes
+    per += (size_t)L.cache_pts * (L.prec == CPrec::F64 ? 16 : 8);
+    if (L.prec == CPrec::BF16X3) per += (size_t)L.cache_pts * 8;
+    per += (size_t)std::max<long long>(L.part_tiles, 1) * 256 * 8;
+    per += Nmax * Nr * 4 + (size_t)3 * d.nbins * BB_NPOLES * 8;
+    b += per * F;
     size_t tw = 0, nt = 0, nblk = 0;
     for (const Level &V : L.lv) {
         tw += V.NG;
@@ -976,9 +989,8 @@ size_t layout_bytes(const PlanData &d, const Layout &L)
     b += tw * 16 + nt * sizeof(FieldTask) + L.lv.size() * sizeof(LevelDesc);
     b += L.items.size() * sizeof(WorkItem) + L.tws.size() * sizeof(TWin) +
          L.units.size() * sizeof(WorkUnit);
-    b += (size_t)std::max<long long>(L.part_tiles, 1) * 256 * 8;
     b += L.recs.size() * sizeof(BinRec) + L.seg.size() * 4;
-    b += d.tri.size() / 3 * 8 + Nmax * Nr * 4 + Nr * 8 + (size_t)3 * d.nbins * BB_NPOLES * 8;
+    b += d.tri.size() / 3 * 8 + Nr * 8;
     return b;
 }
 
@@ -1031,10 +1043,18 @@ int estimator_init(bb_estimator *E)
     if ((rc = dev_alloc(&E->d_lvd, nl, E->owned))) return rc;
     if ((rc = upload(E->d_lvd, E->lvd.data(), sizeof(LevelDesc) * nl, E->stream))) return rc;
 
-    // --- Device buffers --------------------------------------------------------
+    // --- Device buffers (one copy per field of a batch) ---------------------------
+    const size_t F = (size_t)E->batch;
     const size_t top2 = (size_t)L.NGtop * L.NGtop;
-    const size_t cpts = (size_t)std::max<long long>(L.cache_pts, 1);
-    if ((rc = dev_alloc(&E->d_planes, (size_t)Nmax * top2, E->owned))) return rc;
+    const size_t cpts1 = (size_t)std::max<long long>(L.cache_pts, 1);
+    const size_t cpts = cpts1 * F;
+    const size_t nslots1 = (size_t)std::max<long long>(L.part_tiles, 1) * 256;
+    E->bst.planes = (long long)(Nmax * top2);
+    E->bst.cache = (long long)cpts1;
+    E->bst.nz = (long long)Nmax * Nr;
+    E->bst.part = (long long)nslots1;
+    E->bst.bins = (long long)d.nbins * BB_NPOLES;
+    if ((rc = dev_alloc(&E->d_planes, (size_t)Nmax * top2 * F, E->owned))) return rc;
     if (E->cprec != CPrec::F64) {
         float2 *c;
         if ((rc = dev_alloc(&c, cpts, E->owned))) return rc;
@@ -1049,7 +1069,7 @@ int estimator_init(bb_estimator *E)
         if ((rc = dev_alloc(&c, cpts, E->owned))) return rc;
         E->d_cache = c;
     }
-    const size_t nslots = (size_t)std::max<long long>(L.part_tiles, 1) * 256;
+    const size_t nslots = nslots1 * F;
     if ((rc = dev_alloc(&E->d_tasks, tasks.size(), E->owned))) return rc;
     if ((rc = dev_alloc(&E->d_items, L.items.size(), E->owned))) return rc;
     if ((rc = dev_alloc(&E->d_tws, L.tws.size(), E->owned))) return rc;
@@ -1057,9 +1077,9 @@ int estimator_init(bb_estimator *E)
     if ((rc = dev_alloc(&E->d_part, nslots, E->owned))) return rc;
     if ((rc = dev_alloc(&E->d_recs, L.recs.size(), E->owned))) return rc;
     if ((rc = dev_alloc(&E->d_seg, L.seg.size(), E->owned))) return rc;
-    if ((rc = dev_alloc(&E->d_nz, (size_t)Nmax * Nr, E->owned))) return rc;
+    if ((rc = dev_alloc(&E->d_nz, (size_t)Nmax * Nr * F, E->owned))) return rc;
     if ((rc = dev_alloc(&E->d_avg, Nr, E->owned))) return rc;
-    if ((rc = dev_alloc(&E->d_bins, (size_t)3 * d.nbins * BB_NPOLES, E->owned))) return rc;
+    if ((rc = dev_alloc(&E->d_bins, (size_t)3 * d.nbins * BB_NPOLES * F, E->owned))) return rc;
     if (!tasks.empty() && (rc = upload(E->d_tasks, tasks.data(), sizeof(FieldTask) * tasks.size(), E->stream))) return rc;
     if (!L.items.empty() && (rc = upload(E->d_items, L.items.data(), sizeof(WorkItem) * L.items.size(), E->stream))) return rc;
     if (!L.tws.empty() && (rc = upload(E->d_tws, L.tws.data(), sizeof(TWin) * L.tws.size(), E->stream))) return rc;
@@ -1076,30 +1096,87 @@ int estimator_init(bb_estimator *E)
     return BB_OK;
 }
 
-int estimate_device(bb_estimator *E, const double *d_field_in, double *d_bs, double *d_norm,
-                    double *d_biana)
+// Enqueues the estimates of nf fields (back to back from d_fields) on the
+// estimator's stream; outputs back to back per field.  Nothing waits.
+int estimate_enqueue(bb_estimator *E, int nf, const double *d_fields, double *d_bs,
+                     double *d_norm, double *d_biana)
 {
     const PlanData &d = E->plan->d;
     const Layout &L = *E->L;
+    if (nf < 1 || nf > E->batch)
+        return set_error(BB_E_RANGE, "batch of " + std::to_string(nf) + " fields; the estimator "
+                                         "holds " + std::to_string(E->batch));
     hipStream_t s = E->stream;
     HIPCHK(hipEventRecord(E->ev[0], s));
-    const int nz = d.g.Nmax * d.g.Nr;
+    const int nz = d.g.Nmax * d.g.Nr * nf;
     hipLaunchKernelGGL(k_fill_int, dim3((nz + 255) / 256), dim3(256), 0, s, E->d_nz, nz, 0);
     HIPCHK(hipGetLastError());
-    HIPCHK(launch_gather((const double2 *)d_field_in, E->d_planes, L.NN, L.NGtop, L.kmax,
-                         d.g.Nmax, s));
+    HIPCHK(launch_gather((const double2 *)d_fields, E->d_planes, L.NN, L.NGtop, L.kmax,
+                         d.g.Nmax, nf, s));
     HIPCHK(hipEventRecord(E->ev[1], s));
-    int rc = run_fields(E, E->d_planes, L.NGtop, E->d_nz);
+    int rc = run_fields(E, E->d_planes, L.NGtop, E->d_nz, nf);
     if (rc) return rc;
     HIPCHK(hipEventRecord(E->ev[2], s));
     HIPCHK(launch_contract(E->cprec, L.waves, L.tww, E->d_cache, E->d_split, E->d_items,
-                           E->d_units, (int)L.units.size(), E->d_tws, E->d_part, s));
+                           E->d_units, (int)L.units.size(), E->d_tws, E->d_part, E->bst, nf, s));
     HIPCHK(hipEventRecord(E->ev[3], s));
     HIPCHK(launch_binning(E->d_seg, E->d_recs, E->d_part, E->d_num, E->d_nz, E->d_avg, L.bp,
-                          d_bs, d_norm, d_biana, s));
+                          d_bs, d_norm, d_biana, E->bst, nf, s));
     HIPCHK(hipEventRecord(E->ev[4], s));
-    HIPCHK(hipStreamSynchronize(s));
+    E->pending = true;
+    return BB_OK;
+}
+
+// Waits for the enqueued batch and records its stage times.
+int estimate_finish(bb_estimator *E)
+{
+    if (!E->pending) return BB_OK;
+    E->pending = false;
+    HIPCHK(hipStreamSynchronize(E->stream));
     for (int k = 0; k < 4; ++k) HIPCHK(hipEventElapsedTime(&E->ms[k], E->ev[k], E->ev[k + 1]));
+    return BB_OK;
+}
+
+int estimate_device(bb_estimator *E, int nf, const double *d_fields, double *d_bs,
+                    double *d_norm, double *d_biana)
+{
+    int rc = estimate_enqueue(E, nf, d_fields, d_bs, d_norm, d_biana);
+    if (rc) {
+        (void)hipStreamSynchronize(E->stream);
+        E->pending = false;
+        return rc;
+    }
+    return estimate_finish(E);
+}
+
+// Host fields: staged through device memory, outputs copied back.
+int estimate_host(bb_estimator *E, int nf, const double *fields, double *bs, double *norm,
+                  double *biana)
+{
+    const PlanData &d = E->plan->d;
+    const size_t NN = d.g.NN;
+    const size_t fsz = NN * NN * (NN / 2 + 1);   // complex values per field
+    if (nf < 1 || nf > E->batch)
+        return set_error(BB_E_RANGE, "batch of " + std::to_string(nf) + " fields; the estimator "
+                                         "holds " + std::to_string(E->batch));
+    if (E->field_cap < nf) {
+        double2 *p = nullptr;
+        int rc = dev_alloc(&p, fsz * (size_t)nf, E->owned);
+        if (rc) return rc;
+        E->d_field = p;   // an older, smaller staging stays owned until destroy
+        E->field_cap = nf;
+    }
+    HIPCHK(hipMemcpyAsync(E->d_field, fields, fsz * 16 * (size_t)nf, hipMemcpyHostToDevice,
+                          E->stream));
+    const size_t nb = (size_t)d.nbins * BB_NPOLES;
+    double *B = E->d_bins;   // [bs | norm | biana], each nf * nb
+    const size_t span = nb * (size_t)E->batch;
+    int rc = estimate_device(E, nf, (const double *)E->d_field, B, B + span, B + 2 * span);
+    if (rc) return rc;
+    if (bs) HIPCHK(hipMemcpy(bs, B, nb * nf * sizeof(double), hipMemcpyDeviceToHost));
+    if (norm) HIPCHK(hipMemcpy(norm, B + span, nb * nf * sizeof(double), hipMemcpyDeviceToHost));
+    if (biana)
+        HIPCHK(hipMemcpy(biana, B + 2 * span, nb * nf * sizeof(double), hipMemcpyDeviceToHost));
     return BB_OK;
 }
 
@@ -1119,6 +1196,8 @@ int check_options(const bb_options &o)
         return set_error(BB_E_INVALID, "x_splits must be in [0, 1024]");
     if (o.grid_levels < 0 || o.grid_levels > BB_MAX_LEVELS)
         return set_error(BB_E_INVALID, "grid_levels must be in [0, BB_MAX_LEVELS]");
+    if (o.max_batch < 0 || o.max_batch > 1024)
+        return set_error(BB_E_INVALID, "max_batch must be in [0, 1024]");
     return BB_OK;
 }
 
@@ -1140,7 +1219,7 @@ int bb_estimator_footprint_opts(const bb_plan *p, const bb_options *opt, size_t 
         rc = set_error(BB_E_NOMEM, "out of host memory");
     }
     if (rc) return rc;
-    *bytes = layout_bytes(p->d, *L);
+    *bytes = layout_bytes(p->d, *L, (size_t)std::max(1, opt->max_batch));
     return BB_OK;
 }
 
@@ -1188,6 +1267,7 @@ int bb_estimator_create_opts(const bb_plan *p, const bb_options *opt, bb_estimat
     E->plan = p;
     E->opt = *opt;
     E->cprec = cp;
+    E->batch = std::max(1, opt->max_batch);
     try {
         rc = estimator_init(E);
     } catch (const std::bad_alloc &) {
@@ -1231,33 +1311,85 @@ int bb_estimate_device(bb_estimator *e, const double *d_field, double *d_bs, dou
                        double *d_biana)
 {
     if (!e || !d_field) return set_error(BB_E_INVALID, "null argument");
-    return estimate_device(e, d_field, d_bs, d_norm, d_biana);
+    int rc = estimate_finish(e);
+    if (rc) return rc;
+    return estimate_device(e, 1, d_field, d_bs, d_norm, d_biana);
 }
 
 int bb_estimate(bb_estimator *e, const double *field, double *bs, double *norm, double *biana)
 {
     if (!e || !field) return set_error(BB_E_INVALID, "null argument");
-    const PlanData &d = e->plan->d;
-    const size_t NN = d.g.NN;
-    const size_t bytes = NN * NN * (NN + 2) * sizeof(double);
-    if (!e->d_field) {
-        int rc = dev_alloc(&e->d_field, NN * NN * (NN / 2 + 1), e->owned);
-        if (rc) return rc;
-    }
-    HIPCHK(hipMemcpyAsync(e->d_field, field, bytes, hipMemcpyHostToDevice, e->stream));
-    const size_t nb = (size_t)d.nbins * BB_NPOLES;
-    double *B = e->d_bins;
-    int rc = estimate_device(e, (const double *)e->d_field, B, B + nb, B + 2 * nb);
+    int rc = estimate_finish(e);
     if (rc) return rc;
-    if (bs) HIPCHK(hipMemcpy(bs, B, nb * sizeof(double), hipMemcpyDeviceToHost));
-    if (norm) HIPCHK(hipMemcpy(norm, B + nb, nb * sizeof(double), hipMemcpyDeviceToHost));
-    if (biana) HIPCHK(hipMemcpy(biana, B + 2 * nb, nb * sizeof(double), hipMemcpyDeviceToHost));
+    return estimate_host(e, 1, field, bs, norm, biana);
+}
+
+int bb_estimate_batch(bb_estimator *e, int nfields, const double *fields, double *bs,
+                      double *norm, double *biana)
+{
+    if (!e || !fields) return set_error(BB_E_INVALID, "null argument");
+    int rc = estimate_finish(e);
+    if (rc) return rc;
+    return estimate_host(e, nfields, fields, bs, norm, biana);
+}
+
+int bb_estimate_batch_device(bb_estimator *e, int nfields, const double *d_fields, double *d_bs,
+                             double *d_norm, double *d_biana)
+{
+    if (!e || !d_fields) return set_error(BB_E_INVALID, "null argument");
+    int rc = estimate_finish(e);
+    if (rc) return rc;
+    return estimate_device(e, nfields, d_fields, d_bs, d_norm, d_biana);
+}
+
+int bb_estimate_batch_async(bb_estimator *e, int nfields, const double *d_fields, double *d_bs,
+                            double *d_norm, double *d_biana)
+{
+    if (!e || !d_fields) return set_error(BB_E_INVALID, "null argument");
+    int rc = estimate_finish(e);
+    if (rc) return rc;
+    rc = estimate_enqueue(e, nfields, d_fields, d_bs, d_norm, d_biana);
+    if (rc) {
+        (void)hipStreamSynchronize(e->stream);
+        e->pending = false;
+    }
+    return rc;
+}
+
+int bb_estimator_query(bb_estimator *e, int *done)
+{
+    if (!e || !done) return set_error(BB_E_INVALID, "null argument");
+    if (!e->pending) {
+        *done = 1;
+        return BB_OK;
+    }
+    hipError_t q = hipStreamQuery(e->stream);
+    if (q == hipErrorNotReady) {
+        *done = 0;
+        return BB_OK;
+    }
+    if (q != hipSuccess) return set_error(BB_E_DEVICE, hipGetErrorString(q));
+    *done = 1;
+    return estimate_finish(e);
+}
+
+int bb_estimator_wait(bb_estimator *e)
+{
+    if (!e) return set_error(BB_E_INVALID, "null argument");
+    return estimate_finish(e);
+}
+
+int bb_estimator_stream(const bb_estimator *e, void **stream)
+{
+    if (!e || !stream) return set_error(BB_E_INVALID, "null argument");
+    *stream = (void *)e->stream;
     return BB_OK;
 }
 
 int bb_estimator_contraction_sums(bb_estimator *e, double *S)
 {
     if (!e || !S) return set_error(BB_E_INVALID, "null argument");
+    if (int rc = estimate_finish(e)) return rc;
     const Layout &L = *e->L;
     std::vector<double> part((size_t)std::max<long long>(L.part_tiles, 1) * 256);
     HIPCHK(hipMemcpy(part.data(), e->d_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost));
@@ -1281,6 +1413,7 @@ int bb_estimator_triangle_counts(bb_estimator *e, double *num)
 int bb_estimator_field(bb_estimator *e, int level, int plane, int ring, double *out)
 {
     if (!e || !out) return set_error(BB_E_INVALID, "null argument");
+    if (int rc = estimate_finish(e)) return rc;
     const Layout &L = *e->L;
     if (level < 0 || level >= (int)L.lv.size()) return set_error(BB_E_INVALID, "no such level");
     const Level &V = L.lv[level];
@@ -1349,6 +1482,7 @@ int bb_plan_worklist(const bb_plan *p, const bb_options *opt, long long *o)
 int bb_estimator_timings(bb_estimator *e, double *ms4)
 {
     if (!e || !ms4) return set_error(BB_E_INVALID, "null argument");
+    if (int rc = estimate_finish(e)) return rc;
     for (int k = 0; k < 4; ++k) ms4[k] = e->ms[k];
     return BB_OK;
 }

@@ -135,19 +135,23 @@ __device__ double2 *lds_fft(double2 *a, double2 *b, int nb, const FftDesc &fd, c
     return a;
 }
 
-// planes[p][i][j] (NG x NG) = field[i mod NN][j mod NN][p] (complex) for
+// For each field f of a batch (blockIdx.z = f * NG + i; fields and plane sets
+// back to back), planes[p][i][j] (NG x NG) = field[i mod NN][j mod NN][p]
+// (complex) for
 // p < Nmax, where (i, j) on the NG grid stand for the signed modes
 // ia = i <= NG/2 ? i : i - NG (likewise ja).  Modes with |ia| or |ja| above
 // kmax lie outside every ring and are set to zero.  With NG = NN this is the
 // plain plane gather of Df (funcs.c:97-100).  32x32 transpose tiles through
 // LDS keep both the read (along p) and the write (along j) coalesced.
-__global__ __launch_bounds__(256) void k_gather(const double2 *__restrict__ field,
-                                                double2 *__restrict__ planes, int NN, int NG,
+__global__ __launch_bounds__(256) void k_gather(const double2 *__restrict__ fields,
+                                                double2 *__restrict__ planes_all, int NN, int NG,
                                                 int kmax, int Nmax)
 {
     __shared__ double2 tile[32][33];
     const int H = NN / 2 + 1;
-    const int i = blockIdx.z;
+    const int f = blockIdx.z / NG, i = blockIdx.z - f * NG;   // field of the batch, row
+    const double2 *field = fields + (long long)f * NN * NN * H;
+    double2 *planes = planes_all + (long long)f * Nmax * NG * NG;
     const int ia = (i > NG / 2) ? i - NG : i;
     const bool rowok = (ia <= kmax && -ia <= kmax) || NG == NN;
     const int is = ia < 0 ? ia + NN : ia;
@@ -171,11 +175,12 @@ __global__ __launch_bounds__(256) void k_gather(const double2 *__restrict__ fiel
     }
 }
 
-hipError_t launch_gather(const double2 *field, double2 *planes, int NN, int NG, int kmax,
-                         int Nmax, hipStream_t s)
+hipError_t launch_gather(const double2 *fields, double2 *planes, int NN, int NG, int kmax,
+                         int Nmax, int nf, hipStream_t s)
 {
-    dim3 grid((NG + 31) / 32, (Nmax + 31) / 32, NG);
-    hipLaunchKernelGGL(k_gather, grid, dim3(256), 0, s, field, planes, NN, NG, kmax, Nmax);
+    if ((long long)NG * nf > 65535) return hipErrorInvalidValue;
+    dim3 grid((NG + 31) / 32, (Nmax + 31) / 32, NG * nf);
+    hipLaunchKernelGGL(k_gather, grid, dim3(256), 0, s, fields, planes, NN, NG, kmax, Nmax);
     return hipGetLastError();
 }
 
@@ -206,10 +211,14 @@ __global__ __launch_bounds__(256) void k_rowpass(const double2 *__restrict__ pla
                                                  const int2 *__restrict__ blk,
                                                  const LevelDesc *__restrict__ lv,
                                                  const double2 *__restrict__ tw_all,
-                                                 int *__restrict__ nzflag)
+                                                 int *__restrict__ nzflag, BatchStrides bs)
 {
     extern __shared__ double2 lds[];
     const int2 B = blk[blockIdx.x];
+    const int fb = blockIdx.y;   // field of the batch
+    planes += fb * bs.planes;
+    cache += fb * bs.cache;
+    if (nzflag) nzflag += fb * bs.nz;
     const FieldTask T = tasks[B.x];
     const LevelDesc &L = lv[T.level];
     const int N = L.fd.N, rpb = L.rpb;
@@ -268,10 +277,14 @@ __global__ __launch_bounds__(256) void k_colpass(C2 *__restrict__ cache,
                                                  const FieldTask *__restrict__ tasks,
                                                  const int2 *__restrict__ blk,
                                                  const LevelDesc *__restrict__ lv,
-                                                 const double2 *__restrict__ tw_all)
+                                                 const double2 *__restrict__ tw_all,
+                                                 BatchStrides bs)
 {
     extern __shared__ double2 lds[];
     const int2 B = blk[blockIdx.x];
+    const int fb = blockIdx.y;   // field of the batch
+    cache += fb * bs.cache;
+    if (split) split += fb * bs.cache * 4;
     const FieldTask T = tasks[B.x];
     const LevelDesc &L = lv[T.level];
     const int N = L.fd.N, cpb = L.cpb;
@@ -321,8 +334,9 @@ template <typename C2>
 static hipError_t launch_fft(const double2 *planes, int NP, C2 *cache, __bf16 *split,
                              const FieldTask *tasks, const FftBlocks &fb, const LevelDesc *d_lv,
                              const LevelDesc *h_lv, int nlev, const double2 *tw, int *nzflag,
-                             hipStream_t s)
+                             const BatchStrides &bs, int nf, hipStream_t s)
 {
+    if (nf < 1 || nf > 65535) return hipErrorInvalidValue;
     size_t lds_r = 0, lds_c = 0;
     for (int l = 0; l < nlev; ++l) {
         const LevelDesc &L = h_lv[l];
@@ -338,16 +352,16 @@ static hipError_t launch_fft(const double2 *planes, int NP, C2 *cache, __bf16 *s
     if (fb.nrow > 0) {
         (void)hipFuncSetAttribute((const void *)k_rowpass<C2>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_r);
-        hipLaunchKernelGGL(k_rowpass<C2>, dim3(fb.nrow), dim3(256), lds_r, s, planes, NP, cache,
-                           tasks, fb.row, d_lv, tw, nzflag);
+        hipLaunchKernelGGL(k_rowpass<C2>, dim3(fb.nrow, nf), dim3(256), lds_r, s, planes, NP,
+                           cache, tasks, fb.row, d_lv, tw, nzflag, bs);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     if (fb.ncol > 0) {
         (void)hipFuncSetAttribute((const void *)k_colpass<C2>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_c);
-        hipLaunchKernelGGL(k_colpass<C2>, dim3(fb.ncol), dim3(256), lds_c, s, cache, split, tasks,
-                           fb.col, d_lv, tw);
+        hipLaunchKernelGGL(k_colpass<C2>, dim3(fb.ncol, nf), dim3(256), lds_c, s, cache, split,
+                           tasks, fb.col, d_lv, tw, bs);
     }
     return hipGetLastError();
 }
@@ -368,19 +382,20 @@ void fft_blocks(const FieldTask *tasks, int ntasks, const LevelDesc *h_lv, std::
 
 hipError_t launch_fft_f64(const double2 *planes, int NP, double2 *cache, const FieldTask *tasks,
                           const FftBlocks &fb, const LevelDesc *d_lv, const LevelDesc *h_lv,
-                          int nlev, const double2 *tw, int *nzflag, hipStream_t s)
+                          int nlev, const double2 *tw, int *nzflag, const BatchStrides &bs,
+                          int nf, hipStream_t s)
 {
     return launch_fft<double2>(planes, NP, cache, nullptr, tasks, fb, d_lv, h_lv, nlev, tw,
-                               nzflag, s);
+                               nzflag, bs, nf, s);
 }
 
 hipError_t launch_fft_f32(const double2 *planes, int NP, float2 *cache, void *split,
                           const FieldTask *tasks, const FftBlocks &fb, const LevelDesc *d_lv,
                           const LevelDesc *h_lv, int nlev, const double2 *tw, int *nzflag,
-                          hipStream_t s)
+                          const BatchStrides &bs, int nf, hipStream_t s)
 {
     return launch_fft<float2>(planes, NP, cache, (__bf16 *)split, tasks, fb, d_lv, h_lv, nlev, tw,
-                              nzflag, s);
+                              nzflag, bs, nf, s);
 }
 
 }  // namespace bb

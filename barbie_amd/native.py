@@ -29,9 +29,14 @@ EXPORTED = [
     "bb_plan_contractions", "bb_plan_grid_levels", "bb_plan_worklist", "bb_estimator_create", "bb_estimator_create_opts",
     "bb_estimator_destroy", "bb_estimator_grid", "bb_estimator_levels",
     "bb_estimator_footprint", "bb_estimator_footprint_opts", "bb_estimate", "bb_estimate_device",
+    "bb_estimate_batch", "bb_estimate_batch_device", "bb_estimate_batch_async",
+    "bb_estimator_query", "bb_estimator_wait", "bb_estimator_stream",
     "bb_estimator_contraction_sums", "bb_estimator_triangle_counts", "bb_estimator_field",
     "bb_estimator_timings", "bb_estimator_worklist", "bb_write_outputs",
+    "bb_server_create", "bb_server_destroy", "bb_submit", "bb_poll", "bb_wait", "bb_server_pause",
+    "bb_server_stats",
 ]
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 
 # bb_estimator_worklist / bb_plan_worklist entries, in order.
 WORKLIST_KEYS = ("items", "units", "tiles", "max_rings", "staged_rings", "staged_bytes", "grid",
@@ -61,7 +66,13 @@ class BBPlanStats(ctypes.Structure):
 
 class BBOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("full_grid", ctypes.c_int),
-                ("x_splits", ctypes.c_int), ("grid_levels", ctypes.c_int)]
+                ("x_splits", ctypes.c_int), ("grid_levels", ctypes.c_int),
+                ("max_batch", ctypes.c_int)]
+
+
+class BBServerOptions(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_inflight", ctypes.c_int),
+                ("max_batch", ctypes.c_int), ("precision", ctypes.c_int)]
 
 
 class BarbieError(RuntimeError):
@@ -102,6 +113,22 @@ def lib() -> ctypes.CDLL:
                                              ctypes.POINTER(ctypes.c_size_t)]
         L.bb_estimate.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
         L.bb_estimate_device.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 4
+        L.bb_estimate_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.bb_estimate_batch_device.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 4
+        L.bb_estimate_batch_async.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 4
+        L.bb_estimator_query.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+        L.bb_estimator_wait.argtypes = [ctypes.c_void_p]
+        L.bb_estimator_stream.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
+        L.bb_server_create.argtypes = [ctypes.POINTER(BBServerOptions),
+                                       ctypes.POINTER(ctypes.c_void_p)]
+        L.bb_server_destroy.argtypes = [ctypes.c_void_p]
+        L.bb_server_destroy.restype = None
+        L.bb_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(BBGeometry), ctypes.c_void_p,
+                                ctypes.c_int, ctypes.POINTER(ctypes.c_longlong)]
+        L.bb_poll.argtypes = [ctypes.c_void_p, ctypes.c_longlong, ctypes.POINTER(ctypes.c_int)]
+        L.bb_wait.argtypes = [ctypes.c_void_p, ctypes.c_longlong, _dp, _dp, _dp, _dp]
+        L.bb_server_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)]
+        L.bb_server_pause.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.bb_estimator_contraction_sums.argtypes = [ctypes.c_void_p, _dp]
         L.bb_estimator_triangle_counts.argtypes = [ctypes.c_void_p, _dp]
         L.bb_estimator_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
@@ -211,10 +238,11 @@ class Plan:
         return out
 
     def footprint(self, precision: str, full_grid: bool = False, x_splits: int = 0,
-                  grid_levels: int = 0) -> int:
+                  grid_levels: int = 0, max_batch: int = 0) -> int:
         """Device bytes an Estimator with these options holds."""
         b = ctypes.c_size_t()
-        o = BBOptions(PRECISIONS[precision], int(full_grid), int(x_splits), int(grid_levels))
+        o = BBOptions(PRECISIONS[precision], int(full_grid), int(x_splits), int(grid_levels),
+                      int(max_batch))
         _check(lib().bb_estimator_footprint_opts(self.handle, ctypes.byref(o), ctypes.byref(b)))
         return b.value
 
@@ -225,7 +253,7 @@ class Plan:
         n = ctypes.c_int()
         grids = (ctypes.c_int * 16)()
         cl = np.zeros(self.stats["ncontr"], dtype=np.int32)
-        o = BBOptions(PRECISIONS[precision], int(full_grid), 0, int(grid_levels))
+        o = BBOptions(PRECISIONS[precision], int(full_grid), 0, int(grid_levels), 0)
         _check(lib().bb_plan_grid_levels(self.handle, ctypes.byref(o), ctypes.byref(n), grids,
                                          cl.ctypes.data_as(_ip)))
         return [grids[i] for i in range(n.value)], cl
@@ -234,7 +262,7 @@ class Plan:
                  grid_levels: int = 0) -> dict:
         """Work-list figures of an Estimator with these options (host only)."""
         o = (ctypes.c_longlong * len(WORKLIST_KEYS))()
-        opt = BBOptions(PRECISIONS[precision], int(full_grid), int(x_splits), int(grid_levels))
+        opt = BBOptions(PRECISIONS[precision], int(full_grid), int(x_splits), int(grid_levels), 0)
         _check(lib().bb_plan_worklist(self.handle, ctypes.byref(opt), o))
         return dict(zip(WORKLIST_KEYS, list(o)))
 
@@ -248,11 +276,13 @@ class Estimator:
     """Device estimator (bb_estimator) for one plan and precision."""
 
     def __init__(self, plan: Plan, precision: str = "fp64", full_grid: bool = False,
-                 x_splits: int = 0, grid_levels: int = 0):
+                 x_splits: int = 0, grid_levels: int = 0, max_batch: int = 0):
         self.plan = plan
         self.precision = precision
+        self.max_batch = max(1, int(max_batch))
         h = ctypes.c_void_p()
-        o = BBOptions(PRECISIONS[precision], int(full_grid), int(x_splits), int(grid_levels))
+        o = BBOptions(PRECISIONS[precision], int(full_grid), int(x_splits), int(grid_levels),
+                      int(max_batch))
         _check(lib().bb_estimator_create_opts(plan.handle, ctypes.byref(o), ctypes.byref(h)))
         self.handle = h
         ng = ctypes.c_int()
@@ -289,6 +319,34 @@ class Estimator:
                                         ctypes.c_void_p(d_bs), ctypes.c_void_p(d_norm),
                                         ctypes.c_void_p(d_biana)))
 
+    def estimate_batch(self, fields_r2c: np.ndarray):
+        """fields_r2c: float64 (nf, NN, NN, NN+2), nf <= max_batch.  Returns
+        (bs, norm, biana), each (nf, n_k1bin, n_mtbin, n_mtbin, 16)."""
+        g = self.plan.geom
+        f = np.ascontiguousarray(fields_r2c, dtype=np.float64)
+        nf = f.shape[0]
+        assert f.size == nf * g.NN * g.NN * (g.NN + 2), "fields must be nf x NN x NN x (NN+2)"
+        shape = (nf, g.n_k1bin, g.n_mtbin, g.n_mtbin, NPOLES)
+        bs, nm, bi = (np.zeros(shape) for _ in range(3))
+        _check(lib().bb_estimate_batch(self.handle, nf, _p(f), _p(bs), _p(nm), _p(bi)))
+        return bs, nm, bi
+
+    def estimate_batch_device(self, nf: int, d_fields: int, d_bs: int, d_norm: int,
+                              d_biana: int, wait: bool = True):
+        """Device addresses; outputs back to back per field.  wait=False
+        enqueues only (poll with done(), block with wait())."""
+        fn = lib().bb_estimate_batch_device if wait else lib().bb_estimate_batch_async
+        _check(fn(self.handle, int(nf), ctypes.c_void_p(d_fields), ctypes.c_void_p(d_bs),
+                  ctypes.c_void_p(d_norm), ctypes.c_void_p(d_biana)))
+
+    def done(self) -> bool:
+        d = ctypes.c_int()
+        _check(lib().bb_estimator_query(self.handle, ctypes.byref(d)))
+        return bool(d.value)
+
+    def wait(self):
+        _check(lib().bb_estimator_wait(self.handle))
+
     def contraction_sums(self) -> np.ndarray:
         S = np.zeros(self.plan.stats["ncontr"])
         _check(lib().bb_estimator_contraction_sums(self.handle, _p(S)))
@@ -324,3 +382,69 @@ def normalise(geom: Geometry, bs, norm, biana):
     vol = (geom.NN * geom.LL) ** 3
     with np.errstate(divide="ignore", invalid="ignore"):
         return bs / (vol * norm), biana / norm
+
+
+SERVER_STATS_KEYS = ("submitted", "completed", "failed", "batches", "batched_fields",
+                     "lanes_created", "lanes_evicted", "peak_bytes", "queued", "running",
+                     "lanes", "bytes", "budget")
+
+
+class Server:
+    """Request server (bb_server): submit fields of any geometry, wait for
+    their bins.  Fields are numpy arrays (host, copied at submit) or device
+    addresses (int, e.g. a torch tensor's data_ptr(), kept valid until the
+    request is done)."""
+
+    def __init__(self, precision: str = "bf16x3", hbm_budget: int = 0, max_inflight: int = 0,
+                 max_batch: int = 0):
+        o = BBServerOptions(int(hbm_budget), int(max_inflight), int(max_batch),
+                            PRECISIONS[precision])
+        h = ctypes.c_void_p()
+        _check(lib().bb_server_create(ctypes.byref(o), ctypes.byref(h)))
+        self.handle = h
+        self._geoms = {}
+
+    def close(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.bb_server_destroy(h)
+        self.handle = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, geom: Geometry, field) -> int:
+        g = geom.to_c()
+        t = ctypes.c_longlong()
+        if isinstance(field, int):
+            _check(lib().bb_submit(self.handle, ctypes.byref(g), ctypes.c_void_p(field), 1,
+                                   ctypes.byref(t)))
+        else:
+            f = np.ascontiguousarray(field, dtype=np.float64)
+            assert f.size == geom.NN * geom.NN * (geom.NN + 2), "field must be NN x NN x (NN+2)"
+            _check(lib().bb_submit(self.handle, ctypes.byref(g), _p(f), 0, ctypes.byref(t)))
+        self._geoms[t.value] = geom
+        return t.value
+
+    def pause(self, paused: bool = True):
+        """Start no new batches while paused (a burst then batches together)."""
+        _check(lib().bb_server_pause(self.handle, int(paused)))
+
+    def poll(self, ticket: int) -> int:
+        st = ctypes.c_int()
+        _check(lib().bb_poll(self.handle, ticket, ctypes.byref(st)))
+        return st.value
+
+    def wait(self, ticket: int):
+        """Returns (bs, norm, biana, latency_ms[queued, running, total])."""
+        geom = self._geoms.pop(ticket)
+        shape = (geom.n_k1bin, geom.n_mtbin, geom.n_mtbin, NPOLES)
+        bs, nm, bi = (np.zeros(shape) for _ in range(3))
+        lat = np.zeros(3)
+        _check(lib().bb_wait(self.handle, ticket, _p(bs), _p(nm), _p(bi), _p(lat)))
+        return bs, nm, bi, lat
+
+    def stats(self) -> dict:
+        o = (ctypes.c_longlong * len(SERVER_STATS_KEYS))()
+        _check(lib().bb_server_stats(self.handle, o))
+        return dict(zip(SERVER_STATS_KEYS, list(o)))

@@ -70,6 +70,9 @@ typedef struct bb_options {
                        (0: automatic, 1: one grid).  The grids are chosen
                        among the even 2,3,5-smooth sizes to minimise the
                        summed points over all contractions. */
+    int max_batch;  /* fields one launch sequence processes together (0 or 1:
+                       one).  Device buffers per field are held max_batch
+                       times; the work lists are shared. */
 } bb_options;
 
 /* Geometry and binning of one estimate.  Replaces the constants hard-coded at
@@ -164,7 +167,27 @@ int bb_estimate(bb_estimator *e, const double *field_r2c,
 int bb_estimate_device(bb_estimator *e, const double *d_field_r2c,
                        double *d_bin_bs, double *d_bin_norm, double *d_bin_biana);
 
-/* Diagnostics of the last estimate: per-contraction triple-product sums in
+/* Batches (no counterpart in the reference, which reads one field per
+ * process): nfields <= max_batch fields back to back ([nfields][NN][NN][NN+2]
+ * doubles), outputs back to back per field ([nfields][n_k1bin][n_mtbin]
+ * [n_mtbin][16] each).  All fields of a batch share every launch. */
+int bb_estimate_batch(bb_estimator *e, int nfields, const double *fields_r2c,
+                      double *bin_bs, double *bin_norm, double *bin_biana);
+int bb_estimate_batch_device(bb_estimator *e, int nfields, const double *d_fields_r2c,
+                             double *d_bin_bs, double *d_bin_norm, double *d_bin_biana);
+/* Asynchronous form (device pointers): enqueues the batch on the estimator's
+ * stream and returns; bb_estimator_query sets *done without blocking,
+ * bb_estimator_wait blocks.  Device buffers must stay valid until done.  Any
+ * other call on the estimator waits for a pending batch first. */
+int bb_estimate_batch_async(bb_estimator *e, int nfields, const double *d_fields_r2c,
+                            double *d_bin_bs, double *d_bin_norm, double *d_bin_biana);
+int bb_estimator_query(bb_estimator *e, int *done);
+int bb_estimator_wait(bb_estimator *e);
+/* The estimator's HIP stream (hipStream_t), for callers that order their own
+ * copies against it. */
+int bb_estimator_stream(const bb_estimator *e, void **stream);
+
+/* Diagnostics of the last estimate (first field of the last batch): per-contraction triple-product sums in
  * the reference's enumeration order (ncontr doubles, scaled to the NN grid),
  * the GPU triangle counts Num per ring triple (rcount doubles, NN grid), and
  * one plane-ring field D_{plane,ring} on the grid of a level (NG*NG complex,
@@ -192,6 +215,45 @@ int bb_plan_worklist(const bb_plan *p, const bb_options *opt, long long *out);
 /* Per-stage device times of the last estimate in milliseconds:
  * [0] upload/gather, [1] plane-ring FFTs, [2] contraction, [3] binning. */
 int bb_estimator_timings(bb_estimator *e, double *ms4);
+
+/* Request server (no counterpart in the reference, which runs one field per
+ * process: barbie.c:58-397 started by run:1 / many_fields.sh).  One scheduler
+ * thread owns the current HIP device; fields of any geometry are submitted,
+ * batched with queued fields of the same geometry (up to max_batch) and run
+ * on up to max_inflight estimators at once, each on its own stream.  An
+ * estimator ("lane") is created only when its device bytes fit the HBM
+ * budget; idle lanes of other geometries are evicted least recently used
+ * first.  A field's bins are the same bits whatever else runs. */
+typedef struct bb_server bb_server;
+typedef struct bb_server_options {
+    size_t hbm_budget;       /* device bytes for lanes (0: 80% of the free memory) */
+    int max_inflight;        /* lanes running at once (0: 4; at most 16) */
+    int max_batch;           /* fields per lane launch (0: 4; at most 256) */
+    bb_precision precision;
+} bb_server_options;
+enum { BB_REQ_QUEUED = 0, BB_REQ_RUNNING = 1, BB_REQ_DONE = 2, BB_REQ_FAILED = 3 };
+int bb_server_create(const bb_server_options *opt, bb_server **out);
+/* Finishes running batches, drops queued requests, frees everything. */
+void bb_server_destroy(bb_server *s);
+/* field_r2c: [NN][NN][NN+2] doubles; host memory (copied before return) or,
+ * with on_device != 0, device memory that must stay valid until the request
+ * is done.  *ticket identifies the request. */
+int bb_submit(bb_server *s, const bb_geometry *g, const double *field_r2c, int on_device,
+              long long *ticket);
+/* paused != 0: start no new batches (running ones finish), so that a burst
+ * of submissions queues up and batches together; 0 resumes. */
+int bb_server_pause(bb_server *s, int paused);
+/* *state = BB_REQ_*; does not block. */
+int bb_poll(bb_server *s, long long ticket, int *state);
+/* Blocks until the request is done, copies its bins (any pointer may be
+ * NULL; [n_k1bin][n_mtbin][n_mtbin][16] each) and releases the ticket.
+ * latency_ms (may be NULL): [0] queued, [1] running, [2] total, in ms. */
+int bb_wait(bb_server *s, long long ticket, double *bin_bs, double *bin_norm,
+            double *bin_biana, double *latency_ms);
+/* out[13]: submitted, completed, failed, batches, fields in batches, lanes
+ * created, lanes evicted, peak lane bytes, queued, running lanes, lanes,
+ * lane bytes, budget. */
+int bb_server_stats(bb_server *s, long long *out);
 
 /* Normalisation and the 16 output files of barbie.c:383-397:
  * <dir>/BS_test_ii<ii>_N<NN>_R<NRi>.txt with the reference's row filter and

@@ -3,6 +3,14 @@ import sys
 
 import pytest
 
+# Load PyTorch's HIP runtime before libbarbie_amd: the library then binds to
+# the same libamdhip64.so.7 instead of bringing in a second runtime, and the
+# tests that hand torch device buffers to the library share one runtime.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))

@@ -1,0 +1,79 @@
+"""Request server on the GPU: a field's bins are the same bits alone and in a
+mixed, batched, concurrent workload; admission under a small HBM budget
+evicts idle lanes and refuses a request that cannot fit."""
+import numpy as np
+import pytest
+
+from geoms import small, medium, config1
+import barbie_amd as B
+
+pytestmark = pytest.mark.gpu
+
+
+def alone(gf, NRi, prec):
+    g = gf()
+    return B.Estimator(B.Plan(g), prec).estimate(B.make_field(g.NN, NRi=NRi))
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "fp64"])
+def test_results_same_alone_and_in_the_mix(prec):
+    cases = [(medium, 1), (config1, 2), (small, 3), (medium, 4), (config1, 5), (medium, 6),
+             (small, 7), (config1, 8), (medium, 9)]
+    ref = {c: alone(c[0], c[1], prec) for c in cases}
+    srv = B.Server(prec, max_inflight=3, max_batch=3)
+    # a request alone
+    t = srv.submit(medium(), B.make_field(medium().NN, NRi=1))
+    bs, nm, bi, lat = srv.wait(t)
+    for x, y in zip((bs, nm, bi), ref[(medium, 1)]):
+        assert np.array_equal(x, y)
+    assert lat[2] > 0
+    # the mix: interleaved geometries, batched and concurrent (queued as one
+    # burst, so that same-geometry requests share batches)
+    fields = {c: B.make_field(c[0]().NN, NRi=c[1]) for c in cases}
+    srv.pause(True)
+    tickets = [(c, srv.submit(c[0](), fields[c])) for c in cases * 2]
+    assert all(srv.poll(t) == B.native.REQ_QUEUED for _, t in tickets)
+    srv.pause(False)
+    for c, t in tickets:
+        bs, nm, bi, lat = srv.wait(t)
+        for x, y in zip((bs, nm, bi), ref[c]):
+            assert np.array_equal(x, y), c
+    st = srv.stats()
+    assert st["completed"] == 1 + 2 * len(cases) and st["failed"] == 0
+    assert st["batched_fields"] > st["batches"]   # some batches held several fields
+    srv.close()
+
+
+def test_device_resident_fields():
+    torch = pytest.importorskip("torch")
+    g = config1()
+    f = B.make_field(g.NN, NRi=3)
+    ref = alone(config1, 3, "bf16x3")
+    d = torch.from_numpy(f).to("cuda")
+    torch.cuda.synchronize()
+    srv = B.Server("bf16x3")
+    bs, nm, bi, _ = srv.wait(srv.submit(g, d.data_ptr()))
+    assert np.array_equal(bs, ref[0]) and np.array_equal(nm, ref[1])
+    srv.close()
+
+
+def test_admission_evicts_and_refuses():
+    gm, gc = medium(), config1()
+    need = max(B.Plan(gm).footprint("fp64", max_batch=1),
+               B.Plan(gc).footprint("fp64", max_batch=1))
+    field_bytes = max(g.NN * g.NN * (g.NN + 2) * 8 for g in (gm, gc))
+    budget = need + field_bytes + (1 << 20)    # room for one lane at a time
+    srv = B.Server("fp64", hbm_budget=budget, max_inflight=1, max_batch=1)
+    ts = [(gf, k, srv.submit(gf(), B.make_field(gf().NN, NRi=k)))
+          for k, gf in enumerate([medium, config1, medium, config1], start=1)]
+    for gf, k, t in ts:
+        bs, nm, bi, _ = srv.wait(t)
+        assert np.array_equal(bs, alone(gf, k, "fp64")[0])
+    st = srv.stats()
+    assert st["lanes_evicted"] >= 2 and st["peak_bytes"] <= budget
+    srv.close()
+    tiny = B.Server("fp64", hbm_budget=1 << 20)
+    t = tiny.submit(gm, B.make_field(gm.NN, NRi=1))
+    with pytest.raises(B.BarbieError):
+        tiny.wait(t)
+    tiny.close()
