@@ -60,3 +60,36 @@ def write_field(path: str, field: np.ndarray):
 
 def read_field(path: str, NN: int) -> np.ndarray:
     return np.fromfile(path, dtype=np.float64).reshape(NN, NN, NN + 2)
+
+
+def make_field_device(NN: int, NRi: int = 1, fnl: float = 0.3, amp: float = 1e3,
+                      index: float = -1.5, beta: float = 1.0, kcut_frac: float = 1.0 / 3.0,
+                      device: str = "cuda"):
+    """The construction of make_field on the GPU with PyTorch (white noise
+    from torch's generator seeded with seed_for(NRi), so the realization
+    differs from make_field's while the statistics are the same).  Returns a
+    float64 tensor (NN, NN, NN+2) on `device`.  A workload tool for sizes
+    where the host generator is slow; parity tests use make_field."""
+    import torch
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed_for(NRi))
+    white = torch.randn((NN, NN, NN), generator=gen, dtype=torch.float64, device=device)
+    W = torch.fft.rfftn(white)
+    del white
+    k = torch.fft.fftfreq(NN, d=1.0 / NN, dtype=torch.float64, device=device)
+    kz = torch.arange(NN // 2 + 1, dtype=torch.float64, device=device)
+    k2 = k[:, None, None] ** 2 + k[None, :, None] ** 2 + kz[None, None, :] ** 2
+    kk = torch.sqrt(k2)
+    pk = torch.where((kk > 0) & (kk < NN * kcut_frac), amp * kk.clamp_min(1e-300) ** index,
+                     torch.zeros_like(kk))
+    W *= torch.sqrt(pk)
+    del pk
+    dG = torch.fft.irfftn(W, s=(NN, NN, NN))
+    del W
+    var = torch.mean(dG * dG)
+    dG += fnl * (dG * dG - var)
+    D = torch.fft.rfftn(dG)
+    del dG
+    mu2 = torch.where(kk > 0, kz[None, None, :] ** 2 / k2.clamp_min(1e-300), torch.zeros_like(kk))
+    D *= (1.0 + beta * mu2)
+    return torch.view_as_real(D).reshape(NN, NN, NN + 2).contiguous()

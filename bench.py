@@ -1,17 +1,21 @@
 #!/usr/bin/env python3
 """Benchmark: NN=384 field estimates per hour on one MI355X.
 
-Workload: research-size estimate, NN=384 seeded non-Gaussian RSD field
-(r2c float64 384x384x386), Nmax=100 planes, Nr=40 piecewise rings,
+Workload (value): research-size estimate, NN=384 seeded non-Gaussian RSD
+field (r2c float64 384x384x386), Nmax=100 planes, Nr=40 piecewise rings,
 UL_k1=51, LL_k1=1, BW_k1=1, n_mtbin=5 (2 756 382 contractions per field).
 A step is one full estimate of one field already resident in HBM: plane
-gather, plane-ring IFFT cache, contraction, binning (bb_estimate_device).
+gather, plane-ring IFFT caches, contraction, binning (bb_estimate_device).
 
-Prints one JSON line.  `value` is the fields/hour of the fast path
-(--precision, default fp32); the FP64 GPU path is timed in the same run and
-reported beside it.  `roofline` is the contraction kernel; `cpu_baseline`
-is the CPU oracle (single thread) on a bounded sample of the same workload,
-extrapolated by contraction count.
+Prints one JSON line.  `value` is the fields/hour of the split path
+(--precision, default bf16x3); the FP64 GPU path is timed in the same run and
+reported beside it, with the per-multipole error of the split path against
+it at fNL-style coefficients 0.1, 0.3 and 1.0.  `roofline` is the
+contraction kernel; `cpu_baseline` is the CPU oracle on a bounded sample of
+the same workload.  Further legs: `batched` (64 realizations per NN at
+NN = 128, 192, 256, batched launches) and `serving` (request server, Poisson
+arrivals over an NN = 128/256/384/512 mix weighted 4:3:2:1, p50/p99 latency
+at 50/80/95% of the measured saturation rate).
 """
 import argparse
 import json
@@ -23,40 +27,46 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15 / 3}
-HBM_PEAK = 8.0e12
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", default="bf16x3", choices=["fp32", "bf16x3", "fp64"])
     ap.add_argument("--full-grid", action="store_true",
-                    help="run the transforms and sums on the NN grid instead of the compact grid")
+                    help="run the transforms and sums on the NN grid (the reference's grid)")
     ap.add_argument("--NN", type=int, default=384)
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
-                    help="target CPU time of the oracle sample (0 disables it)")
+                    help="target CPU time of the single-thread oracle sample (0 disables it)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the fNL 0.1 / 1.0 fields")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a reference point for "
                          "HBM byte counters)")
     return ap.parse_args()
 
 
-def geometry(NN):
-    import barbie_amd as B
+def geometry(B, NN):
+    """NN=384: the reference geometry.  Other sizes: Nmax = NN/4,
+    Nr = round(NN/10), UL = floor(NN/7), reference ring rule truncated at Nr;
+    NN=512: Nmax = 134, Nr = 54 uniform rings of width Nmax/Nr, UL = 73."""
     if NN == 384:
         return B.Geometry(NN=384, UL_k1=51, LL_k1=1, BW_k1=1, Nmax=100, Nr=40, n_mtbin=5)
-    # other sizes: Nmax = NN/4, Nr = round(NN/10), UL = NN/7 (batched-form rule)
+    if NN == 512:
+        return B.Geometry(NN=512, UL_k1=73, LL_k1=1, BW_k1=1, Nmax=134, Nr=54, n_mtbin=5,
+                          ring_rule=B.RINGS_UNIFORM, ring_width=134 / 54, last_ring_to_nmax=1)
     return B.Geometry(NN=NN, UL_k1=NN // 7, LL_k1=1, BW_k1=1, Nmax=NN // 4,
                       Nr=int(round(NN / 10)), n_mtbin=5)
 
 
 def time_path(B, torch, plan, precision, d_field, steps, warmup, full_grid=False):
     est = B.Estimator(plan, precision, full_grid=full_grid)
-    grid = est.levels
     wl = est.worklist()
+    levels = est.levels
     nb = plan.geom.nbins * 16
     out = torch.zeros(3 * nb, dtype=torch.float64, device="cuda")
     ptr = out.data_ptr()
@@ -75,12 +85,139 @@ def time_path(B, torch, plan, precision, d_field, steps, warmup, full_grid=False
     st = np.mean(np.array(stage), axis=0)
     bins = out.cpu().numpy()
     est.close()
-    return (t1 - t0) * 1e3 / steps, st, bins, grid, wl
+    return (t1 - t0) * 1e3 / steps, st, bins, levels, wl
+
+
+def per_multipole_err(bins, ref, nb):
+    import numpy as np
+    a = bins[:nb].reshape(-1, 16)
+    b = ref[:nb].reshape(-1, 16)
+    return [float(np.linalg.norm(a[:, i] - b[:, i]) / max(np.linalg.norm(b[:, i]), 1e-300))
+            for i in range(16)]
+
+
+def estimate_once(B, torch, plan, precision, d_field):
+    est = B.Estimator(plan, precision)
+    nb = plan.geom.nbins * 16
+    out = torch.zeros(3 * nb, dtype=torch.float64, device="cuda")
+    ptr = out.data_ptr()
+    est.estimate_device(d_field.data_ptr(), ptr, ptr + 8 * nb, ptr + 16 * nb)
+    r = out.cpu().numpy()
+    est.close()
+    return r
+
+
+def accuracy_leg(B, torch, plan, g, precision):
+    """Split path against the FP64 GPU path for fNL-style 0.1 and 1.0."""
+    out = {}
+    nb = g.nbins * 16
+    for fnl in (0.1, 1.0):
+        d = B.make_field_device(g.NN, NRi=1, fnl=fnl)
+        torch.cuda.synchronize()
+        a = estimate_once(B, torch, plan, precision, d)
+        b = estimate_once(B, torch, plan, "fp64", d)
+        out[str(fnl)] = max(per_multipole_err(a, b, nb))
+        del d
+    return out
+
+
+def batched_leg(B, torch, precision, nfields=64, distinct=8, reps=3):
+    """64 realizations per NN as batched launches; fields/hour batched and
+    one at a time on the same estimator."""
+    res = {}
+    for NN in (128, 192, 256):
+        g = geometry(B, NN)
+        plan = B.Plan(g)
+        nb = g.nbins * 16
+        fsz = NN * NN * (NN + 2)
+        base = torch.stack([B.make_field_device(NN, NRi=k + 1) for k in range(distinct)])
+        fields = base.repeat(nfields // distinct, 1, 1, 1).contiguous()
+        del base
+        est = B.Estimator(plan, precision, max_batch=nfields)
+        out = torch.zeros(3, nfields, nb, dtype=torch.float64, device="cuda")
+        ptrs = (fields.data_ptr(), out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr())
+        est.estimate_batch_device(nfields, *ptrs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            est.estimate_batch_device(nfields, *ptrs)
+        torch.cuda.synchronize()
+        tb = (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        for k in range(nfields):
+            est.estimate_batch_device(1, fields.data_ptr() + 8 * fsz * k, ptrs[1], ptrs[2],
+                                      ptrs[3])
+        torch.cuda.synchronize()
+        t1 = time.perf_counter() - t0
+        res[str(NN)] = {"fields_per_hour_batched": nfields * 3600.0 / tb,
+                        "fields_per_hour_one_at_a_time": nfields * 3600.0 / t1,
+                        "batch": nfields, "ms_per_batch": tb * 1e3,
+                        "ncontr": plan.stats["ncontr"],
+                        "realizations": f"{distinct} seeded fields x {nfields // distinct} "
+                                        "copies (cost does not depend on the values)"}
+        est.close()
+        del fields, out
+        torch.cuda.empty_cache()
+    return res
+
+
+def serving_leg(B, torch, precision, n_sat=200, n_load=300, seed=7):
+    """Request server over the NN mix 128/256/384/512 (weights 4:3:2:1), one
+    device-resident field per NN.  Saturation: a burst of n_sat requests.
+    Then Poisson arrivals at 50/80/95% of that rate, n_load requests each."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sizes, weights = [128, 256, 384, 512], np.array([4, 3, 2, 1], dtype=float)
+    geoms = {NN: geometry(B, NN) for NN in sizes}
+    fields = {NN: B.make_field_device(NN, NRi=1) for NN in sizes}
+    torch.cuda.synchronize()
+    srv = B.Server(precision, max_inflight=4, max_batch=4)
+    for NN in sizes:   # lanes and plans built before timing
+        srv.wait(srv.submit(geoms[NN], fields[NN].data_ptr()))
+
+    def draw(n):
+        return rng.choice(sizes, size=n, p=weights / weights.sum())
+
+    mix = draw(n_sat)
+    srv.pause(True)
+    tickets = [srv.submit(geoms[NN], fields[NN].data_ptr()) for NN in mix]
+    t0 = time.perf_counter()
+    srv.pause(False)
+    for t in tickets:
+        srv.wait(t)
+    sat = n_sat / (time.perf_counter() - t0)
+    out = {"saturation_req_per_s": sat, "mix": "NN 128/256/384/512 weighted 4:3:2:1",
+           "max_inflight": 4, "max_batch": 4, "loads": {}}
+    for load in (0.5, 0.8, 0.95):
+        lam = load * sat
+        mix = draw(n_load)
+        gaps = rng.exponential(1.0 / lam, size=n_load)
+        tickets = []
+        t_next = time.perf_counter()
+        for NN, gap in zip(mix, gaps):
+            t_next += gap
+            while True:
+                dt = t_next - time.perf_counter()
+                if dt <= 0:
+                    break
+                if dt > 0.0005:
+                    time.sleep(dt - 0.0004)
+            tickets.append(srv.submit(geoms[NN], fields[NN].data_ptr()))
+        lat = np.array([srv.wait(t)[3][2] for t in tickets])
+        out["loads"][str(load)] = {"rate_req_per_s": lam, "p50_ms": float(np.percentile(lat, 50)),
+                                   "p99_ms": float(np.percentile(lat, 99)),
+                                   "mean_ms": float(lat.mean())}
+    st = srv.stats()
+    out["batches"] = st["batches"]
+    out["batched_fields"] = st["batched_fields"]
+    srv.close()
+    return out
 
 
 def cpu_baseline(B, plan, field, target_s):
-    """CPU oracle, 1 thread, on the p1 prefix whose contraction count makes
-    about target_s of work; extrapolated to the whole field."""
+    """CPU oracle on the p1 prefix whose contraction count makes about
+    target_s of single-thread work, extrapolated by contraction count; the
+    same sample again on all of the host share's cores (16 threads)."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
@@ -88,8 +225,7 @@ def cpu_baseline(B, plan, field, target_s):
     pc = plan.contractions()
     counts = np.bincount(pc[:, 0], minlength=g.Nmax)
     ncontr = len(pc)
-    # ~0.45 ms per contraction at NN=384 on one core: pick the p1 prefix
-    per = 3.0e-9 * g.NN * g.NN
+    per = 3.0e-9 * g.NN * g.NN   # ~0.45 ms per contraction at NN=384 on one core
     want = max(1, int(target_s / per))
     k = int(np.searchsorted(np.cumsum(counts), want)) + 1
     k = max(1, min(k, g.Nmax))
@@ -98,13 +234,27 @@ def cpu_baseline(B, plan, field, target_s):
     params = {kk: getattr(g, kk) for kk in ("NN", "Nmax", "Nr", "n_mtbin", "UL_k1", "LL_k1",
                                             "BW_k1", "LL", "ff", "beta")}
     t0 = time.perf_counter()
-    _, _, _, n, _ = O.estimate(params, lo, hi, av, rd, field, p1_begin=0, p1_end=k)
+    _, _, _, n, _ = O.estimate(params, lo, hi, av, rd, field, p1_begin=0, p1_end=k, nthreads=1)
     dt = time.perf_counter() - t0
     est_s = dt * ncontr / n
+    # All cores: the oracle's threads split the p1 range, so the sample is
+    # the last 16 p1 planes that hold contractions.
+    threads = 16
+    nzp = np.nonzero(counts)[0]
+    p1e = int(nzp.max()) + 1
+    p1b = max(0, p1e - threads)
+    t0 = time.perf_counter()
+    _, _, _, nm, _ = O.estimate(params, lo, hi, av, rd, field, p1_begin=p1b, p1_end=p1e,
+                                nthreads=threads)
+    dtm = time.perf_counter() - t0
     return {"value": 3600.0 / est_s, "unit": "fields/hour", "cores": 1, "kind": "port",
             "sample": f"CPU oracle (FP64, own FFT, 1 thread) over p1 in [0,{k}): {n} of {ncontr} "
                       f"contractions in {dt:.1f} s, extrapolated by contraction count to "
-                      f"{est_s:.0f} s/field"}
+                      f"{est_s:.0f} s/field",
+            "all_cores": {"value": 3600.0 / (dtm * ncontr / nm), "cores": threads,
+                          "sample": f"p1 in [{p1b},{p1e}) with {threads} OpenMP threads (one "
+                                    f"p1 plane each): {nm} contractions in {dtm:.1f} s, "
+                                    "extrapolated by contraction count"}}
 
 
 def main():
@@ -118,7 +268,7 @@ def main():
     import barbie_amd as B
 
     torch.cuda.set_device(0)
-    g = geometry(a.NN)
+    g = geometry(B, a.NN)
     plan = B.Plan(g)
     s = plan.stats
     field = B.make_field(g.NN, NRi=1, fnl=0.3)
@@ -130,21 +280,28 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    ms, st, bins, grid, wl = time_path(B, torch, plan, a.precision, d_field, a.steps, a.warmup,
-                                       a.full_grid)
+    ms, st, bins, levels, wl = time_path(B, torch, plan, a.precision, d_field, a.steps,
+                                         a.warmup, a.full_grid)
     res = {"fast": (ms, st, bins)}
-    if not a.skip_fp64:
+    if not a.skip_fp64 and a.precision != "fp64":
         r64 = time_path(B, torch, plan, "fp64", d_field, a.steps, a.warmup, a.full_grid)
         res["fp64"] = r64[:3]
         wl64 = r64[4]
 
-    # algorithmic contraction flops: Re(P C) = 2 FMAs per point per
-    # contraction, summed over the points of each contraction's grid level
-    w_contr = 4.0 * wl["points_summed"]
+    # Algorithmic contraction work per field, as the reference computes it:
+    # Re(P C) is two real FMAs per point of the NN grid per contraction.
+    w_ref = 4.0 * g.NN * g.NN * s["ncontr"]
+    # Work this build performs: the same over each contraction's own grid.
+    w_done = 4.0 * wl["points_summed"]
     t_contr = st[2] * 1e-3
-    achieved = w_contr / t_contr
     peak = PEAK[a.precision]
     value = 3600e3 / ms
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", f"contraction_{a.precision}_nn{g.NN}.txt")
+    if os.path.exists(prof):
+        for line in open(prof):
+            if line.startswith("traffic_bytes_per_launch:"):
+                traffic = float(line.split(":")[1])
     out = {
         "metric": "NN=384 field estimates/hour, split-FP32 vs FP64 GPU; p50/p99 request latency",
         "value": value,
@@ -161,33 +318,41 @@ def main():
         "config": {"workload": f"estimate NN={g.NN} Nmax={g.Nmax} Nr={g.Nr} UL_k1={g.UL_k1} "
                                f"LL_k1={g.LL_k1} BW_k1={g.BW_k1} n_mtbin={g.n_mtbin}",
                    "ncontr": s["ncontr"], "fields_used": s["nfields_used"],
-                   "precision": a.precision, "grid_levels": grid},
+                   "precision": a.precision, "grid_levels": levels},
         "stages_ms": {"gather": st[0], "fft_cache": st[1], "contraction": st[2],
                       "binning": st[3]},
         "worklist": wl,
         "roofline": {"bound": "mfma", "kernel": "contraction",
-                     "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
-                     "work": f"4 * sum_c NG(c)^2 = {w_contr:.4g} flop per launch (grid levels "
-                             f"{grid}); bf16x3 peak taken as 2.5 PF/3"},
+                     "achieved": w_ref / t_contr / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                     "frac": w_ref / t_contr / peak, "traffic": traffic,
+                     "work": f"W_contr = 4 * NN^2 * ncontr = {w_ref:.4g} flop per launch "
+                             f"(reference grid); performed: 4 * sum_c NG(c)^2 = {w_done:.4g} "
+                             f"(grid levels {levels}), {w_done / t_contr / 1e12:.2f} TFLOP/s; "
+                             "bf16x3 peak taken as 2.5 PF/3"},
     }
     if "fp64" in res:
         ms64, st64, bins64 = res["fp64"]
+        nb = g.nbins * 16
         out["fp64"] = {"fields_per_hour": 3600e3 / ms64, "ms_per_step": ms64,
                        "stages_ms": {"gather": st64[0], "fft_cache": st64[1],
                                      "contraction": st64[2], "binning": st64[3]},
-                       "contraction_TFLOPs": w_contr / (st64[2] * 1e-3) / 1e12,
-                       "contraction_frac_fp64_peak": w_contr / (st64[2] * 1e-3) / PEAK["fp64"],
+                       "contraction_TFLOPs_reference_work": w_ref / (st64[2] * 1e-3) / 1e12,
                        "worklist": wl64}
         out["speedup_vs_fp64"] = ms64 / ms
-        nb = g.nbins * 16
-        err = {}
-        for name, k in (("bs", 0),):
-            a_ = bins[k * nb:(k + 1) * nb].reshape(-1, 16)
-            b_ = bins64[k * nb:(k + 1) * nb].reshape(-1, 16)
-            err[name] = [float(np.linalg.norm(a_[:, ii] - b_[:, ii]) /
-                               max(np.linalg.norm(b_[:, ii]), 1e-300)) for ii in range(16)]
-        out["rel_err_vs_fp64_per_multipole"] = err["bs"]
+        out["rel_err_vs_fp64_per_multipole"] = per_multipole_err(bins, bins64, nb)
+        if not a.no_accuracy:
+            acc = accuracy_leg(B, torch, plan, g, a.precision)
+            acc["0.3"] = max(out["rel_err_vs_fp64_per_multipole"])
+            out["max_rel_err_vs_fp64_by_fnl"] = acc
+    del d_field
+    torch.cuda.empty_cache()
+    if not a.no_batched:
+        out["batched"] = batched_leg(B, torch, a.precision)
+    if not a.no_serving:
+        sv = serving_leg(B, torch, a.precision)
+        out["serving"] = sv
+        out["p50_ms"] = sv["loads"]["0.8"]["p50_ms"]
+        out["p99_ms"] = sv["loads"]["0.8"]["p99_ms"]
     if a.cpu_seconds > 0:
         out["cpu_baseline"] = cpu_baseline(B, plan, field, a.cpu_seconds)
     print(json.dumps(out), flush=True)
