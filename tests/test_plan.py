@@ -125,3 +125,36 @@ def test_write_outputs_format_matches_oracle(tmp_path):
     for ii in range(16):
         name = f"BS_test_ii{ii}_N16_R5.txt"
         assert (tmp_path / "a" / name).read_text() == (tmp_path / "b" / name).read_text()
+
+
+@pytest.mark.parametrize("gf", [medium, config1])
+def test_bins_follow_the_law_of_cosines(gf):
+    """Shape bins (barbie.c:287-298, 343-345) against an independent form:
+    with K sorted descending (stable on ties, funcs.c:113-126), T = K1/K0 and
+    MU is the cosine of the triangle's angle between its two longest sides,
+    (K0^2 + K1^2 - K2^2) / (2 K0 K1), and K1_bin = floor((K0 - LL_k1)/BW_k1).
+    Contractions whose MU or T lies within 1e-9 of a bin edge are skipped."""
+    g = gf()
+    plan = B.Plan(g)
+    _, _, av = plan.ring_edges()
+    c = plan.contractions().astype(np.int64)
+    p = c[:, [0, 2, 4]].astype(float)
+    r = c[:, [1, 3, 5]]
+    K = np.sqrt(av[r] ** 2 + p ** 2)
+    order = np.argsort(-K, axis=1, kind="stable")
+    Ks = np.take_along_axis(K, order, axis=1)
+    T = Ks[:, 1] / Ks[:, 0]
+    MU = (Ks[:, 0] ** 2 + Ks[:, 1] ** 2 - Ks[:, 2] ** 2) / (2 * Ks[:, 0] * Ks[:, 1])
+    n = g.n_mtbin
+    kb = np.floor((Ks[:, 0] - g.LL_k1) / g.BW_k1).astype(int)
+    tb = np.where(T == 1.0, n - 1, np.floor((T - 0.5) / (0.5 / n))).astype(int)
+    mb = np.where(MU == 1.0, n - 1, np.floor((MU - 0.5) / (0.5 / n))).astype(int)
+    flat = (kb * n + tb) * n + mb
+
+    def near_edge(x):
+        y = (x - 0.5) / (0.5 / n)
+        return np.abs(y - np.round(y)) < 1e-9
+    ok = ~(near_edge(T) & (T != 1.0)) & ~(near_edge(MU) & (MU != 1.0))
+    assert ok.mean() > 0.9
+    assert np.array_equal(flat[ok], c[ok, 7])
+    assert np.all((MU >= 0.5 - 1e-12) & (MU <= 1.0 + 1e-12))
