@@ -90,8 +90,24 @@ __global__ __launch_bounds__(kBinThreads) void k_binning(const int32_t *__restri
 #pragma unroll
     for (int k = 0; k < 16; ++k) abs_[k] = anm[k] = abi[k] = 0.0;
 
-    for (int c = seg[b] + tid; c < seg[b + 1]; c += kBinThreads) {
-        const BinRec R = recs[c];
+    // Software-pipelined over the thread's records: the record and partial
+    // sums of the next one are loaded before the current one is binned.
+    const int end = seg[b + 1];
+    int c = seg[b] + tid;
+    BinRec Rn{};
+    double Sn = 0.0;
+    if (c < end) {
+        Rn = recs[c];
+        for (int sp = 0; sp < Rn.nsp; ++sp) Sn += S[(long long)Rn.pos + 256 * sp];
+    }
+    for (; c < end; c += kBinThreads) {
+        const BinRec R = Rn;
+        const double Sv = Sn;   // sum of the x-split partials, fixed order
+        if (c + kBinThreads < end) {
+            Rn = recs[c + kBinThreads];
+            Sn = 0.0;
+            for (int sp = 0; sp < Rn.nsp; ++sp) Sn += S[(long long)Rn.pos + 256 * sp];
+        }
         const double a1 = avg[R.r1], a2 = avg[R.r2], a3 = avg[R.r3];
         const double p1 = R.p1, p2 = R.p2, p3 = R.p3;
         double K[3][2] = {{sqrt(a1 * a1 + p1 * p1), p1},
@@ -117,8 +133,6 @@ __global__ __launch_bounds__(kBinThreads) void k_binning(const int32_t *__restri
         const double P0 = 1.0 / (k0 * k0), P1 = 1.0 / (k1 * k1), P2 = 1.0 / (k2 * k2);
         double Biana = 2. * bp.ff * (P0 * P1 + P1 * P2 + P2 * P0);
         Biana = Biana * (1 + bp.beta * m1 * m1) * (1 + bp.beta * m2 * m2) * (1 + bp.beta * m3 * m3);
-        double Sv = 0.0;  // sum of the x-split partials, fixed order
-        for (int sp = 0; sp < R.nsp; ++sp) Sv += S[(long long)R.pos + 256 * sp];
         // barbie.c:355-357.  The sum is exactly zero iff one of its three
         // fields is identically zero; the fields' zero flags decide it.
         if (!nzflag[R.f1] || !nzflag[R.f2] || !nzflag[R.f3]) {

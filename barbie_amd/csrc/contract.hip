@@ -72,32 +72,38 @@ __device__ __forceinline__ void row_tables(const WorkItem &it, int *row_slot, in
 // Staging of one chunk: the item's rings, each kRowBytes contiguous bytes of
 // its field (raw values, or the bf16 split operands for split rows), moved as
 // 8-byte units.  Branch-free: every thread moves kPF units; units past the
-// item's rings read ring 0 and land in a spare LDS ring (row kRowCap), and
-// bytes past the end of the field read as zero.
+// item's rings read ring 0's place again and land in a spare LDS ring (row
+// kRowCap).  Field strides are multiples of a chunk, so a chunk never runs
+// past its field.  Each unit's source, as an offset in 8-byte words from the
+// item's cache base with the split flag in bit 31, is computed once per
+// workgroup (init); a chunk then costs a select and a shift-add per unit.
 template <int NT>
 struct Stager {
     const char *cache;
     const char *split;
-    long long field_bytes;   // bytes between fields (the padding past a field's points is zero)
     int nunits;              // 8-byte units this chunk: nrows * kRowBytes / 8
-    int nrows;
+    unsigned uo[kPF];
     uint2 pf[kPF];
 
-    __device__ __forceinline__ void load(const int *row_slot, const int *row_split,
-                                         long long chunk_off)
+    // field_words: 8-byte words between fields (fits 31 bits with the slots;
+    // checked on the host).
+    __device__ __forceinline__ void init(const int *row_slot, const int *row_split, int nrows,
+                                         long long field_words)
     {
 #pragma unroll
         for (int e = 0; e < kPF; ++e) {
             const int u = threadIdx.x + e * NT;
             const int r = min(u >> 5, nrows - 1), k = u & 31;   // 32 units per ring
-            const long long off = chunk_off + k * 8;
-            const long long offc = off < field_bytes ? off : 0;
-            const char *base = row_split[r] ? split : cache;
-            uint2 v = *reinterpret_cast<const uint2 *>(base + (long long)row_slot[r] * field_bytes + offc);
-            const bool keep = off < field_bytes;
-            v.x = keep ? v.x : 0u;
-            v.y = keep ? v.y : 0u;
-            pf[e] = v;
+            uo[e] = (unsigned)(row_slot[r] * field_words + k) | (row_split[r] ? 0x80000000u : 0u);
+        }
+    }
+    __device__ __forceinline__ void load(long long chunk_off)
+    {
+        const char *bc = cache + chunk_off, *bs = split + chunk_off;
+#pragma unroll
+        for (int e = 0; e < kPF; ++e) {
+            const char *b = (uo[e] & 0x80000000u) ? bs : bc;
+            pf[e] = *reinterpret_cast<const uint2 *>(b + (size_t)(uo[e] & 0x7fffffffu) * 8);
         }
     }
     __device__ __forceinline__ void store(unsigned char *rows)
@@ -201,18 +207,17 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_contract_k4(
     Stager<64 * NW> S;
     S.cache = reinterpret_cast<const char *>(cache + it.off);
     S.split = S.cache;
-    S.field_bytes = it.fstride * (long long)sizeof(C2);
     S.nunits = nrows * (kRowBytes / 8);
-    S.nrows = nrows;
     __syncthreads();
+    S.init(row_slot, row_split, nrows, it.fstride * (long long)sizeof(C2) / 8);
 
-    if (cb < ce) S.load(row_slot, row_split, cb * kRowBytes);
+    if (cb < ce) S.load(cb * kRowBytes);
     int since = 0;
     for (long long c = cb; c < ce; ++c) {
         __syncthreads();
         S.store(rows);
         __syncthreads();
-        if (c + 1 < ce) S.load(row_slot, row_split, (c + 1) * kRowBytes);
+        if (c + 1 < ce) S.load((c + 1) * kRowBytes);
 #pragma unroll
         for (int s = 0; s < CH / 4; ++s) {   // 4 points per MFMA pair
             const int so = s * 4 * (int)sizeof(C2);
@@ -368,18 +373,17 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_contract_bf16x3(
     Stager<64 * NW> S;
     S.cache = reinterpret_cast<const char *>(cache + it.off);
     S.split = reinterpret_cast<const char *>(splitc + it.off * 4);
-    S.field_bytes = it.fstride * 8;
     S.nunits = nrows * (kRowBytes / 8);
-    S.nrows = nrows;
     __syncthreads();
+    S.init(row_slot, row_split, nrows, it.fstride);
 
-    if (cb < ce) S.load(row_slot, row_split, cb * kRowBytes);
+    if (cb < ce) S.load(cb * kRowBytes);
     int since = 0;
     for (long long c = cb; c < ce; ++c) {
         __syncthreads();
         S.store(rows);
         __syncthreads();
-        if (c + 1 < ce) S.load(row_slot, row_split, (c + 1) * kRowBytes);
+        if (c + 1 < ce) S.load((c + 1) * kRowBytes);
         bf16x8 arh = {}, arl = {}, aih = {}, ail = {};
 #pragma unroll
         for (int j = 0; j < TWW; ++j) {
@@ -439,13 +443,8 @@ static hipError_t launch_nw(CPrec p, int tww, const void *cache, const void *spl
 {
     dim3 grid((unsigned)nunits, (unsigned)nf);
     const size_t lds = kTableBytes + (size_t)(kRowCap + 1) * kPitch;   // + spare ring
-    if (p == CPrec::F64 && tww <= kTWWF64Narrow) {
-        auto k = k_contract_k4<double, NW, kTWWF64Narrow, 16 / NW>;
-        (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds);
-        hipLaunchKernelGGL(k, grid, dim3(64 * NW), lds, s, (const double2 *)cache, items, units,
-                           tws, out, bs);
-    } else if (p == CPrec::F64) {
+    (void)tww;
+    if (p == CPrec::F64) {
         auto k = k_contract_k4<double, NW, kTWW, 8 / NW>;
         (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds);

@@ -79,7 +79,6 @@ constexpr int kWaves = 8;       // most waves of a contraction workgroup (4 or 8
 constexpr int kTWW = 12;        // tile windows per wave (slots reserved per wave)
 constexpr int kTWW32 = 8;       // tile windows per wave, BF16x3 form
 constexpr int kTWWF32 = 6;      // tile windows per wave, FP32 form
-constexpr int kTWWF64Narrow = 6;  // FP64 form sized for two workgroups per CU
 constexpr int kMaxBlk = 48;     // ring blocks per item
 constexpr int kRowCap = 240;    // staged rings per item (8 waves; 30 per wave)
 inline int rowcap_for(int waves) { return 30 * waves; }

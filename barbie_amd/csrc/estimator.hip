@@ -434,6 +434,11 @@ int check_lists(const Layout &L, size_t ncontr)
             if (it.blk_slot[b] < 0 || it.fstride < it.npts || it.fstride % 32 != 0 ||
                 it.off + (long long)(it.blk_slot[b] + it.blk_rows[b]) * it.fstride > L.cache_pts)
                 return set_error(BB_E_INVALID, "internal: staged ring outside the cache");
+            // the contraction addresses a ring as a 31-bit count of 8-byte words
+            const long long words = (long long)(it.blk_slot[b] + it.blk_rows[b]) * it.fstride *
+                                    (L.prec == CPrec::F64 ? 2 : 1);
+            if (words >= (1LL << 31))
+                return set_error(BB_E_RANGE, "a grid level's cache exceeds 16 GiB");
         }
         if (rows != it.nrows || it.nrows > kRowCap || it.nblk > kMaxBlk || item_sp[i] < 1)
             return set_error(BB_E_INVALID, "internal: work item row blocks are inconsistent");
@@ -578,15 +583,7 @@ int layout_waves()
     return kWaves;
 }
 
-int layout_tww(CPrec p)
-{
-    int t = contract_tww(p);
-    if (const char *ev = std::getenv("BARBIE_CONTRACT_TWW")) {
-        const int v = std::atoi(ev);
-        if (p == CPrec::F64 && (v == kTWW || v == kTWWF64Narrow)) t = v;
-    }
-    return t;
-}
+int layout_tww(CPrec p) { return contract_tww(p); }
 
 int build_layout(const PlanData &d, CPrec prec, const bb_options &opt, int waves, int tww,
                  int max_levels, Layout &L)
@@ -710,6 +707,12 @@ int build_layout(const PlanData &d, CPrec prec, const bb_options &opt, int waves
             L.recs[at[k.bin]++] = R;
         }
     }
+    // Within a bin, records in partial-sum order: neighbouring threads then
+    // read neighbouring partial sums.  (The bin's summation order is fixed by
+    // this order.)
+    for (int b = 0; b < d.nbins; ++b)
+        std::sort(L.recs.begin() + L.seg[b], L.recs.begin() + L.seg[b + 1],
+                  [](const BinRec &x, const BinRec &y) { return x.pos < y.pos; });
     L.bp.nbins = d.nbins;
     L.bp.n_mtbin = d.g.n_mtbin;
     L.bp.NN = NN;

@@ -184,6 +184,11 @@ hipError_t launch_gather(const double2 *fields, double2 *planes, int NN, int NG,
     return hipGetLastError();
 }
 
+// Threads per block of both passes, and the most points a thread loads
+// (rpb * N and cpb * N stay within kPassThreads * kPassPer).
+constexpr int kPassThreads = 256;
+constexpr int kPassPer = 10;
+
 // Precision of the cache the passes write: FP64, or FP32 (the column pass
 // still transforms in FP64; the row-pass output is rounded to FP32 once).
 template <typename C2>
@@ -205,7 +210,7 @@ __device__ __forceinline__ double2 load_as(float2 v) { return make_double2(v.x, 
 // NP x NP plane grid (NP >= N): a mode (ia, ja) inside the ring satisfies
 // |ia|, |ja| < N/2 and sits at (ia mod NP, ja mod NP) there.
 template <typename C2>
-__global__ __launch_bounds__(256) void k_rowpass(const double2 *__restrict__ planes, int NP,
+__global__ __launch_bounds__(kPassThreads) void k_rowpass(const double2 *__restrict__ planes, int NP,
                                                  C2 *__restrict__ cache,
                                                  const FieldTask *__restrict__ tasks,
                                                  const int2 *__restrict__ blk,
@@ -229,26 +234,39 @@ __global__ __launch_bounds__(256) void k_rowpass(const double2 *__restrict__ pla
     double2 *twl = lds, *a = lds + N, *b = a + rpb * N;
     for (int t = threadIdx.x; t < N; t += blockDim.x) twl[t] = tw_all[L.tw_off + t];
     const unsigned long long mN = L.fd.mN;
+    // All of this thread's loads are issued before any LDS store, so they
+    // are in flight together (at most kPassPer per thread).
     int nz = 0;
-    for (int idx = threadIdx.x; idx < nb * N; idx += blockDim.x) {
-        const int bt = (int)udiv((unsigned)idx, mN), j = idx - bt * N;
-        const int q = q0 + bt;
-        const int i = (q <= T.rowband) ? q : N - nrows + q;
-        const int ia = (i > N / 2) ? (i - N) : i;
-        const int ja = (j > N / 2) ? (j - N) : j;
-        const int n = ia * ia + ja * ja;
-        const bool in = (n >= T.nlo && n < T.nhi);
+    double2 vals[kPassPer];
+#pragma unroll
+    for (int e = 0; e < kPassPer; ++e) {
+        const int idx = threadIdx.x + e * kPassThreads;
         double2 v = make_double2(0.0, 0.0);
-        if (in) {
-            if (T.plane < 0) {
-                v = make_double2(1.0, 0.0);
-            } else if (!(T.plane == 0 && i == 0 && j == 0)) {
-                const int ip = ia < 0 ? ia + NP : ia, jp = ja < 0 ? ja + NP : ja;
-                v = planes[((long long)T.plane * NP + ip) * NP + jp];
+        if (idx < nb * N) {
+            const int bt = (int)udiv((unsigned)idx, mN), j = idx - bt * N;
+            const int q = q0 + bt;
+            const int i = (q <= T.rowband) ? q : N - nrows + q;
+            const int ia = (i > N / 2) ? (i - N) : i;
+            const int ja = (j > N / 2) ? (j - N) : j;
+            const int n = ia * ia + ja * ja;
+            if (n >= T.nlo && n < T.nhi) {
+                if (T.plane < 0) {
+                    v = make_double2(1.0, 0.0);
+                } else if (!(T.plane == 0 && i == 0 && j == 0)) {
+                    const int ip = ia < 0 ? ia + NP : ia, jp = ja < 0 ? ja + NP : ja;
+                    v = planes[((long long)T.plane * NP + ip) * NP + jp];
+                }
             }
         }
-        nz |= (v.x != 0.0 || v.y != 0.0);
-        a[bt * N + j] = v;
+        vals[e] = v;
+    }
+#pragma unroll
+    for (int e = 0; e < kPassPer; ++e) {
+        const int idx = threadIdx.x + e * kPassThreads;
+        if (idx < nb * N) {
+            nz |= (vals[e].x != 0.0 || vals[e].y != 0.0);
+            a[idx] = vals[e];   // a[bt * N + j] with idx = bt * N + j
+        }
     }
     if (nzflag && __any(nz) && (threadIdx.x & 63) == 0) atomicOr(&nzflag[T.zid], 1);
     __syncthreads();
@@ -272,7 +290,7 @@ __global__ __launch_bounds__(256) void k_rowpass(const double2 *__restrict__ pla
 // and lo = bf16(v - hi) of the FP32 value v.  Field strides are multiples of
 // 8, so no group of 8 straddles two fields.
 template <typename C2>
-__global__ __launch_bounds__(256) void k_colpass(C2 *__restrict__ cache,
+__global__ __launch_bounds__(kPassThreads) void k_colpass(C2 *__restrict__ cache,
                                                  __bf16 *__restrict__ split,
                                                  const FieldTask *__restrict__ tasks,
                                                  const int2 *__restrict__ blk,
@@ -296,14 +314,24 @@ __global__ __launch_bounds__(256) void k_colpass(C2 *__restrict__ cache,
     double2 *twl = lds, *a = lds + N, *b = a + cpb * N;
     for (int t = threadIdx.x; t < N; t += blockDim.x) twl[t] = tw_all[L.tw_off + t];
     const int sh = L.cpb_log2;
-    // layout a[c * N + i]
-    for (int idx = threadIdx.x; idx < cpb * N; idx += blockDim.x) {
-        const int i = idx >> sh, c = idx & (cpb - 1);
-        const int ia = (i > N / 2) ? (i - N) : i;
-        const bool inband = (ia <= T.rowband && -ia <= T.rowband) || nrows == N;
+    // layout a[c * N + i]; all loads issued before the LDS stores
+    double2 vals[kPassPer];
+#pragma unroll
+    for (int e = 0; e < kPassPer; ++e) {
+        const int idx = threadIdx.x + e * kPassThreads;
         double2 v = make_double2(0.0, 0.0);
-        if (inband && j0 + c < N) v = load_as(fld[(long long)i * N + j0 + c]);
-        a[c * N + i] = v;
+        if (idx < cpb * N) {
+            const int i = idx >> sh, c = idx & (cpb - 1);
+            const int ia = (i > N / 2) ? (i - N) : i;
+            const bool inband = (ia <= T.rowband && -ia <= T.rowband) || nrows == N;
+            if (inband && j0 + c < N) v = load_as(fld[(long long)i * N + j0 + c]);
+        }
+        vals[e] = v;
+    }
+#pragma unroll
+    for (int e = 0; e < kPassPer; ++e) {
+        const int idx = threadIdx.x + e * kPassThreads;
+        if (idx < cpb * N) a[(idx & (cpb - 1)) * N + (idx >> sh)] = vals[e];
     }
     __syncthreads();
     double2 *res = lds_fft(a, b, cpb, L.fd, twl);
@@ -343,8 +371,8 @@ static hipError_t launch_fft(const double2 *planes, int NP, C2 *cache, __bf16 *s
         const int N = L.fd.N;
         if (L.stride < (long long)N * N || L.stride % 8 != 0 || L.rpb < 1 || L.cpb < 1)
             return hipErrorInvalidValue;
-        if (L.cpb != (1 << L.cpb_log2) || N >= 4096 || (size_t)L.rpb * N >= (1u << 20) ||
-            (size_t)L.cpb * N >= (1u << 20))
+        if (L.cpb != (1 << L.cpb_log2) || N >= 4096 || L.rpb * N > kPassThreads * kPassPer ||
+            L.cpb * N > kPassThreads * kPassPer)
             return hipErrorInvalidValue;
         lds_r = std::max(lds_r, (2 * (size_t)L.rpb + 1) * N * sizeof(double2));
         lds_c = std::max(lds_c, (2 * (size_t)L.cpb + 1) * N * sizeof(double2));
@@ -352,7 +380,7 @@ static hipError_t launch_fft(const double2 *planes, int NP, C2 *cache, __bf16 *s
     if (fb.nrow > 0) {
         (void)hipFuncSetAttribute((const void *)k_rowpass<C2>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_r);
-        hipLaunchKernelGGL(k_rowpass<C2>, dim3(fb.nrow, nf), dim3(256), lds_r, s, planes, NP,
+        hipLaunchKernelGGL(k_rowpass<C2>, dim3(fb.nrow, nf), dim3(kPassThreads), lds_r, s, planes, NP,
                            cache, tasks, fb.row, d_lv, tw, nzflag, bs);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
@@ -360,7 +388,7 @@ static hipError_t launch_fft(const double2 *planes, int NP, C2 *cache, __bf16 *s
     if (fb.ncol > 0) {
         (void)hipFuncSetAttribute((const void *)k_colpass<C2>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_c);
-        hipLaunchKernelGGL(k_colpass<C2>, dim3(fb.ncol, nf), dim3(256), lds_c, s, cache, split,
+        hipLaunchKernelGGL(k_colpass<C2>, dim3(fb.ncol, nf), dim3(kPassThreads), lds_c, s, cache, split,
                            tasks, fb.col, d_lv, tw, bs);
     }
     return hipGetLastError();
