@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <array>
+#include <chrono>
 #include <map>
 #include <new>
 #include <string>
@@ -571,11 +572,12 @@ int default_levels(const bb_options &opt)
         const int v = std::atoi(ev);
         if (v >= 1 && v <= BB_MAX_LEVELS) return v;
     }
-    return 8;
+    return 6;
 }
 
-int layout_waves()
+int layout_waves(const bb_options &opt)
 {
+    if (opt.waves == 4 || opt.waves == 8) return opt.waves;
     if (const char *ev = std::getenv("BARBIE_CONTRACT_WAVES")) {
         const int w = std::atoi(ev);
         if (w == 4 || w == 8) return w;
@@ -726,7 +728,7 @@ int build_layout(const PlanData &d, CPrec prec, const bb_options &opt, int waves
 int get_layout(const bb_plan *p, CPrec prec, const bb_options &opt,
                std::shared_ptr<const Layout> &out)
 {
-    const int waves = layout_waves(), tww = layout_tww(prec), lev = default_levels(opt);
+    const int waves = layout_waves(opt), tww = layout_tww(prec), lev = default_levels(opt);
     const std::string key = std::to_string((int)prec) + "/" + std::to_string(opt.full_grid != 0) +
                             "/" + std::to_string(opt.x_splits) + "/" + std::to_string(lev) + "/" +
                             std::to_string(waves) + "/" + std::to_string(tww);
@@ -1201,6 +1203,8 @@ int check_options(const bb_options &o)
         return set_error(BB_E_INVALID, "grid_levels must be in [0, BB_MAX_LEVELS]");
     if (o.max_batch < 0 || o.max_batch > 1024)
         return set_error(BB_E_INVALID, "max_batch must be in [0, 1024]");
+    if (o.waves != 0 && o.waves != 4 && o.waves != 8)
+        return set_error(BB_E_INVALID, "waves must be 0, 4 or 8");
     return BB_OK;
 }
 
@@ -1246,6 +1250,68 @@ int bb_plan_grid_levels(const bb_plan *p, const bb_options *opt, int *nlevels, i
         for (size_t l = 0; l < L->lv.size(); ++l) grids[l] = L->lv[l].NG;
     if (clevel)
         for (size_t c = 0; c < L->lvl.size(); ++c) clevel[c] = L->lvl[c];
+    return BB_OK;
+}
+
+int bb_tune(const bb_plan *p, const bb_options *base, const double *d_field, int repeats,
+            bb_options *best, double *best_ms)
+{
+    if (!p || !base || !best) return set_error(BB_E_INVALID, "null argument");
+    CPrec cp;
+    int rc = precision_of(base->precision, cp);
+    if (!rc) rc = check_options(*base);
+    if (rc) return rc;
+    repeats = std::max(1, std::min(repeats, 16));
+    const PlanData &d = p->d;
+    const size_t fsz = (size_t)d.g.NN * d.g.NN * (d.g.NN / 2 + 1);
+    const size_t nb = (size_t)d.nbins * BB_NPOLES;
+    // Timing does not depend on the field's values: without one, zeros.
+    double2 *zf = nullptr;
+    double *bins = nullptr;
+    if (!d_field) {
+        if (hipMalloc(&zf, fsz * sizeof(double2)) != hipSuccess || hipMemset(zf, 0, fsz * 16) != hipSuccess) {
+            (void)hipGetLastError();
+            if (zf) (void)hipFree(zf);
+            return set_error(BB_E_DEVMEM, "tuning field");
+        }
+        d_field = (const double *)zf;
+    }
+    if (hipMalloc(&bins, 3 * nb * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        if (zf) (void)hipFree(zf);
+        return set_error(BB_E_DEVMEM, "tuning bins");
+    }
+    double best_t = 1e300;
+    bb_options win = *base;
+    const int lev_hi = base->grid_levels > 0 ? base->grid_levels : 8;
+    for (int waves : {8, 4})
+        for (int lev = 3; lev <= lev_hi; ++lev) {
+            bb_options o = *base;
+            o.waves = waves;
+            o.grid_levels = lev;
+            o.max_batch = 1;
+            bb_estimator *E = nullptr;
+            if (bb_estimator_create_opts(p, &o, &E)) continue;   // e.g. out of memory: skip
+            double t = 1e300;
+            int ok = estimate_device(E, 1, d_field, bins, bins + nb, bins + 2 * nb) == BB_OK;
+            for (int r = 0; ok && r < repeats; ++r) {
+                const auto t0 = std::chrono::steady_clock::now();
+                ok = estimate_device(E, 1, d_field, bins, bins + nb, bins + 2 * nb) == BB_OK;
+                t = std::min(t, std::chrono::duration<double, std::milli>(
+                                    std::chrono::steady_clock::now() - t0).count());
+            }
+            bb_estimator_destroy(E);
+            if (ok && t < best_t) {
+                best_t = t;
+                win = o;
+            }
+        }
+    (void)hipFree(bins);
+    if (zf) (void)hipFree(zf);
+    if (best_t >= 1e300) return set_error(BB_E_DEVICE, "no candidate configuration ran");
+    win.max_batch = base->max_batch;
+    *best = win;
+    if (best_ms) *best_ms = best_t;
     return BB_OK;
 }
 

@@ -63,6 +63,8 @@ struct Model {
     bb_plan *plan = nullptr;
     size_t fsz = 0;   // doubles per field: NN * NN * (NN + 2)
     size_t nb = 0;    // doubles per output array: nbins * 16
+    bool tuned = false;
+    bb_options eo{};  // estimator options of this geometry's lanes
     std::vector<double *> pinned;   // free pinned host field buffers
     ~Model()
     {
@@ -127,12 +129,8 @@ namespace {
 
 int lane_bytes(Model *m, const bb_server_options &o, size_t *bytes)
 {
-    bb_options eo;
-    std::memset(&eo, 0, sizeof eo);
-    eo.precision = o.precision;
-    eo.max_batch = o.max_batch;
     size_t b = 0;
-    int rc = bb_estimator_footprint_opts(m->plan, &eo, &b);
+    int rc = bb_estimator_footprint_opts(m->plan, &m->eo, &b);
     if (rc) return rc;
     b += (size_t)o.max_batch * m->fsz * 8 + 3 * (size_t)o.max_batch * m->nb * 8;
     *bytes = b;
@@ -162,6 +160,12 @@ void bb_server::free_lane(Lane *L)
 int bb_server::make_lane(Model *m, Lane **out)
 {
     *out = nullptr;
+    if (opt.tune && !m->tuned) {
+        // First lane of this geometry: time the candidate layouts once.
+        bb_options best;
+        if (bb_tune(m->plan, &m->eo, nullptr, 2, &best, nullptr) == BB_OK) m->eo = best;
+        m->tuned = true;
+    }
     size_t need = 0;
     int rc = lane_bytes(m, opt, &need);
     if (rc) return rc;
@@ -184,11 +188,7 @@ int bb_server::make_lane(Model *m, Lane **out)
     if (used + need > budget) return BB_E_DEVMEM;   // caller waits or fails the request
     auto L = std::make_unique<Lane>();
     L->m = m;
-    bb_options eo;
-    std::memset(&eo, 0, sizeof eo);
-    eo.precision = opt.precision;
-    eo.max_batch = opt.max_batch;
-    rc = bb_estimator_create_opts(m->plan, &eo, &L->est);
+    rc = bb_estimator_create_opts(m->plan, &m->eo, &L->est);
     if (!rc) rc = bb_estimator_stream(L->est, (void **)&L->stream);
     const size_t B = (size_t)opt.max_batch;
     if (!rc && hipMalloc(&L->d_in, B * m->fsz * 8) != hipSuccess)
@@ -413,6 +413,9 @@ int bb_submit(bb_server *S, const bb_geometry *g, const double *field_r2c, int o
         if ((rc = bb_plan_stats_get(nm->plan, &st))) return rc;
         nm->fsz = (size_t)g->NN * g->NN * (g->NN + 2);
         nm->nb = (size_t)st.nbins * BB_NPOLES;
+        std::memset(&nm->eo, 0, sizeof nm->eo);
+        nm->eo.precision = S->opt.precision;
+        nm->eo.max_batch = S->opt.max_batch;
         std::lock_guard<std::mutex> lk(S->mu);
         auto ins = S->models.emplace(key, std::move(nm));
         m = ins.first->second.get();

@@ -26,7 +26,8 @@ RINGS_REFERENCE, RINGS_UNIFORM, RINGS_EXPLICIT = 0, 1, 2
 EXPORTED = [
     "bb_last_error", "bb_version", "bb_geometry_reference", "bb_plan_create",
     "bb_plan_destroy", "bb_plan_stats_get", "bb_plan_ring_edges", "bb_plan_triples",
-    "bb_plan_contractions", "bb_plan_grid_levels", "bb_plan_worklist", "bb_estimator_create", "bb_estimator_create_opts",
+    "bb_plan_contractions", "bb_plan_grid_levels", "bb_plan_worklist", "bb_tune",
+    "bb_estimator_create", "bb_estimator_create_opts",
     "bb_estimator_destroy", "bb_estimator_grid", "bb_estimator_levels",
     "bb_estimator_footprint", "bb_estimator_footprint_opts", "bb_estimate", "bb_estimate_device",
     "bb_estimate_batch", "bb_estimate_batch_device", "bb_estimate_batch_async",
@@ -67,12 +68,12 @@ class BBPlanStats(ctypes.Structure):
 class BBOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("full_grid", ctypes.c_int),
                 ("x_splits", ctypes.c_int), ("grid_levels", ctypes.c_int),
-                ("max_batch", ctypes.c_int)]
+                ("max_batch", ctypes.c_int), ("waves", ctypes.c_int)]
 
 
 class BBServerOptions(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_inflight", ctypes.c_int),
-                ("max_batch", ctypes.c_int), ("precision", ctypes.c_int)]
+                ("max_batch", ctypes.c_int), ("precision", ctypes.c_int), ("tune", ctypes.c_int)]
 
 
 class BarbieError(RuntimeError):
@@ -136,6 +137,8 @@ def lib() -> ctypes.CDLL:
         L.bb_estimator_levels.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), _ip]
         L.bb_estimator_footprint_opts.argtypes = [ctypes.c_void_p, ctypes.POINTER(BBOptions),
                                                   ctypes.POINTER(ctypes.c_size_t)]
+        L.bb_tune.argtypes = [ctypes.c_void_p, ctypes.POINTER(BBOptions), ctypes.c_void_p,
+                              ctypes.c_int, ctypes.POINTER(BBOptions), _dp]
         L.bb_plan_worklist.argtypes = [ctypes.c_void_p, ctypes.POINTER(BBOptions),
                                        ctypes.POINTER(ctypes.c_longlong)]
         L.bb_plan_grid_levels.argtypes = [ctypes.c_void_p, ctypes.POINTER(BBOptions),
@@ -242,7 +245,7 @@ class Plan:
         """Device bytes an Estimator with these options holds."""
         b = ctypes.c_size_t()
         o = BBOptions(PRECISIONS[precision], int(full_grid), int(x_splits), int(grid_levels),
-                      int(max_batch))
+                      int(max_batch), 0)
         _check(lib().bb_estimator_footprint_opts(self.handle, ctypes.byref(o), ctypes.byref(b)))
         return b.value
 
@@ -253,7 +256,7 @@ class Plan:
         n = ctypes.c_int()
         grids = (ctypes.c_int * 16)()
         cl = np.zeros(self.stats["ncontr"], dtype=np.int32)
-        o = BBOptions(PRECISIONS[precision], int(full_grid), 0, int(grid_levels), 0)
+        o = BBOptions(PRECISIONS[precision], int(full_grid), 0, int(grid_levels), 0, 0)
         _check(lib().bb_plan_grid_levels(self.handle, ctypes.byref(o), ctypes.byref(n), grids,
                                          cl.ctypes.data_as(_ip)))
         return [grids[i] for i in range(n.value)], cl
@@ -262,9 +265,21 @@ class Plan:
                  grid_levels: int = 0) -> dict:
         """Work-list figures of an Estimator with these options (host only)."""
         o = (ctypes.c_longlong * len(WORKLIST_KEYS))()
-        opt = BBOptions(PRECISIONS[precision], int(full_grid), int(x_splits), int(grid_levels), 0)
+        opt = BBOptions(PRECISIONS[precision], int(full_grid), int(x_splits), int(grid_levels), 0,
+                        0)
         _check(lib().bb_plan_worklist(self.handle, ctypes.byref(opt), o))
         return dict(zip(WORKLIST_KEYS, list(o)))
+
+    def tune(self, precision: str = "bf16x3", full_grid: bool = False, d_field: int = 0,
+             repeats: int = 2, max_levels: int = 0):
+        """bb_tune: times candidate layouts on the current device; returns
+        (options dict for Estimator(**options), ms per estimate)."""
+        base = BBOptions(PRECISIONS[precision], int(full_grid), 0, int(max_levels), 0, 0)
+        best = BBOptions()
+        ms = ctypes.c_double()
+        _check(lib().bb_tune(self.handle, ctypes.byref(base), ctypes.c_void_p(d_field or None),
+                             int(repeats), ctypes.byref(best), ctypes.byref(ms)))
+        return {"grid_levels": best.grid_levels, "waves": best.waves}, ms.value
 
     def write_outputs(self, bs, norm, biana, outdir: str, NRi: int):
         bs, norm, biana = (np.ascontiguousarray(a, dtype=np.float64) for a in (bs, norm, biana))
@@ -276,13 +291,13 @@ class Estimator:
     """Device estimator (bb_estimator) for one plan and precision."""
 
     def __init__(self, plan: Plan, precision: str = "fp64", full_grid: bool = False,
-                 x_splits: int = 0, grid_levels: int = 0, max_batch: int = 0):
+                 x_splits: int = 0, grid_levels: int = 0, max_batch: int = 0, waves: int = 0):
         self.plan = plan
         self.precision = precision
         self.max_batch = max(1, int(max_batch))
         h = ctypes.c_void_p()
         o = BBOptions(PRECISIONS[precision], int(full_grid), int(x_splits), int(grid_levels),
-                      int(max_batch))
+                      int(max_batch), int(waves))
         _check(lib().bb_estimator_create_opts(plan.handle, ctypes.byref(o), ctypes.byref(h)))
         self.handle = h
         ng = ctypes.c_int()
@@ -396,9 +411,9 @@ class Server:
     request is done)."""
 
     def __init__(self, precision: str = "bf16x3", hbm_budget: int = 0, max_inflight: int = 0,
-                 max_batch: int = 0):
+                 max_batch: int = 0, tune: bool = False):
         o = BBServerOptions(int(hbm_budget), int(max_inflight), int(max_batch),
-                            PRECISIONS[precision])
+                            PRECISIONS[precision], int(tune))
         h = ctypes.c_void_p()
         _check(lib().bb_server_create(ctypes.byref(o), ctypes.byref(h)))
         self.handle = h

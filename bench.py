@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the fNL 0.1 / 1.0 fields")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="use the default layout instead of the autotuned one (bb_tune)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a reference point for "
                          "HBM byte counters)")
@@ -63,8 +65,8 @@ def geometry(B, NN):
                       Nr=int(round(NN / 10)), n_mtbin=5)
 
 
-def time_path(B, torch, plan, precision, d_field, steps, warmup, full_grid=False):
-    est = B.Estimator(plan, precision, full_grid=full_grid)
+def time_path(B, torch, plan, precision, d_field, steps, warmup, full_grid=False, opts=None):
+    est = B.Estimator(plan, precision, full_grid=full_grid, **(opts or {}))
     wl = est.worklist()
     levels = est.levels
     nb = plan.geom.nbins * 16
@@ -280,11 +282,20 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    tuned = {}
+    for prec in ([a.precision] + ([] if a.skip_fp64 or a.precision == "fp64" else ["fp64"])):
+        if a.no_tune:
+            tuned[prec] = ({}, None, 0.0)
+        else:
+            t0 = time.perf_counter()
+            opts, tms = plan.tune(prec, full_grid=a.full_grid, d_field=d_field.data_ptr())
+            tuned[prec] = (opts, tms, time.perf_counter() - t0)
     ms, st, bins, levels, wl = time_path(B, torch, plan, a.precision, d_field, a.steps,
-                                         a.warmup, a.full_grid)
+                                         a.warmup, a.full_grid, tuned[a.precision][0])
     res = {"fast": (ms, st, bins)}
-    if not a.skip_fp64 and a.precision != "fp64":
-        r64 = time_path(B, torch, plan, "fp64", d_field, a.steps, a.warmup, a.full_grid)
+    if "fp64" in tuned and a.precision != "fp64":
+        r64 = time_path(B, torch, plan, "fp64", d_field, a.steps, a.warmup, a.full_grid,
+                        tuned["fp64"][0])
         res["fp64"] = r64[:3]
         wl64 = r64[4]
 
@@ -321,6 +332,8 @@ def main():
                    "precision": a.precision, "grid_levels": levels},
         "stages_ms": {"gather": st[0], "fft_cache": st[1], "contraction": st[2],
                       "binning": st[3]},
+        "tuned": {p: {"options": o, "tune_ms_per_estimate": m, "tune_seconds": ts}
+                  for p, (o, m, ts) in tuned.items()},
         "worklist": wl,
         "roofline": {"bound": "mfma", "kernel": "contraction",
                      "achieved": w_ref / t_contr / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",

@@ -73,6 +73,8 @@ typedef struct bb_options {
     int max_batch;  /* fields one launch sequence processes together (0 or 1:
                        one).  Device buffers per field are held max_batch
                        times; the work lists are shared. */
+    int waves;      /* waves per contraction workgroup: 4 or 8 (0: 8, or the
+                       environment variable BARBIE_CONTRACT_WAVES) */
 } bb_options;
 
 /* Geometry and binning of one estimate.  Replaces the constants hard-coded at
@@ -148,6 +150,16 @@ int bb_estimator_grid(const bb_estimator *e, int *NG);
 /* Grid levels in ascending size: *nlevels and up to BB_MAX_LEVELS grids. */
 #define BB_MAX_LEVELS 16
 int bb_estimator_levels(const bb_estimator *e, int *nlevels, int *grids);
+/* Autotuner (no counterpart in the reference): times an estimate for each
+ * candidate layout -- waves per contraction workgroup (8, 4) x number of grid
+ * levels (3 .. base->grid_levels, or 8) -- on the current device and returns
+ * in *best the fastest options (base with waves and grid_levels set) and its
+ * time per estimate in *best_ms (may be NULL).  d_field: a device field to
+ * time with, or NULL for zeros (the work does not depend on the values).
+ * Each candidate runs once untimed and `repeats` times timed (best kept).
+ * The bins of every candidate are the same to FP64 rounding. */
+int bb_tune(const bb_plan *p, const bb_options *base, const double *d_field, int repeats,
+            bb_options *best, double *best_ms);
 /* Device bytes an estimator of this plan and precision holds (default
  * options); the layout it measures is cached in the plan and reused by
  * bb_estimator_create. */
@@ -230,6 +242,7 @@ typedef struct bb_server_options {
     int max_inflight;        /* lanes running at once (0: 4; at most 16) */
     int max_batch;           /* fields per lane launch (0: 4; at most 256) */
     bb_precision precision;
+    int tune;                /* 1: run bb_tune once per geometry, before its first lane */
 } bb_server_options;
 enum { BB_REQ_QUEUED = 0, BB_REQ_RUNNING = 1, BB_REQ_DONE = 2, BB_REQ_FAILED = 3 };
 int bb_server_create(const bb_server_options *opt, bb_server **out);

@@ -229,3 +229,18 @@ def test_cli_end_to_end(tmp_path):
     assert rel_err(dump[1], o["nm"]) <= 1e-12
     assert (outdir / f"time_{g.NN}_R7.txt").read_text().strip().splitlines()[-1].startswith(
         "Total time")
+
+
+def test_tuned_layout_matches_oracle():
+    """bb_tune picks among layouts that all give the reference's sums."""
+    o = oracle_run(medium, NRi=1)
+    g = o["g"]
+    plan = B.Plan(g)
+    opts, ms = plan.tune("fp64", repeats=1)
+    assert opts["waves"] in (4, 8) and 3 <= opts["grid_levels"] <= 8 and ms > 0
+    est = B.Estimator(plan, "fp64", **opts)
+    bs, nm, bi = est.estimate(o["field"])
+    obs = o["bs"].reshape(bs.shape)
+    assert rel_err(bs, obs) <= 1e-10
+    four = B.Estimator(plan, "fp64", waves=4, grid_levels=3).estimate(o["field"])
+    assert rel_err(four[0], obs) <= 1e-10

@@ -108,3 +108,13 @@ def test_level_count_option_is_validated():
     plan = B.Plan(medium())
     with pytest.raises(B.BarbieError):
         plan.grid_levels("fp64", grid_levels=17)
+
+
+def test_waves_option_is_validated():
+    plan = B.Plan(medium())
+    o = B.native.BBOptions(B.PRECISIONS["fp64"], 0, 0, 0, 0, 5)
+    out = (B.native.ctypes.c_longlong * len(B.native.WORKLIST_KEYS))()
+    rc = B.lib().bb_plan_worklist(plan.handle, B.native.ctypes.byref(o), out)
+    assert rc == -1
+    w4 = plan.worklist("fp64")
+    assert w4["waves"] == 8
