@@ -139,7 +139,7 @@ int main(int argc, char *argv[])
         return 4;
     }
     double t_done = now();
-    double ms[4];
+    double ms[5];
     bb_estimator_timings(est, ms);
 
     char dir[512];

@@ -784,11 +784,18 @@ struct bb_estimator {
     double *d_avg = nullptr;
     double *d_bins = nullptr;
     hipEvent_t ev[5] = {};
-    float ms[4] = {0, 0, 0, 0};
+    float ms[5] = {0, 0, 0, 0, 0};
+    // Captured launch sequence (use_graph) and the arguments it was captured for.
+    hipGraphExec_t gexec = nullptr;
+    int g_nf = 0;
+    const double *g_in = nullptr;
+    double *g_out[3] = {nullptr, nullptr, nullptr};
+    bool graph_run = false;   // the pending batch ran as a graph
 
     ~bb_estimator()
     {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
         for (void *p : owned) (void)hipFree(p);
@@ -1101,44 +1108,99 @@ int estimator_init(bb_estimator *E)
     return BB_OK;
 }
 
-// Enqueues the estimates of nf fields (back to back from d_fields) on the
-// estimator's stream; outputs back to back per field.  Nothing waits.
-int estimate_enqueue(bb_estimator *E, int nf, const double *d_fields, double *d_bs,
-                     double *d_norm, double *d_biana)
+// The launch sequence of one batch: nonzero flags cleared by a fill kernel,
+// plane gather, transforms, contraction, binning.  `timed` records the stage
+// events between the launches (not inside a captured graph).
+int enqueue_kernels(bb_estimator *E, int nf, const double *d_fields, double *d_bs,
+                    double *d_norm, double *d_biana, bool timed)
 {
     const PlanData &d = E->plan->d;
     const Layout &L = *E->L;
-    if (nf < 1 || nf > E->batch)
-        return set_error(BB_E_RANGE, "batch of " + std::to_string(nf) + " fields; the estimator "
-                                         "holds " + std::to_string(E->batch));
     hipStream_t s = E->stream;
-    HIPCHK(hipEventRecord(E->ev[0], s));
     const int nz = d.g.Nmax * d.g.Nr * nf;
     hipLaunchKernelGGL(k_fill_int, dim3((nz + 255) / 256), dim3(256), 0, s, E->d_nz, nz, 0);
     HIPCHK(hipGetLastError());
     HIPCHK(launch_gather((const double2 *)d_fields, E->d_planes, L.NN, L.NGtop, L.kmax,
                          d.g.Nmax, nf, s));
-    HIPCHK(hipEventRecord(E->ev[1], s));
+    if (timed) HIPCHK(hipEventRecord(E->ev[1], s));
     int rc = run_fields(E, E->d_planes, L.NGtop, E->d_nz, nf);
     if (rc) return rc;
-    HIPCHK(hipEventRecord(E->ev[2], s));
+    if (timed) HIPCHK(hipEventRecord(E->ev[2], s));
     HIPCHK(launch_contract(E->cprec, L.waves, L.tww, E->d_cache, E->d_split, E->d_items,
                            E->d_units, (int)L.units.size(), E->d_tws, E->d_part, E->bst, nf, s));
-    HIPCHK(hipEventRecord(E->ev[3], s));
+    if (timed) HIPCHK(hipEventRecord(E->ev[3], s));
     HIPCHK(launch_binning(E->d_seg, E->d_recs, E->d_part, E->d_num, E->d_nz, E->d_avg, L.bp,
                           d_bs, d_norm, d_biana, E->bst, nf, s));
-    HIPCHK(hipEventRecord(E->ev[4], s));
-    E->pending = true;
     return BB_OK;
 }
 
-// Waits for the enqueued batch and records its stage times.
+// Enqueues the estimates of nf fields (back to back from d_fields) on the
+// estimator's stream; outputs back to back per field.  Nothing waits.  With
+// use_graph, the first call for a set of arguments runs eagerly and captures
+// the sequence into a graph; later calls with the same arguments replay it.
+int estimate_enqueue(bb_estimator *E, int nf, const double *d_fields, double *d_bs,
+                     double *d_norm, double *d_biana)
+{
+    if (nf < 1 || nf > E->batch)
+        return set_error(BB_E_RANGE, "batch of " + std::to_string(nf) + " fields; the estimator "
+                                         "holds " + std::to_string(E->batch));
+    hipStream_t s = E->stream;
+    const bool same = E->gexec && E->g_nf == nf && E->g_in == d_fields && E->g_out[0] == d_bs &&
+                      E->g_out[1] == d_norm && E->g_out[2] == d_biana;
+    HIPCHK(hipEventRecord(E->ev[0], s));
+    if (E->opt.use_graph && same) {
+        HIPCHK(hipGraphLaunch(E->gexec, s));
+        E->graph_run = true;
+    } else {
+        int rc = enqueue_kernels(E, nf, d_fields, d_bs, d_norm, d_biana, true);
+        if (rc) return rc;
+        E->graph_run = false;
+    }
+    HIPCHK(hipEventRecord(E->ev[4], s));
+    E->pending = true;
+    if (E->opt.use_graph && !same) {
+        if (E->gexec) {
+            HIPCHK(hipGraphExecDestroy(E->gexec));
+            E->gexec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+        const int rc = enqueue_kernels(E, nf, d_fields, d_bs, d_norm, d_biana, false);
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return set_error(BB_E_DEVICE, "graph capture failed");
+        const hipError_t ei = hipGraphInstantiate(&E->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            E->gexec = nullptr;
+            return set_error(BB_E_DEVICE, "graph instantiation failed");
+        }
+        E->g_nf = nf;
+        E->g_in = d_fields;
+        E->g_out[0] = d_bs;
+        E->g_out[1] = d_norm;
+        E->g_out[2] = d_biana;
+    }
+    return BB_OK;
+}
+
+// Waits for the enqueued batch and records its stage times ([4]: total; a
+// replayed graph has no stage boundaries, its stages read 0).
 int estimate_finish(bb_estimator *E)
 {
     if (!E->pending) return BB_OK;
     E->pending = false;
     HIPCHK(hipStreamSynchronize(E->stream));
-    for (int k = 0; k < 4; ++k) HIPCHK(hipEventElapsedTime(&E->ms[k], E->ev[k], E->ev[k + 1]));
+    HIPCHK(hipEventElapsedTime(&E->ms[4], E->ev[0], E->ev[4]));
+    if (E->graph_run) {
+        for (int k = 0; k < 4; ++k) E->ms[k] = 0.0f;
+    } else {
+        for (int k = 0; k < 4; ++k)
+            HIPCHK(hipEventElapsedTime(&E->ms[k], E->ev[k], k == 3 ? E->ev[4] : E->ev[k + 1]));
+    }
     return BB_OK;
 }
 
@@ -1548,11 +1610,11 @@ int bb_plan_worklist(const bb_plan *p, const bb_options *opt, long long *o)
     return BB_OK;
 }
 
-int bb_estimator_timings(bb_estimator *e, double *ms4)
+int bb_estimator_timings(bb_estimator *e, double *ms5)
 {
-    if (!e || !ms4) return set_error(BB_E_INVALID, "null argument");
+    if (!e || !ms5) return set_error(BB_E_INVALID, "null argument");
     if (int rc = estimate_finish(e)) return rc;
-    for (int k = 0; k < 4; ++k) ms4[k] = e->ms[k];
+    for (int k = 0; k < 5; ++k) ms5[k] = e->ms[k];
     return BB_OK;
 }
 

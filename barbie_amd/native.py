@@ -68,7 +68,8 @@ class BBPlanStats(ctypes.Structure):
 class BBOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("full_grid", ctypes.c_int),
                 ("x_splits", ctypes.c_int), ("grid_levels", ctypes.c_int),
-                ("max_batch", ctypes.c_int), ("waves", ctypes.c_int)]
+                ("max_batch", ctypes.c_int), ("waves", ctypes.c_int),
+                ("use_graph", ctypes.c_int)]
 
 
 class BBServerOptions(ctypes.Structure):
@@ -291,13 +292,14 @@ class Estimator:
     """Device estimator (bb_estimator) for one plan and precision."""
 
     def __init__(self, plan: Plan, precision: str = "fp64", full_grid: bool = False,
-                 x_splits: int = 0, grid_levels: int = 0, max_batch: int = 0, waves: int = 0):
+                 x_splits: int = 0, grid_levels: int = 0, max_batch: int = 0, waves: int = 0,
+                 use_graph: bool = False):
         self.plan = plan
         self.precision = precision
         self.max_batch = max(1, int(max_batch))
         h = ctypes.c_void_p()
         o = BBOptions(PRECISIONS[precision], int(full_grid), int(x_splits), int(grid_levels),
-                      int(max_batch), int(waves))
+                      int(max_batch), int(waves), int(use_graph))
         _check(lib().bb_estimator_create_opts(plan.handle, ctypes.byref(o), ctypes.byref(h)))
         self.handle = h
         ng = ctypes.c_int()
@@ -387,7 +389,8 @@ class Estimator:
         return dict(zip(WORKLIST_KEYS, list(o)))
 
     def timings(self) -> np.ndarray:
-        t = np.zeros(4)
+        """[gather, fft, contraction, binning, total] ms of the last estimate."""
+        t = np.zeros(5)
         _check(lib().bb_estimator_timings(self.handle, _p(t)))
         return t
 

@@ -75,6 +75,11 @@ typedef struct bb_options {
                        times; the work lists are shared. */
     int waves;      /* waves per contraction workgroup: 4 or 8 (0: 8, or the
                        environment variable BARBIE_CONTRACT_WAVES) */
+    int use_graph;  /* 1: capture the launch sequence of a batch into a HIP
+                       graph the first time it runs with a set of device
+                       buffers, and replay the graph when the same buffers
+                       come again (bb_estimate and bb_estimate_batch use the
+                       estimator's own buffers, so they always replay) */
 } bb_options;
 
 /* Geometry and binning of one estimate.  Replaces the constants hard-coded at
@@ -224,9 +229,10 @@ int bb_estimator_worklist(const bb_estimator *e, long long *out);
  * on the host without a device ([13]: tile window slots executed, i.e.
  * items x waves x windows per wave). */
 int bb_plan_worklist(const bb_plan *p, const bb_options *opt, long long *out);
-/* Per-stage device times of the last estimate in milliseconds:
- * [0] upload/gather, [1] plane-ring FFTs, [2] contraction, [3] binning. */
-int bb_estimator_timings(bb_estimator *e, double *ms4);
+/* Device times of the last estimate in milliseconds: [0] plane gather,
+ * [1] plane-ring FFTs, [2] contraction, [3] binning, [4] total.  A replayed
+ * graph (use_graph) has no stage boundaries: [0]-[3] read 0. */
+int bb_estimator_timings(bb_estimator *e, double *ms5);
 
 /* Request server (no counterpart in the reference, which runs one field per
  * process: barbie.c:58-397 started by run:1 / many_fields.sh).  One scheduler
