@@ -207,6 +207,7 @@ struct Layout {
     BinParams bp{};
     int max_rows = 0;
     long long staged_rows = 0, staged_bytes = 0, useful_pts = 0;
+    long long total_cost = 0;          // tile windows of the busiest waves x chunks, all items
     double wave_balance = 1.0;
 };
 
@@ -334,9 +335,15 @@ int pack_items(std::vector<PairBlock> &blocks, const Level &L, bool split_b, int
 // of the windows and of pos into partial-tile positions; units are ordered
 // largest first.
 constexpr long long kTargetUnits = 2048;
+// Largest cost of one unit (tile windows of the busiest wave x chunks).  Long
+// workgroups delay the kernels of other streams that wait for a free CU
+// (the request server runs several estimates at once), so large estimates get
+// more units rather than longer ones.
+constexpr long long kMaxUnitCost = 1024;
 
 int make_units(WorkLists &W, int chunk, int x_splits, std::vector<WorkUnit> &units,
-               std::vector<int32_t> &pos, std::vector<int16_t> &nsp, long long &part_tiles)
+               std::vector<int32_t> &pos, std::vector<int16_t> &nsp, long long &part_tiles,
+               long long *total_cost)
 {
     const size_t n = W.items.size();
     std::vector<int> nch(n), sp(n);
@@ -345,10 +352,12 @@ int make_units(WorkLists &W, int chunk, int x_splits, std::vector<WorkUnit> &uni
         nch[i] = (int)((W.items[i].npts + chunk - 1) / chunk);
         total += (long long)W.item_work[i] * nch[i];
     }
-    const long long unit_cost = std::max<long long>(1, total / kTargetUnits);
+    if (total_cost) *total_cost = total;
+    const long long unit_cost =
+        std::max<long long>(1, std::min(total / kTargetUnits, kMaxUnitCost));
     for (size_t i = 0; i < n; ++i) {
         const long long cost = (long long)W.item_work[i] * nch[i];
-        int s = x_splits > 0 ? x_splits : (int)std::min<long long>(64, (cost + unit_cost - 1) / unit_cost);
+        int s = x_splits > 0 ? x_splits : (int)std::min<long long>(256, (cost + unit_cost - 1) / unit_cost);
         sp[i] = std::max(1, std::min(s, nch[i]));
     }
     std::vector<int> tile_item(W.ntw, -1);
@@ -681,7 +690,8 @@ int build_layout(const PlanData &d, CPrec prec, const bb_options &opt, int waves
         L.staged_bytes += (long long)it.nrows * it.npts * (long long)elem;   // padding excluded
     }
     for (size_t c = 0; c < d.contr.size(); ++c) L.useful_pts += L.lv[lvl[c]].npts;
-    int rc = make_units(W, L.chunk, opt.x_splits, L.units, L.pos, L.nsp, L.part_tiles);
+    int rc = make_units(W, L.chunk, opt.x_splits, L.units, L.pos, L.nsp, L.part_tiles,
+                        &L.total_cost);
     if (rc) return rc;
     L.items.swap(W.items);
     L.tws.swap(W.tws);
@@ -791,6 +801,7 @@ struct bb_estimator {
     const double *g_in = nullptr;
     double *g_out[3] = {nullptr, nullptr, nullptr};
     bool graph_run = false;   // the pending batch ran as a graph
+    hipStream_t own_stream = nullptr;   // created with the estimator; `stream` may be lent
 
     ~bb_estimator()
     {
@@ -799,7 +810,7 @@ struct bb_estimator {
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
         for (void *p : owned) (void)hipFree(p);
-        if (stream) (void)hipStreamDestroy(stream);
+        if (own_stream) (void)hipStreamDestroy(own_stream);
     }
 };
 
@@ -897,7 +908,7 @@ int compute_num(bb_estimator *E)
     std::vector<PairBlock> blocks;
     if (rcount > 0) pair_blocks(cs, 0, 0, 0, 0, blocks);
     int rc = pack_items(blocks, U, false, contract_tww(CPrec::F64), kWaves, W, L.pos);
-    if (!rc) rc = make_units(W, L.chunk, 0, L.units, L.pos, L.nsp, L.part_tiles);
+    if (!rc) rc = make_units(W, L.chunk, 0, L.units, L.pos, L.nsp, L.part_tiles, nullptr);
     if (rc) return rc;
     L.items.swap(W.items);
     L.tws.swap(W.tws);
@@ -1018,7 +1029,8 @@ int estimator_init(bb_estimator *E)
         (void)hipGetLastError();
         return set_error(BB_E_DEVICE, "no HIP device available");
     }
-    HIPCHK(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&E->own_stream, hipStreamNonBlocking));
+    E->stream = E->own_stream;
     for (auto &e : E->ev) HIPCHK(hipEventCreate(&e));
     if ((rc = compute_num(E))) return rc;
     if ((rc = dev_alloc(&E->d_num, E->num_host.size(), E->owned))) return rc;
@@ -1345,9 +1357,12 @@ int bb_tune(const bb_plan *p, const bb_options *base, const double *d_field, int
     }
     double best_t = 1e300;
     bb_options win = *base;
-    const int lev_hi = base->grid_levels > 0 ? base->grid_levels : 8;
+    std::vector<int> levs;
+    for (int lev : {4, 5, 6, 8})
+        if (base->grid_levels <= 0 || lev <= base->grid_levels) levs.push_back(lev);
+    if (levs.empty()) levs.push_back(base->grid_levels);
     for (int waves : {8, 4})
-        for (int lev = 3; lev <= lev_hi; ++lev) {
+        for (int lev : levs) {
             bb_options o = *base;
             o.waves = waves;
             o.grid_levels = lev;
@@ -1517,6 +1532,14 @@ int bb_estimator_stream(const bb_estimator *e, void **stream)
     return BB_OK;
 }
 
+int bb_estimator_set_stream(bb_estimator *e, void *stream)
+{
+    if (!e) return set_error(BB_E_INVALID, "null argument");
+    if (int rc = estimate_finish(e)) return rc;
+    e->stream = stream ? (hipStream_t)stream : e->own_stream;
+    return BB_OK;
+}
+
 int bb_estimator_contraction_sums(bb_estimator *e, double *S)
 {
     if (!e || !S) return set_error(BB_E_INVALID, "null argument");
@@ -1583,6 +1606,7 @@ static void worklist_of(const Layout &L, long long *o)
     o[11] = L.part_tiles;
     o[12] = L.useful_pts;
     o[13] = (long long)L.items.size() * L.waves * L.tww;
+    o[14] = L.total_cost;
 }
 
 int bb_estimator_worklist(const bb_estimator *e, long long *o)

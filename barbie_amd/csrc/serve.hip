@@ -8,9 +8,10 @@
 //    output buffers; a lane is created when its device bytes fit the HBM
 //    budget, after evicting idle lanes of other geometries (least recently
 //    used first);
-//  - continuous batching: whenever a lane is idle, the oldest queued request
-//    takes it together with the following queued requests of the same
-//    geometry, up to max_batch, as one launch sequence;
+//  - continuous batching: whenever a lane is idle, the next request (shortest
+//    expected time first, with a waiting bound) takes it together with the
+//    queued requests of the same geometry, up to max_batch, as one launch
+//    sequence;
 //  - up to max_inflight lanes run at once (one stream each), so small and
 //    large estimates overlap on the device.
 // A field's bins do not depend on what else runs: its lane's buffers are its
@@ -65,6 +66,7 @@ struct Model {
     size_t nb = 0;    // doubles per output array: nbins * 16
     bool tuned = false;
     bb_options eo{};  // estimator options of this geometry's lanes
+    double svc_ms = 0.0;   // running estimate of one field's time on a lane (0: none yet)
     std::vector<double *> pinned;   // free pinned host field buffers
     ~Model()
     {
@@ -88,7 +90,8 @@ struct Req {
 struct Lane {
     Model *m = nullptr;
     bb_estimator *est = nullptr;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // the server stream lent while busy
+    int slot = -1;
     double *d_in = nullptr;     // batch fields
     double *d_out = nullptr;    // bs | norm | biana, each batch * nb
     double *h_out = nullptr;    // pinned copy of d_out
@@ -111,6 +114,13 @@ struct bb_server {
     std::map<std::string, std::unique_ptr<Model>> models;
     std::vector<std::unique_ptr<Lane>> lanes;
     long long next_id = 1;
+    // max_inflight streams, lent to busy lanes.  With reserved CUs, streams
+    // [0, nshort) run on the reserved CUs and take short requests only; the
+    // others run on the remaining CUs.
+    std::vector<hipStream_t> streams;
+    std::vector<char> stream_busy;
+    int reserved = 0;   // reserved CUs (0: no reservation)
+    int nshort = 0;     // streams on the reserved CUs
     long long stats[8] = {0};   // submitted, completed, failed, batches, fields in batches,
                                 // lanes created, lanes evicted, peak bytes
     bool stop = false;
@@ -189,7 +199,6 @@ int bb_server::make_lane(Model *m, Lane **out)
     auto L = std::make_unique<Lane>();
     L->m = m;
     rc = bb_estimator_create_opts(m->plan, &m->eo, &L->est);
-    if (!rc) rc = bb_estimator_stream(L->est, (void **)&L->stream);
     const size_t B = (size_t)opt.max_batch;
     if (!rc && hipMalloc(&L->d_in, B * m->fsz * 8) != hipSuccess)
         rc = set_error(BB_E_DEVMEM, "lane input buffer");
@@ -240,25 +249,69 @@ bool bb_server::complete_lanes()
                 r->pinned = nullptr;
             }
         }
+        if (!L->reqs.empty() && L->reqs[0]->t_start > 0) {
+            const double run = (t - L->reqs[0]->t_start) / (double)L->reqs.size();
+            Model *m = L->m;
+            m->svc_ms = m->svc_ms > 0 ? 0.8 * m->svc_ms + 0.2 * run : run;
+        }
         L->reqs.clear();
         L->busy = false;
         L->last_used = t;
+        if (L->slot >= 0) stream_busy[L->slot] = 0;
+        L->slot = -1;
         any = true;
     }
     return any;
 }
 
-// Starts batches while requests wait and lanes are free.  The oldest queued
-// request goes first; later requests of its geometry join its batch.
+// Scheduling constants: a field expected to take longer than kLongMs on a
+// lane is "long"; long fields may hold at most max_inflight - 1 lanes, so a
+// short request always finds a lane soon.  A batch holds as many queued
+// fields of its geometry as fit in about kBatchMs.
+constexpr double kLongMs = 2.0;
+constexpr double kBatchMs = 8.0;
+
+// Picks the request that starts next: the one of the geometry with the
+// shortest expected time (so short requests do not queue behind long ones),
+// except that a request waiting longer than max(50 ms, 8 times its expected
+// time) goes first, oldest first (no request waits forever).  Geometries not
+// yet timed count as shortest; ties keep arrival order.  Long requests are
+// skipped while `long_ok` is false.
+Req *pick(std::deque<Req *> &queue, double now, bool long_ok)
+{
+    Req *best = nullptr;
+    for (Req *r : queue) {
+        const double svc = r->m->svc_ms;
+        if (!long_ok && svc > kLongMs) continue;
+        if (now - r->t_submit > std::max(50.0, 8.0 * svc)) return r;   // oldest overdue
+        if (!best || svc < best->m->svc_ms) best = r;
+    }
+    return best;
+}
+
+// Starts batches while requests wait and lanes are free; later requests of
+// the picked request's geometry join its batch.
 bool bb_server::dispatch()
 {
     bool any = false;
     while (!queue.empty() && !paused) {
-        int busy = 0;
-        for (auto &L : lanes) busy += L->busy;
+        int busy = 0, busy_long = 0;
+        for (auto &L : lanes) {
+            busy += L->busy;
+            busy_long += L->busy && L->m->svc_ms > kLongMs;
+        }
         if (busy >= opt.max_inflight) break;
-        Req *r = queue.front();
+        // Long requests may not take the last free lane (with reserved CUs:
+        // never stream 0), so a short request always finds one soon.
+        bool long_ok = opt.max_inflight == 1 || busy_long < opt.max_inflight - 1;
+        if (reserved > 0) {
+            long_ok = false;
+            for (size_t k = nshort; k < streams.size(); ++k) long_ok = long_ok || !stream_busy[k];
+        }
+        Req *r = pick(queue, now_ms(), long_ok);
+        if (!r) break;   // only long requests wait, and the last lane is kept for short ones
         Model *m = r->m;
+        const bool is_long = m->svc_ms > kLongMs;
         Lane *lane = nullptr;
         for (auto &L : lanes)
             if (!L->busy && L->m == m) {
@@ -271,13 +324,17 @@ bool bb_server::dispatch()
             if (rc) {
                 fail(r, rc, rc == BB_E_DEVMEM ? "the request does not fit the HBM budget"
                                               : bb_last_error());
-                queue.pop_front();
+                queue.erase(std::find(queue.begin(), queue.end(), r));
                 any = true;
                 continue;
             }
         }
-        std::vector<Req *> batch;
-        for (auto it = queue.begin(); it != queue.end() && (int)batch.size() < opt.max_batch;) {
+        std::vector<Req *> batch{r};
+        queue.erase(std::find(queue.begin(), queue.end(), r));
+        const int cap = m->svc_ms > 0
+                            ? std::max(1, std::min(opt.max_batch, (int)(kBatchMs / m->svc_ms)))
+                            : opt.max_batch;
+        for (auto it = queue.begin(); it != queue.end() && (int)batch.size() < cap;) {
             if ((*it)->m == m) {
                 batch.push_back(*it);
                 it = queue.erase(it);
@@ -287,6 +344,19 @@ bool bb_server::dispatch()
         }
         const size_t fsz = m->fsz, nb = m->nb, B = (size_t)opt.max_batch, n = batch.size();
         int rc = BB_OK;
+        // Lend a free server stream (fewer busy lanes than streams, so one is
+        // free): short requests prefer the reserved streams, long ones never
+        // take them.
+        int slot = (reserved > 0 && is_long) ? nshort : 0;
+        while (slot < (int)streams.size() && stream_busy[slot]) ++slot;
+        if (slot == (int)streams.size()) {
+            rc = set_error(BB_E_DEVICE, "internal: no free stream");
+        } else {
+            lane->slot = slot;
+            stream_busy[slot] = 1;
+            lane->stream = streams[slot];
+            rc = bb_estimator_set_stream(lane->est, lane->stream);
+        }
         const double t = now_ms();
         for (size_t k = 0; k < n && !rc; ++k) {
             batch[k]->t_start = t;
@@ -304,8 +374,10 @@ bool bb_server::dispatch()
                                   hipMemcpyDeviceToHost, lane->stream) != hipSuccess)
             rc = set_error(BB_E_DEVICE, "output copy failed");
         if (rc) {
-            (void)hipStreamSynchronize(lane->stream);
+            if (lane->stream) (void)hipStreamSynchronize(lane->stream);
             (void)hipGetLastError();
+            if (lane->slot >= 0) stream_busy[lane->slot] = 0;
+            lane->slot = -1;
             for (Req *b : batch) {
                 fail(b, rc, bb_last_error());
                 if (b->pinned) {
@@ -369,6 +441,34 @@ int bb_server_create(const bb_server_options *o, bb_server **out)
     if (S->opt.max_batch == 0) S->opt.max_batch = 4;
     S->device = dev;
     S->budget = o->hbm_budget ? o->hbm_budget : freeb / 10 * 8;
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        ncu = 0;
+    const int want = o->reserved_cus < 0 ? 0 : o->reserved_cus > 0 ? o->reserved_cus : ncu / 8;
+    if (S->opt.max_inflight >= 2 && ncu >= 16 && want > 0 && want < ncu) {
+        S->reserved = want;
+        S->nshort = std::max(1, S->opt.max_inflight / 2);
+    }
+    for (int k = 0; k < S->opt.max_inflight; ++k) {
+        hipStream_t st = nullptr;
+        hipError_t e;
+        if (S->reserved > 0) {
+            std::vector<uint32_t> mask((ncu + 31) / 32, 0u);
+            for (int cu = 0; cu < ncu; ++cu)
+                if ((cu < S->reserved) == (k < S->nshort)) mask[cu / 32] |= 1u << (cu % 32);
+            e = hipExtStreamCreateWithCUMask(&st, (uint32_t)mask.size(), mask.data());
+        } else {
+            e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        }
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            for (hipStream_t x : S->streams) (void)hipStreamDestroy(x);
+            delete S;
+            return set_error(BB_E_DEVICE, "stream creation failed");
+        }
+        S->streams.push_back(st);
+        S->stream_busy.push_back(0);
+    }
     S->th = std::thread([S] { S->loop(); });
     *out = S;
     return BB_OK;
@@ -387,6 +487,7 @@ void bb_server_destroy(bb_server *S)
         if (L->busy) (void)bb_estimator_wait(L->est);
         S->free_lane(L.get());
     }
+    for (hipStream_t x : S->streams) (void)hipStreamDestroy(x);
     for (auto &kv : S->reqs) {
         if (kv.second->pinned) (void)hipHostFree(kv.second->pinned);
         delete kv.second;
@@ -394,32 +495,61 @@ void bb_server_destroy(bb_server *S)
     delete S;
 }
 
+namespace {
+
+// The model (plan and lane options) of a geometry, created on first use.
+int get_model(bb_server *S, const bb_geometry *g, Model **out)
+{
+    const std::string key = geometry_key(*g);
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        auto it = S->models.find(key);
+        if (it != S->models.end()) {
+            *out = it->second.get();
+            return BB_OK;
+        }
+    }
+    auto nm = std::make_unique<Model>();
+    int rc = bb_plan_create(g, &nm->plan);
+    if (rc) return rc;
+    bb_plan_stats st;
+    if ((rc = bb_plan_stats_get(nm->plan, &st))) return rc;
+    nm->fsz = (size_t)g->NN * g->NN * (g->NN + 2);
+    nm->nb = (size_t)st.nbins * BB_NPOLES;
+    std::memset(&nm->eo, 0, sizeof nm->eo);
+    nm->eo.precision = S->opt.precision;
+    nm->eo.max_batch = S->opt.max_batch;
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto ins = S->models.emplace(key, std::move(nm));
+    *out = ins.first->second.get();
+    return BB_OK;
+}
+
+}  // namespace
+
+int bb_server_prepare(bb_server *S, const bb_geometry *g, int nlanes)
+{
+    if (!S || !g) return set_error(BB_E_INVALID, "null argument");
+    Model *m = nullptr;
+    int rc = get_model(S, g, &m);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(S->mu);
+    int have = 0;
+    for (auto &L : S->lanes) have += L->m == m;
+    for (; have < std::min(nlanes, S->opt.max_inflight); ++have) {
+        Lane *L = nullptr;
+        if ((rc = S->make_lane(m, &L))) return rc == BB_E_DEVMEM ? set_error(rc, "lanes do not fit the HBM budget") : rc;
+    }
+    return BB_OK;
+}
+
 int bb_submit(bb_server *S, const bb_geometry *g, const double *field_r2c, int on_device,
               long long *ticket)
 {
     if (!S || !g || !field_r2c || !ticket) return set_error(BB_E_INVALID, "null argument");
-    const std::string key = geometry_key(*g);
     Model *m = nullptr;
-    {
-        std::lock_guard<std::mutex> lk(S->mu);
-        auto it = S->models.find(key);
-        if (it != S->models.end()) m = it->second.get();
-    }
-    if (!m) {
-        auto nm = std::make_unique<Model>();
-        int rc = bb_plan_create(g, &nm->plan);
-        if (rc) return rc;
-        bb_plan_stats st;
-        if ((rc = bb_plan_stats_get(nm->plan, &st))) return rc;
-        nm->fsz = (size_t)g->NN * g->NN * (g->NN + 2);
-        nm->nb = (size_t)st.nbins * BB_NPOLES;
-        std::memset(&nm->eo, 0, sizeof nm->eo);
-        nm->eo.precision = S->opt.precision;
-        nm->eo.max_batch = S->opt.max_batch;
-        std::lock_guard<std::mutex> lk(S->mu);
-        auto ins = S->models.emplace(key, std::move(nm));
-        m = ins.first->second.get();
-    }
+    int rc = get_model(S, g, &m);
+    if (rc) return rc;
     auto *r = new (std::nothrow) Req;
     if (!r) return set_error(BB_E_NOMEM, "out of host memory");
     r->m = m;

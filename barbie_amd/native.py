@@ -31,10 +31,11 @@ EXPORTED = [
     "bb_estimator_destroy", "bb_estimator_grid", "bb_estimator_levels",
     "bb_estimator_footprint", "bb_estimator_footprint_opts", "bb_estimate", "bb_estimate_device",
     "bb_estimate_batch", "bb_estimate_batch_device", "bb_estimate_batch_async",
-    "bb_estimator_query", "bb_estimator_wait", "bb_estimator_stream",
+    "bb_estimator_query", "bb_estimator_wait", "bb_estimator_stream", "bb_estimator_set_stream",
     "bb_estimator_contraction_sums", "bb_estimator_triangle_counts", "bb_estimator_field",
     "bb_estimator_timings", "bb_estimator_worklist", "bb_write_outputs",
     "bb_server_create", "bb_server_destroy", "bb_submit", "bb_poll", "bb_wait", "bb_server_pause",
+    "bb_server_prepare",
     "bb_server_stats",
 ]
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
@@ -42,7 +43,7 @@ REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 # bb_estimator_worklist / bb_plan_worklist entries, in order.
 WORKLIST_KEYS = ("items", "units", "tiles", "max_rings", "staged_rings", "staged_bytes", "grid",
                  "chunk", "waves", "wave_balance_x1000", "levels", "partial_tiles",
-                 "points_summed", "slots")
+                 "points_summed", "slots", "total_cost")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -74,7 +75,8 @@ class BBOptions(ctypes.Structure):
 
 class BBServerOptions(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_inflight", ctypes.c_int),
-                ("max_batch", ctypes.c_int), ("precision", ctypes.c_int), ("tune", ctypes.c_int)]
+                ("max_batch", ctypes.c_int), ("precision", ctypes.c_int), ("tune", ctypes.c_int),
+                ("reserved_cus", ctypes.c_int)]
 
 
 class BarbieError(RuntimeError):
@@ -121,6 +123,7 @@ def lib() -> ctypes.CDLL:
         L.bb_estimator_query.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
         L.bb_estimator_wait.argtypes = [ctypes.c_void_p]
         L.bb_estimator_stream.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
+        L.bb_estimator_set_stream.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.bb_server_create.argtypes = [ctypes.POINTER(BBServerOptions),
                                        ctypes.POINTER(ctypes.c_void_p)]
         L.bb_server_destroy.argtypes = [ctypes.c_void_p]
@@ -131,6 +134,7 @@ def lib() -> ctypes.CDLL:
         L.bb_wait.argtypes = [ctypes.c_void_p, ctypes.c_longlong, _dp, _dp, _dp, _dp]
         L.bb_server_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)]
         L.bb_server_pause.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.bb_server_prepare.argtypes = [ctypes.c_void_p, ctypes.POINTER(BBGeometry), ctypes.c_int]
         L.bb_estimator_contraction_sums.argtypes = [ctypes.c_void_p, _dp]
         L.bb_estimator_triangle_counts.argtypes = [ctypes.c_void_p, _dp]
         L.bb_estimator_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
@@ -414,9 +418,9 @@ class Server:
     request is done)."""
 
     def __init__(self, precision: str = "bf16x3", hbm_budget: int = 0, max_inflight: int = 0,
-                 max_batch: int = 0, tune: bool = False):
+                 max_batch: int = 0, tune: bool = False, reserved_cus: int = 0):
         o = BBServerOptions(int(hbm_budget), int(max_inflight), int(max_batch),
-                            PRECISIONS[precision], int(tune))
+                            PRECISIONS[precision], int(tune), int(reserved_cus))
         h = ctypes.c_void_p()
         _check(lib().bb_server_create(ctypes.byref(o), ctypes.byref(h)))
         self.handle = h
@@ -443,6 +447,11 @@ class Server:
             _check(lib().bb_submit(self.handle, ctypes.byref(g), _p(f), 0, ctypes.byref(t)))
         self._geoms[t.value] = geom
         return t.value
+
+    def prepare(self, geom: Geometry, nlanes: int = 1):
+        """Create lanes of a geometry ahead of its traffic."""
+        g = geom.to_c()
+        _check(lib().bb_server_prepare(self.handle, ctypes.byref(g), int(nlanes)))
 
     def pause(self, paused: bool = True):
         """Start no new batches while paused (a burst then batches together)."""

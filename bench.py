@@ -173,9 +173,16 @@ def serving_leg(B, torch, precision, n_sat=200, n_load=300, seed=7):
     geoms = {NN: geometry(B, NN) for NN in sizes}
     fields = {NN: B.make_field_device(NN, NRi=1) for NN in sizes}
     torch.cuda.synchronize()
-    srv = B.Server(precision, max_inflight=4, max_batch=4)
-    for NN in sizes:   # lanes and plans built before timing
+    srv = B.Server(precision, max_inflight=4, max_batch=4, tune=True)
+    t0 = time.perf_counter()
+    for NN in sizes:   # plans, autotuning and lanes before timing
+        srv.prepare(geoms[NN], 4)
         srv.wait(srv.submit(geoms[NN], fields[NN].data_ptr()))
+    setup_s = time.perf_counter() - t0
+    alone = {}
+    for NN in sizes:   # one request at a time: service time per size
+        lat = [srv.wait(srv.submit(geoms[NN], fields[NN].data_ptr()))[3][2] for _ in range(3)]
+        alone[str(NN)] = float(np.median(lat))
 
     def draw(n):
         return rng.choice(sizes, size=n, p=weights / weights.sum())
@@ -189,7 +196,8 @@ def serving_leg(B, torch, precision, n_sat=200, n_load=300, seed=7):
         srv.wait(t)
     sat = n_sat / (time.perf_counter() - t0)
     out = {"saturation_req_per_s": sat, "mix": "NN 128/256/384/512 weighted 4:3:2:1",
-           "max_inflight": 4, "max_batch": 4, "loads": {}}
+           "max_inflight": 4, "max_batch": 4, "setup_seconds": setup_s,
+           "alone_ms_by_NN": alone, "loads": {}}
     for load in (0.5, 0.8, 0.95):
         lam = load * sat
         mix = draw(n_load)
@@ -206,12 +214,18 @@ def serving_leg(B, torch, precision, n_sat=200, n_load=300, seed=7):
                     time.sleep(dt - 0.0004)
             tickets.append(srv.submit(geoms[NN], fields[NN].data_ptr()))
         lat = np.array([srv.wait(t)[3][2] for t in tickets])
+        by_nn = {str(NN): {"p50_ms": float(np.percentile(lat[mix == NN], 50)),
+                           "p99_ms": float(np.percentile(lat[mix == NN], 99))}
+                 for NN in sizes if np.any(mix == NN)}
         out["loads"][str(load)] = {"rate_req_per_s": lam, "p50_ms": float(np.percentile(lat, 50)),
                                    "p99_ms": float(np.percentile(lat, 99)),
-                                   "mean_ms": float(lat.mean())}
+                                   "mean_ms": float(lat.mean()), "by_NN": by_nn}
     st = srv.stats()
     out["batches"] = st["batches"]
     out["batched_fields"] = st["batched_fields"]
+    out["lanes_created"] = st["lanes_created"]
+    out["lanes_evicted"] = st["lanes_evicted"]
+    out["lane_bytes"] = st["bytes"]
     srv.close()
     return out
 

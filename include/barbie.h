@@ -157,7 +157,8 @@ int bb_estimator_grid(const bb_estimator *e, int *NG);
 int bb_estimator_levels(const bb_estimator *e, int *nlevels, int *grids);
 /* Autotuner (no counterpart in the reference): times an estimate for each
  * candidate layout -- waves per contraction workgroup (8, 4) x number of grid
- * levels (3 .. base->grid_levels, or 8) -- on the current device and returns
+ * levels (4, 5, 6, 8, those not above base->grid_levels when it is set) --
+ * on the current device and returns
  * in *best the fastest options (base with waves and grid_levels set) and its
  * time per estimate in *best_ms (may be NULL).  d_field: a device field to
  * time with, or NULL for zeros (the work does not depend on the values).
@@ -203,6 +204,10 @@ int bb_estimator_wait(bb_estimator *e);
 /* The estimator's HIP stream (hipStream_t), for callers that order their own
  * copies against it. */
 int bb_estimator_stream(const bb_estimator *e, void **stream);
+/* Runs the estimator's work on `stream` (hipStream_t, owned by the caller and
+ * valid while used) from now on; NULL returns to the estimator's own stream.
+ * Waits for a pending batch first. */
+int bb_estimator_set_stream(bb_estimator *e, void *stream);
 
 /* Diagnostics of the last estimate (first field of the last batch): per-contraction triple-product sums in
  * the reference's enumeration order (ncontr doubles, scaled to the NN grid),
@@ -223,11 +228,12 @@ int bb_estimator_field(bb_estimator *e, int level, int plane, int ring, double *
  * their level (points summed).  The environment variables
  * BARBIE_CONTRACT_WAVES (4 or 8) and BARBIE_GRID_LEVELS (overrides
  * grid_levels) are read when the estimator is created. */
-#define BB_WORKLIST_N 14
+#define BB_WORKLIST_N 15
 int bb_estimator_worklist(const bb_estimator *e, long long *out);
 /* The same work-list figures for an estimator with these options, computed
  * on the host without a device ([13]: tile window slots executed, i.e.
- * items x waves x windows per wave). */
+ * items x waves x windows per wave; [14]: the units' summed cost, tile
+ * windows of each item's busiest wave x chunks). */
 int bb_plan_worklist(const bb_plan *p, const bb_options *opt, long long *out);
 /* Device times of the last estimate in milliseconds: [0] plane gather,
  * [1] plane-ring FFTs, [2] contraction, [3] binning, [4] total.  A replayed
@@ -238,7 +244,11 @@ int bb_estimator_timings(bb_estimator *e, double *ms5);
  * process: barbie.c:58-397 started by run:1 / many_fields.sh).  One scheduler
  * thread owns the current HIP device; fields of any geometry are submitted,
  * batched with queued fields of the same geometry (up to max_batch) and run
- * on up to max_inflight estimators at once, each on its own stream.  An
+ * on up to max_inflight estimators at once, each on its own stream.  The
+ * geometry with the shortest measured time per field starts first, except
+ * that a request waiting longer than max(50 ms, 8 x its expected time) goes
+ * first, oldest first; fields expected to take over 2 ms hold at most
+ * max_inflight - 1 lanes, and a batch holds about 8 ms of fields.  An
  * estimator ("lane") is created only when its device bytes fit the HBM
  * budget; idle lanes of other geometries are evicted least recently used
  * first.  A field's bins are the same bits whatever else runs. */
@@ -249,6 +259,11 @@ typedef struct bb_server_options {
     int max_batch;           /* fields per lane launch (0: 4; at most 256) */
     bb_precision precision;
     int tune;                /* 1: run bb_tune once per geometry, before its first lane */
+    int reserved_cus;        /* CUs for short requests (expected under 2 ms): half of
+                                the max_inflight streams run on them and take short
+                                requests only, so that these never wait behind the
+                                workgroups of long estimates; the other streams run
+                                on the remaining CUs (0: 1/8 of the CUs; < 0: none) */
 } bb_server_options;
 enum { BB_REQ_QUEUED = 0, BB_REQ_RUNNING = 1, BB_REQ_DONE = 2, BB_REQ_FAILED = 3 };
 int bb_server_create(const bb_server_options *opt, bb_server **out);
@@ -259,6 +274,10 @@ void bb_server_destroy(bb_server *s);
  * is done.  *ticket identifies the request. */
 int bb_submit(bb_server *s, const bb_geometry *g, const double *field_r2c, int on_device,
               long long *ticket);
+/* Creates up to nlanes lanes (at most max_inflight) of a geometry now
+ * (autotuning it first when the server tunes), so that traffic does not pay
+ * for lane creation; the lanes count against the HBM budget like any other. */
+int bb_server_prepare(bb_server *s, const bb_geometry *g, int nlanes);
 /* paused != 0: start no new batches (running ones finish), so that a burst
  * of submissions queues up and batches together; 0 resumes. */
 int bb_server_pause(bb_server *s, int paused);
