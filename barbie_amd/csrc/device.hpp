@@ -42,8 +42,8 @@ struct FieldTask {
 struct LevelDesc {
     FftDesc fd;
     int tw_off;          // first twiddle of the level in the twiddle table
-    int rpb, cpb;        // transforms per block of the row and column passes (cpb: 2^k)
-    int cpb_log2;
+    int rpb, cpb;        // transforms per block of the row and column passes (2^k)
+    int rpb_log2, cpb_log2;
     int max_rows;        // widest row band of the level's fields
     long long off;       // first point of the level in the cache
     long long stride;    // points between fields (multiple of 8)

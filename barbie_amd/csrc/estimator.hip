@@ -157,6 +157,7 @@ void pair_blocks(std::vector<CItem> &cs, int pa, int pb, int pc, int flags,
 
 struct WorkLists {
     std::vector<WorkItem> items;
+    std::vector<uint8_t> kind;    // per item: 0 the layout's precision, 1 FP32 MFMA (mixed layouts)
     std::vector<TWin> tws;
     std::vector<int> item_work;   // tile windows of the busiest wave of each item
     int ntw = 0;                  // output tiles
@@ -198,7 +199,9 @@ struct Layout {
     long long cache_pts = 0;           // points of all levels' fields
     std::vector<WorkItem> items;
     std::vector<TWin> tws;
-    std::vector<WorkUnit> units;
+    std::vector<WorkUnit> units;       // the layout's precision
+    std::vector<WorkUnit> units_f;     // mixed BF16x3 layouts: items run by the FP32 kernel
+    int mix = 0;                       // percent of the cost given to the FP32 kernel
     long long ntiles = 0, part_tiles = 0;
     std::vector<int32_t> pos;          // contraction -> first partial sum
     std::vector<int16_t> nsp;          // contraction -> partial sums
@@ -324,6 +327,7 @@ int pack_items(std::vector<PairBlock> &blocks, const Level &L, bool split_b, int
         W.tw_slots += (long long)waves * work;
         W.item_work.push_back(work);
         W.items.push_back(it);
+        W.kind.push_back(0);
         a = b;
     }
     return BB_OK;
@@ -427,7 +431,9 @@ int check_lists(const Layout &L, size_t ncontr)
             seen[q] = 1;
         }
     std::vector<int> item_sp(L.items.size(), 0);
-    for (const WorkUnit &u : L.units) {
+    std::vector<WorkUnit> all(L.units);
+    all.insert(all.end(), L.units_f.begin(), L.units_f.end());
+    for (const WorkUnit &u : all) {
         if (u.item < 0 || (size_t)u.item >= L.items.size())
             return set_error(BB_E_INVALID, "internal: unit without an item");
         const WorkItem &it = L.items[u.item];
@@ -658,6 +664,8 @@ int build_layout(const PlanData &d, CPrec prec, const bb_options &opt, int waves
     // rings.
     WorkLists W;
     L.pos.assign(std::max<size_t>(d.contr.size(), 1), 0);
+    L.mix = prec == CPrec::BF16X3 ? std::max(0, std::min(opt.mix_fp32, 90)) : 0;
+    long long cost_all = 0, cost_f = 0;
     for (int l = 0; l < nl; ++l) {
         std::map<std::pair<int, int>, std::vector<CItem>> groups;
         for (size_t c = 0; c < d.contr.size(); ++c) {
@@ -672,15 +680,31 @@ int build_layout(const PlanData &d, CPrec prec, const bb_options &opt, int waves
             std::sort(key.begin(), key.end());
             by_set[key].push_back(kv.first);
         }
-        std::vector<PairBlock> blocks;
-        for (auto &kv : by_set)
+        // Mixed layouts: whole plane sets go to the FP32 kernel while its share
+        // of the cost (tile windows x points) stays within `mix` percent.
+        std::vector<PairBlock> blocks, blocks_f;
+        for (auto &kv : by_set) {
+            std::vector<PairBlock> sb;
             for (auto &gk : kv.second) {
                 const int p1 = gk.first, p2 = gk.second, p3 = -p1 - p2;
                 const int flags = (p2 < 0 ? TW_CONJ2 : 0) | (p3 < 0 ? TW_CONJ3 : 0);
-                pair_blocks(groups[gk], p1, std::abs(p2), std::abs(p3), flags, blocks);
+                pair_blocks(groups[gk], p1, std::abs(p2), std::abs(p3), flags, sb);
             }
+            long long c = 0;
+            for (const PairBlock &B : sb) c += B.nwin;
+            c *= L.lv[l].npts;
+            const bool to_f = L.mix > 0 && (cost_f + c) * 100 <= (long long)L.mix * (cost_all + c);
+            cost_all += c;
+            if (to_f) cost_f += c;
+            std::vector<PairBlock> &dst = to_f ? blocks_f : blocks;
+            for (PairBlock &B : sb) dst.push_back(std::move(B));
+        }
         int rc = pack_items(blocks, L.lv[l], prec == CPrec::BF16X3, tww, waves, W, L.pos);
         if (rc) return rc;
+        const size_t first_f = W.items.size();
+        rc = pack_items(blocks_f, L.lv[l], false, contract_tww(CPrec::F32), waves, W, L.pos);
+        if (rc) return rc;
+        for (size_t i = first_f; i < W.items.size(); ++i) W.kind[i] = 1;
     }
     L.ntiles = W.ntw;
     L.max_rows = W.max_rows;
@@ -690,9 +714,11 @@ int build_layout(const PlanData &d, CPrec prec, const bb_options &opt, int waves
         L.staged_bytes += (long long)it.nrows * it.npts * (long long)elem;   // padding excluded
     }
     for (size_t c = 0; c < d.contr.size(); ++c) L.useful_pts += L.lv[lvl[c]].npts;
-    int rc = make_units(W, L.chunk, opt.x_splits, L.units, L.pos, L.nsp, L.part_tiles,
+    std::vector<WorkUnit> units;
+    int rc = make_units(W, L.chunk, opt.x_splits, units, L.pos, L.nsp, L.part_tiles,
                         &L.total_cost);
     if (rc) return rc;
+    for (const WorkUnit &u : units) (W.kind[u.item] ? L.units_f : L.units).push_back(u);
     L.items.swap(W.items);
     L.tws.swap(W.tws);
     if ((rc = check_lists(L, d.contr.size()))) return rc;
@@ -741,7 +767,8 @@ int get_layout(const bb_plan *p, CPrec prec, const bb_options &opt,
     const int waves = layout_waves(opt), tww = layout_tww(prec), lev = default_levels(opt);
     const std::string key = std::to_string((int)prec) + "/" + std::to_string(opt.full_grid != 0) +
                             "/" + std::to_string(opt.x_splits) + "/" + std::to_string(lev) + "/" +
-                            std::to_string(waves) + "/" + std::to_string(tww);
+                            std::to_string(waves) + "/" + std::to_string(tww) + "/" +
+                            std::to_string(prec == CPrec::BF16X3 ? opt.mix_fp32 : 0);
     bb_plan *mp = const_cast<bb_plan *>(p);
     {
         std::lock_guard<std::mutex> g(mp->mu);
@@ -785,6 +812,9 @@ struct bb_estimator {
     WorkItem *d_items = nullptr;
     TWin *d_tws = nullptr;
     WorkUnit *d_units = nullptr;
+    WorkUnit *d_units_f = nullptr;    // mixed layouts: units of the FP32 kernel
+    hipStream_t aux = nullptr;        // mixed layouts: the FP32 kernel's stream
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     double *d_part = nullptr;         // partial tiles
     BinRec *d_recs = nullptr;
     int32_t *d_seg = nullptr;
@@ -809,7 +839,11 @@ struct bb_estimator {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
+        if (aux) (void)hipStreamSynchronize(aux);
         for (void *p : owned) (void)hipFree(p);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (aux) (void)hipStreamDestroy(aux);
         if (own_stream) (void)hipStreamDestroy(own_stream);
     }
 };
@@ -843,7 +877,9 @@ int level_desc(const Level &V, int tw_off, LevelDesc &D)
     D.rpb = fft_rows_per_block(V.NG);
     D.cpb = fft_cols_per_block(V.NG);
     while ((1 << D.cpb_log2) < D.cpb) ++D.cpb_log2;
-    if ((1 << D.cpb_log2) != D.cpb) return set_error(BB_E_INVALID, "internal: column batch");
+    while ((1 << D.rpb_log2) < D.rpb) ++D.rpb_log2;
+    if ((1 << D.cpb_log2) != D.cpb || (1 << D.rpb_log2) != D.rpb || (D.cpb * V.NG) % 8 != 0)
+        return set_error(BB_E_INVALID, "internal: FFT batch sizes");
     D.max_rows = V.max_rows;
     D.off = V.off;
     D.stride = V.stride;
@@ -1098,6 +1134,15 @@ int estimator_init(bb_estimator *E)
     if ((rc = dev_alloc(&E->d_items, L.items.size(), E->owned))) return rc;
     if ((rc = dev_alloc(&E->d_tws, L.tws.size(), E->owned))) return rc;
     if ((rc = dev_alloc(&E->d_units, L.units.size(), E->owned))) return rc;
+    if (!L.units_f.empty()) {
+        if ((rc = dev_alloc(&E->d_units_f, L.units_f.size(), E->owned))) return rc;
+        if ((rc = upload(E->d_units_f, L.units_f.data(), sizeof(WorkUnit) * L.units_f.size(),
+                         E->stream)))
+            return rc;
+        HIPCHK(hipStreamCreateWithFlags(&E->aux, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&E->ev_fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&E->ev_join, hipEventDisableTiming));
+    }
     if ((rc = dev_alloc(&E->d_part, nslots, E->owned))) return rc;
     if ((rc = dev_alloc(&E->d_recs, L.recs.size(), E->owned))) return rc;
     if ((rc = dev_alloc(&E->d_seg, L.seg.size(), E->owned))) return rc;
@@ -1138,8 +1183,18 @@ int enqueue_kernels(bb_estimator *E, int nf, const double *d_fields, double *d_b
     int rc = run_fields(E, E->d_planes, L.NGtop, E->d_nz, nf);
     if (rc) return rc;
     if (timed) HIPCHK(hipEventRecord(E->ev[2], s));
+    const bool mixed = !L.units_f.empty();
+    if (mixed) {   // the FP32 items run concurrently on the auxiliary stream
+        HIPCHK(hipEventRecord(E->ev_fork, s));
+        HIPCHK(hipStreamWaitEvent(E->aux, E->ev_fork, 0));
+        HIPCHK(launch_contract(CPrec::F32, L.waves, contract_tww(CPrec::F32), E->d_cache, nullptr,
+                               E->d_items, E->d_units_f, (int)L.units_f.size(), E->d_tws,
+                               E->d_part, E->bst, nf, E->aux));
+        HIPCHK(hipEventRecord(E->ev_join, E->aux));
+    }
     HIPCHK(launch_contract(E->cprec, L.waves, L.tww, E->d_cache, E->d_split, E->d_items,
                            E->d_units, (int)L.units.size(), E->d_tws, E->d_part, E->bst, nf, s));
+    if (mixed) HIPCHK(hipStreamWaitEvent(s, E->ev_join, 0));
     if (timed) HIPCHK(hipEventRecord(E->ev[3], s));
     HIPCHK(launch_binning(E->d_seg, E->d_recs, E->d_part, E->d_num, E->d_nz, E->d_avg, L.bp,
                           d_bs, d_norm, d_biana, E->bst, nf, s));
@@ -1279,6 +1334,8 @@ int check_options(const bb_options &o)
         return set_error(BB_E_INVALID, "max_batch must be in [0, 1024]");
     if (o.waves != 0 && o.waves != 4 && o.waves != 8)
         return set_error(BB_E_INVALID, "waves must be 0, 4 or 8");
+    if (o.mix_fp32 < 0 || o.mix_fp32 > 90)
+        return set_error(BB_E_INVALID, "mix_fp32 must be in [0, 90]");
     return BB_OK;
 }
 
@@ -1357,15 +1414,18 @@ int bb_tune(const bb_plan *p, const bb_options *base, const double *d_field, int
     }
     double best_t = 1e300;
     bb_options win = *base;
-    std::vector<int> levs;
-    for (int lev : {4, 5, 6, 8})
+    std::vector<int> levs, mixes{base->mix_fp32};
+    for (int lev : (cp == CPrec::BF16X3 ? std::vector<int>{4, 6} : std::vector<int>{4, 5, 6, 8}))
         if (base->grid_levels <= 0 || lev <= base->grid_levels) levs.push_back(lev);
     if (levs.empty()) levs.push_back(base->grid_levels);
+    if (cp == CPrec::BF16X3) mixes = {0, 30, 45};
     for (int waves : {8, 4})
-        for (int lev : levs) {
+        for (int lev : levs)
+            for (int mix : mixes) {
             bb_options o = *base;
             o.waves = waves;
             o.grid_levels = lev;
+            o.mix_fp32 = mix;
             o.max_batch = 1;
             bb_estimator *E = nullptr;
             if (bb_estimator_create_opts(p, &o, &E)) continue;   // e.g. out of memory: skip
@@ -1573,19 +1633,27 @@ int bb_estimator_field(bb_estimator *e, int level, int plane, int ring, double *
     const Level &V = L.lv[level];
     const int f = V.fid(plane, ring);
     if (f < 0) return set_error(BB_E_INVALID, "field not held at this level (unused by the plan)");
-    const size_t n = (size_t)V.npts;
+    const size_t n = (size_t)V.npts, N = (size_t)V.NG;
     const size_t at = (size_t)V.off + (size_t)f * (size_t)V.stride;
+    // the cache holds point (i, j) at j * N + i; out is row-major
     if (e->cprec == CPrec::F64) {
-        HIPCHK(hipMemcpy(out, (const double2 *)e->d_cache + at, n * sizeof(double2),
+        std::vector<double2> tmp(n);
+        HIPCHK(hipMemcpy(tmp.data(), (const double2 *)e->d_cache + at, n * sizeof(double2),
                          hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < N; ++i)
+            for (size_t j = 0; j < N; ++j) {
+                out[2 * (i * N + j)] = tmp[j * N + i].x;
+                out[2 * (i * N + j) + 1] = tmp[j * N + i].y;
+            }
     } else {
         std::vector<float2> tmp(n);
         HIPCHK(hipMemcpy(tmp.data(), (const float2 *)e->d_cache + at, n * sizeof(float2),
                          hipMemcpyDeviceToHost));
-        for (size_t k = 0; k < n; ++k) {
-            out[2 * k] = tmp[k].x;
-            out[2 * k + 1] = tmp[k].y;
-        }
+        for (size_t i = 0; i < N; ++i)
+            for (size_t j = 0; j < N; ++j) {
+                out[2 * (i * N + j)] = tmp[j * N + i].x;
+                out[2 * (i * N + j) + 1] = tmp[j * N + i].y;
+            }
     }
     return BB_OK;
 }
@@ -1593,7 +1661,7 @@ int bb_estimator_field(bb_estimator *e, int level, int plane, int ring, double *
 static void worklist_of(const Layout &L, long long *o)
 {
     o[0] = (long long)L.items.size();
-    o[1] = (long long)L.units.size();
+    o[1] = (long long)(L.units.size() + L.units_f.size());
     o[2] = L.ntiles;
     o[3] = L.max_rows;
     o[4] = L.staged_rows;

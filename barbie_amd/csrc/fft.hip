@@ -203,10 +203,14 @@ __device__ __forceinline__ float2 store_as<float2>(double2 v)
 __device__ __forceinline__ double2 load_as(double2 v) { return v; }
 __device__ __forceinline__ double2 load_as(float2 v) { return make_double2(v.x, v.y); }
 
+// Fields are stored column-major: point (i, j) of an N x N field sits at
+// x = j * N + i.  (The contraction sums over all points, so the order is
+// free; this one makes the column pass read and write contiguous memory.)
+//
 // Row pass over the tasks of every level, one block per entry of `blk`:
-// block {t, x} transforms rows [x * rpb, x * rpb + rpb) of task t's row band.
-// Row i of field fid is written in place of the level's cache, at
-// cache + off + fid * stride + i * N.  The plane values come from the
+// block {t, x} transforms rows [x * rpb, x * rpb + rpb) of task t's row band
+// and writes them into the field's cache slot (cache + off + fid * stride),
+// column-major, in place of the final field.  The plane values come from the
 // NP x NP plane grid (NP >= N): a mode (ia, ja) inside the ring satisfies
 // |ia|, |ja| < N/2 and sits at (ia mod NP, ja mod NP) there.
 template <typename C2>
@@ -271,24 +275,71 @@ __global__ __launch_bounds__(kPassThreads) void k_rowpass(const double2 *__restr
     if (nzflag && __any(nz) && (threadIdx.x & 63) == 0) atomicOr(&nzflag[T.zid], 1);
     __syncthreads();
     double2 *res = lds_fft(a, b, nb, L.fd, twl);
+    // Column-major store: consecutive threads take consecutive rows of one
+    // column (runs of nb points).
     C2 *dst = cache + L.off + (long long)T.fid * L.stride;
-    for (int idx = threadIdx.x; idx < nb * N; idx += blockDim.x) {
-        const int bt = (int)udiv((unsigned)idx, mN), j = idx - bt * N;
+    const int rsh = L.rpb_log2;
+    for (int idx = threadIdx.x; idx < rpb * N; idx += blockDim.x) {
+        const int bt = idx & (rpb - 1), j = idx >> rsh;
+        if (bt >= nb) continue;
         const int q = q0 + bt;
         const int i = (q <= T.rowband) ? q : N - nrows + q;
-        dst[(long long)i * N + j] = store_as<C2>(res[bt * N + j]);
+        dst[(long long)j * N + i] = store_as<C2>(res[bt * N + j]);
+    }
+}
+
+// 8 consecutive points of the result to the cache (and the bf16 operands),
+// as 16-byte stores; v[t] are FP64 values.
+template <typename C2>
+__device__ __forceinline__ void store8(C2 *dst, __bf16 *sp, const double2 *v);
+template <>
+__device__ __forceinline__ void store8<double2>(double2 *dst, __bf16 *, const double2 *v)
+{
+#pragma unroll
+    for (int t = 0; t < 8; ++t) dst[t] = v[t];
+}
+template <>
+__device__ __forceinline__ void store8<float2>(float2 *dst, __bf16 *sp, const double2 *v)
+{
+    float re[8], im[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        re[t] = (float)v[t].x;
+        im[t] = (float)v[t].y;
+    }
+    float4 *d4 = reinterpret_cast<float4 *>(dst);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) d4[t] = make_float4(re[2 * t], im[2 * t], re[2 * t + 1], im[2 * t + 1]);
+    if (sp) {
+        // [Re hi x8][Re lo x8][Im hi x8][Im lo x8], hi = bf16(v), lo = bf16(v - hi)
+        __bf16 g[32];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const __bf16 hr = (__bf16)re[t], hm = (__bf16)im[t];
+            g[t] = hr;
+            g[8 + t] = (__bf16)(re[t] - (float)hr);
+            g[16 + t] = hm;
+            g[24 + t] = (__bf16)(im[t] - (float)hm);
+        }
+        const uint4 *src = reinterpret_cast<const uint4 *>(g);
+        uint4 *s4 = reinterpret_cast<uint4 *>(sp);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) s4[t] = src[t];
     }
 }
 
 // Column pass over the tasks of every level, in place, one block per entry
 // of `blk`: block {t, x} transforms columns [x * cpb, x * cpb + cpb) of task
-// t's field along i.  Rows outside the
-// ring's band were not written by the row pass and are taken as zero.  With
-// `split` set (FP32 caches), the field is also written as bf16 hi/lo
-// operands at the same point offsets: for every 8 consecutive points, 32
-// bf16 values [Re hi x8][Re lo x8][Im hi x8][Im lo x8], with hi = bf16(v)
-// and lo = bf16(v - hi) of the FP32 value v.  Field strides are multiples of
-// 8, so no group of 8 straddles two fields.
+// t's field along i.  Those columns are the contiguous points
+// [j0 * N, (j0 + cpb) * N) of the field (column-major); j0 * N and cpb * N
+// are multiples of 8.
+// Rows outside the ring's band were not written by the row pass and are
+// taken as zero.  The result is written 8 points at a time; with `split`
+// set (FP32 caches) also as bf16 hi/lo operands at the same point offsets:
+// for every 8 consecutive points, 32 bf16 values [Re hi x8][Re lo x8]
+// [Im hi x8][Im lo x8], with hi = bf16(v) and lo = bf16(v - hi) of the FP32
+// value v.  Past the field's N*N points (up to a multiple of 8, inside the
+// padded stride) zeros are written.
 template <typename C2>
 __global__ __launch_bounds__(kPassThreads) void k_colpass(C2 *__restrict__ cache,
                                                  __bf16 *__restrict__ split,
@@ -309,54 +360,52 @@ __global__ __launch_bounds__(kPassThreads) void k_colpass(C2 *__restrict__ cache
     const int j0 = B.y * cpb;
     if (j0 >= N) return;
     const int nrows = min(N, 2 * T.rowband + 1);
-    const long long base = L.off + (long long)T.fid * L.stride;
+    const long long base = L.off + (long long)T.fid * L.stride + (long long)j0 * N;
+    const int npt = min(cpb, N - j0) * N;   // this block's points
     C2 *fld = cache + base;
     double2 *twl = lds, *a = lds + N, *b = a + cpb * N;
     for (int t = threadIdx.x; t < N; t += blockDim.x) twl[t] = tw_all[L.tw_off + t];
-    const int sh = L.cpb_log2;
-    // layout a[c * N + i]; all loads issued before the LDS stores
+    const unsigned long long mN = L.fd.mN;
+    // layout a[c * N + i] = point idx of the block; all loads issued before
+    // the LDS stores
     double2 vals[kPassPer];
 #pragma unroll
     for (int e = 0; e < kPassPer; ++e) {
         const int idx = threadIdx.x + e * kPassThreads;
         double2 v = make_double2(0.0, 0.0);
-        if (idx < cpb * N) {
-            const int i = idx >> sh, c = idx & (cpb - 1);
+        if (idx < npt) {
+            const int c = (int)udiv((unsigned)idx, mN), i = idx - c * N;
             const int ia = (i > N / 2) ? (i - N) : i;
             const bool inband = (ia <= T.rowband && -ia <= T.rowband) || nrows == N;
-            if (inband && j0 + c < N) v = load_as(fld[(long long)i * N + j0 + c]);
+            if (inband) v = load_as(fld[idx]);
         }
         vals[e] = v;
     }
 #pragma unroll
     for (int e = 0; e < kPassPer; ++e) {
         const int idx = threadIdx.x + e * kPassThreads;
-        if (idx < cpb * N) a[(idx & (cpb - 1)) * N + (idx >> sh)] = vals[e];
+        if (idx < cpb * N) a[idx] = vals[e];
     }
     __syncthreads();
     double2 *res = lds_fft(a, b, cpb, L.fd, twl);
-    for (int idx = threadIdx.x; idx < cpb * N; idx += blockDim.x) {
-        const int i = idx >> sh, c = idx & (cpb - 1);
-        if (j0 + c >= N) continue;
-        const long long x = (long long)i * N + j0 + c;
-        const C2 o = store_as<C2>(res[c * N + i]);
-        fld[x] = o;
-        if (split) {
-            const float vr = (float)o.x, vi = (float)o.y;
-            const __bf16 hr = (__bf16)vr, hi = (__bf16)vi;
-            __bf16 *g = split + base * 4 + (x >> 3) * 32 + (x & 7);
-            g[0] = hr;
-            g[8] = (__bf16)(vr - (float)hr);
-            g[16] = hi;
-            g[24] = (__bf16)(vi - (float)hi);
+    const int ngroups = (npt + 7) >> 3;
+    for (int g = threadIdx.x; g < ngroups; g += blockDim.x) {
+        double2 v[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int x = 8 * g + t;
+            v[t] = x < npt ? res[x] : make_double2(0.0, 0.0);
         }
+        store8<C2>(fld + 8 * g, split ? split + (base + 8 * g) * 4 : nullptr, v);
     }
 }
 
 // Transforms per block: about 1024 points of work for 256 threads, within
-// the 160 KiB of LDS up to N = 600 ((2 b + 1) N double2 values).
+// the 160 KiB of LDS up to N = 600 ((2 b + 1) N double2 values).  A column
+// block's points start and end on whole 8-point groups: N is even and the
+// column batch at least 4.
 int fft_rows_per_block(int N) { return N >= 256 ? 4 : N >= 128 ? 8 : N >= 64 ? 16 : 32; }
-int fft_cols_per_block(int N) { return N >= 512 ? 4 : N >= 128 ? 8 : N >= 64 ? 16 : 32; }
+int fft_cols_per_block(int N) { return N >= 320 ? 4 : N >= 128 ? 8 : N >= 64 ? 16 : 32; }
 
 template <typename C2>
 static hipError_t launch_fft(const double2 *planes, int NP, C2 *cache, __bf16 *split,
@@ -371,8 +420,8 @@ static hipError_t launch_fft(const double2 *planes, int NP, C2 *cache, __bf16 *s
         const int N = L.fd.N;
         if (L.stride < (long long)N * N || L.stride % 8 != 0 || L.rpb < 1 || L.cpb < 1)
             return hipErrorInvalidValue;
-        if (L.cpb != (1 << L.cpb_log2) || N >= 4096 || L.rpb * N > kPassThreads * kPassPer ||
-            L.cpb * N > kPassThreads * kPassPer)
+        if (L.cpb != (1 << L.cpb_log2) || L.rpb != (1 << L.rpb_log2) || (L.cpb * N) % 8 != 0 ||
+            N >= 4096 || L.rpb * N > kPassThreads * kPassPer || L.cpb * N > kPassThreads * kPassPer)
             return hipErrorInvalidValue;
         lds_r = std::max(lds_r, (2 * (size_t)L.rpb + 1) * N * sizeof(double2));
         lds_c = std::max(lds_c, (2 * (size_t)L.cpb + 1) * N * sizeof(double2));

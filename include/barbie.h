@@ -75,6 +75,10 @@ typedef struct bb_options {
                        times; the work lists are shared. */
     int waves;      /* waves per contraction workgroup: 4 or 8 (0: 8, or the
                        environment variable BARBIE_CONTRACT_WAVES) */
+    int mix_fp32;   /* BF16X3 only: percent (0-90) of the contraction's cost given
+                       to the FP32-MFMA kernel, which runs concurrently with the
+                       BF16x3 kernel on a second stream (the two load the matrix
+                       cores and the vector ALUs differently); 0: none */
     int use_graph;  /* 1: capture the launch sequence of a batch into a HIP
                        graph the first time it runs with a set of device
                        buffers, and replay the graph when the same buffers
@@ -157,9 +161,10 @@ int bb_estimator_grid(const bb_estimator *e, int *NG);
 int bb_estimator_levels(const bb_estimator *e, int *nlevels, int *grids);
 /* Autotuner (no counterpart in the reference): times an estimate for each
  * candidate layout -- waves per contraction workgroup (8, 4) x number of grid
- * levels (4, 5, 6, 8, those not above base->grid_levels when it is set) --
- * on the current device and returns
- * in *best the fastest options (base with waves and grid_levels set) and its
+ * levels (4, 5, 6, 8; BF16X3: 4, 6; those not above base->grid_levels when it
+ * is set) x, for BF16X3, mix_fp32 (0, 30, 45) -- on the current device and
+ * returns
+ * in *best the fastest options (base with waves, grid_levels, mix_fp32 set) and its
  * time per estimate in *best_ms (may be NULL).  d_field: a device field to
  * time with, or NULL for zeros (the work does not depend on the values).
  * Each candidate runs once untimed and `repeats` times timed (best kept).

@@ -244,3 +244,24 @@ def test_tuned_layout_matches_oracle():
     assert rel_err(bs, obs) <= 1e-10
     four = B.Estimator(plan, "fp64", waves=4, grid_levels=3).estimate(o["field"])
     assert rel_err(four[0], obs) <= 1e-10
+
+
+@pytest.mark.parametrize("gf", [medium, config1])
+def test_mixed_bf16_fp32_layout_within_tolerance(gf):
+    """BF16x3 with 40% of the cost on the concurrent FP32 kernel: within the
+    BF16x3 bar (1e-4 norm-wise per multipole) of the FP64 oracle; Num exact."""
+    o = oracle_run(gf, NRi=1)
+    g = o["g"]
+    plan = B.Plan(g)
+    est = B.Estimator(plan, "bf16x3", mix_fp32=40)
+    wl = est.worklist()
+    bs, nm, bi = est.estimate(o["field"])
+    obs, onm = (o[k].reshape(bs.shape) for k in ("bs", "nm"))
+    errs = [np.linalg.norm(bs[..., i] - obs[..., i]) / np.linalg.norm(obs[..., i]) for i in range(16)]
+    assert max(errs) <= 1e-4, errs
+    assert rel_err(nm, onm) <= 1e-12
+    S = est.contraction_sums()
+    So = o["contr"][:, 6]
+    assert np.array_equal(S == 0.0, So == 0.0)
+    # the mixed estimator's units include both kernels' units
+    assert wl["units"] > B.Estimator(plan, "bf16x3").worklist()["units"] // 4
