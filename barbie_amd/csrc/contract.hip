@@ -282,18 +282,38 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
+// The bf16 pairs (-1, 0) and (0, -1) in registers.  They are written by
+// v_mov_b32 with 32-bit literals: this compiler folds the constant (-1, 0)
+// into the inline constant -1.0, which v_dot2c_f32_bf16 reads as the FP32
+// bit pattern 0xbf800000, i.e. the pair (0, -1).
+struct NegUnit {
+    bf16x2 m0, m1;
+    __device__ __forceinline__ NegUnit()
+    {
+        unsigned u0, u1;
+        asm volatile("v_mov_b32 %0, 0xbf80" : "=v"(u0));
+        asm volatile("v_mov_b32 %0, 0xbf800000" : "=v"(u1));
+        m0 = __builtin_bit_cast(bf16x2, u0);
+        m1 = __builtin_bit_cast(bf16x2, u1);
+    }
+};
+
 // hi = bf16(x) (round to nearest even), lo = bf16(x - hi), eight values.
-// Each pair is converted once; the FP32 value of hi is read back from the
-// packed bits (low half << 16, high half masked).
-__device__ __forceinline__ void split8(const f32x8 &x, bf16x8 &hi, bf16x8 &lo)
+// Each pair is converted once (v_cvt_pk_bf16_f32).  The residual x - hi is
+// one bf16 dot product per value, dot2(hi pair, (-1, 0)) + x (or (0, -1)):
+// the product of a bf16 with -1 is exact and x - hi is representable in FP32
+// (hi is within a factor 2 of x), so the sum is exact.
+__device__ __forceinline__ void split8(const f32x8 &x, const NegUnit &nu, bf16x8 &hi, bf16x8 &lo)
 {
+    const bf16x2 m0 = nu.m0, m1 = nu.m1;
     u32x4 h, l;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const f32x2 v = {x[2 * q], x[2 * q + 1]};
-        const unsigned hb = __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
-        const f32x2 r = {v.x - __uint_as_float(hb << 16), v.y - __uint_as_float(hb & 0xffff0000u)};
-        h[q] = hb;
+        const bf16x2 hb = __builtin_convertvector(v, bf16x2);
+        const f32x2 r = {__builtin_amdgcn_fdot2_f32_bf16(hb, m0, v.x, false),
+                         __builtin_amdgcn_fdot2_f32_bf16(hb, m1, v.y, false)};
+        h[q] = __builtin_bit_cast(unsigned, hb);
         l[q] = __builtin_bit_cast(unsigned, __builtin_convertvector(r, bf16x2));
     }
     hi = __builtin_bit_cast(bf16x8, h);
@@ -306,8 +326,8 @@ __device__ __forceinline__ void split8(const f32x8 &x, bf16x8 &hi, bf16x8 &lo)
 // parameters, so that the signs become instruction modifiers.
 template <bool C2, bool C3>
 __device__ __forceinline__ void pair_operands(const unsigned char *rows, int oa1, int oa2,
-                                              bf16x8 &arh, bf16x8 &arl, bf16x8 &aih,
-                                              bf16x8 &ail)
+                                              const NegUnit &nu, bf16x8 &arh, bf16x8 &arl,
+                                              bf16x8 &aih, bf16x8 &ail)
 {
     const float4 *r1p = reinterpret_cast<const float4 *>(rows + oa1);
     const float4 *r2p = reinterpret_cast<const float4 *>(rows + oa2);
@@ -322,8 +342,8 @@ __device__ __forceinline__ void pair_operands(const unsigned char *rows, int oa1
         ai[2 * h] = C3 ? i0 : -i0;
         ai[2 * h + 1] = C3 ? i1 : -i1;
     }
-    split8(pr, arh, arl);
-    split8(ai, aih, ail);
+    split8(pr, nu, arh, arl);
+    split8(ai, nu, aih, ail);
 }
 
 // A staged ring of 32 points: raw rows hold float2 per point; split rows
@@ -362,6 +382,7 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_contract_bf16x3(
         fl[j] = uni(TW[j].flags);
     }
 
+    const NegUnit nu;
     f32x4 acc[TWW];
     double accd[TWW][4];
 #pragma unroll
@@ -396,10 +417,10 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_contract_bf16x3(
             const bf16x8 bil = *reinterpret_cast<const bf16x8 *>(bp + 48);
             if (fl[j] & TW_NEWA) {
                 switch (fl[j] & (TW_CONJ2 | TW_CONJ3)) {
-                case 0: pair_operands<false, false>(rows, oa1[j], oa2[j], arh, arl, aih, ail); break;
-                case TW_CONJ2: pair_operands<true, false>(rows, oa1[j], oa2[j], arh, arl, aih, ail); break;
-                case TW_CONJ3: pair_operands<false, true>(rows, oa1[j], oa2[j], arh, arl, aih, ail); break;
-                default: pair_operands<true, true>(rows, oa1[j], oa2[j], arh, arl, aih, ail); break;
+                case 0: pair_operands<false, false>(rows, oa1[j], oa2[j], nu, arh, arl, aih, ail); break;
+                case TW_CONJ2: pair_operands<true, false>(rows, oa1[j], oa2[j], nu, arh, arl, aih, ail); break;
+                case TW_CONJ3: pair_operands<false, true>(rows, oa1[j], oa2[j], nu, arh, arl, aih, ail); break;
+                default: pair_operands<true, true>(rows, oa1[j], oa2[j], nu, arh, arl, aih, ail); break;
                 }
             }
             acc[j] = mfma_bf16(arh, brh, acc[j]);

@@ -5,13 +5,15 @@
 // D_{p,r}(x) = sum_{k in ring r} Delta(kx, ky, kz=p) exp(+2 pi i k.x / NN),
 // unnormalised (FFTW_BACKWARD), with the DC mode of plane 0 zeroed.
 //
-// Two passes per field, each a batch of 1-D Stockham FFTs in LDS (FP64),
-// one launch each for the fields of every grid level:
+// Two passes per field, each a batch of 1-D Stockham FFTs in LDS (FP64 for
+// FP64 caches, FP32 for FP32 caches), one launch each for the fields of
+// every grid level:
 //  - row pass: for each row i that can hold masked modes, gather the plane
 //    row, apply the ring mask (integer radius bounds), transform along j,
 //    write the row into the field's cache slot;
 //  - column pass: transform along i in place (rows outside the ring's band
-//    are zero), write the field to the cache (FP64, or rounded to FP32).
+//    are zero), write the field to the cache (and, for FP32 caches, the
+//    bf16 hi/lo operands).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,73 +23,96 @@
 
 namespace bb {
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+// Complex arithmetic on double2 (FP64 caches) or float2 (FP32 caches).
+template <typename V2>
+__device__ __forceinline__ V2 mk(decltype(V2::x) x, decltype(V2::x) y)
 {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    V2 r;
+    r.x = x;
+    r.y = y;
+    return r;
 }
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+template <typename V2>
+__device__ __forceinline__ V2 cmul(V2 a, V2 b)
+{
+    return mk<V2>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename V2>
+__device__ __forceinline__ V2 cadd(V2 a, V2 b) { return mk<V2>(a.x + b.x, a.y + b.y); }
+template <typename V2>
+__device__ __forceinline__ V2 csub(V2 a, V2 b) { return mk<V2>(a.x - b.x, a.y - b.y); }
 // multiply by +i
-__device__ __forceinline__ double2 cmuli(double2 a) { return make_double2(-a.y, a.x); }
+template <typename V2>
+__device__ __forceinline__ V2 cmuli(V2 a) { return mk<V2>(-a.y, a.x); }
 
 // Length-R DFT with exp(+2 pi i q s / R), in place, for R = 2, 3, 4, 5.
-template <int R>
-__device__ __forceinline__ void butterfly(double2 *v);
-template <>
-__device__ __forceinline__ void butterfly<4>(double2 *v)
-{
-    double2 a = v[0], b = v[1], c = v[2], d = v[3];
-    double2 s0 = cadd(a, c), d0 = csub(a, c), s1 = cadd(b, d), d1 = cmuli(csub(b, d));
-    v[0] = cadd(s0, s1);
-    v[1] = cadd(d0, d1);
-    v[2] = csub(s0, s1);
-    v[3] = csub(d0, d1);
-}
-template <>
-__device__ __forceinline__ void butterfly<2>(double2 *v)
-{
-    double2 a = v[0], b = v[1];
-    v[0] = cadd(a, b);
-    v[1] = csub(a, b);
-}
-template <>
-__device__ __forceinline__ void butterfly<3>(double2 *v)
-{
-    const double h = 0.86602540378443864676;  // sqrt(3)/2
-    double2 a = v[0], t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
-    double2 m = make_double2(a.x - 0.5 * t1.x, a.y - 0.5 * t1.y);
-    double2 j = make_double2(-h * t2.y, h * t2.x);  // i*h*t2
-    v[0] = cadd(a, t1);
-    v[1] = cadd(m, j);
-    v[2] = csub(m, j);
-}
-template <>
-__device__ __forceinline__ void butterfly<5>(double2 *v)
-{
-    const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
-    const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
-    double2 a = v[0];
-    double2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
-    double2 u1 = csub(v[1], v[4]), u2 = csub(v[2], v[3]);
-    double2 m1 = make_double2(a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y);
-    double2 m2 = make_double2(a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y);
-    // i*(s1*u1 + s2*u2) and i*(s2*u1 - s1*u2)
-    double2 n1 = make_double2(-(s1 * u1.y + s2 * u2.y), s1 * u1.x + s2 * u2.x);
-    double2 n2 = make_double2(-(s2 * u1.y - s1 * u2.y), s2 * u1.x - s1 * u2.x);
-    v[0] = cadd(a, cadd(t1, t2));
-    v[1] = cadd(m1, n1);
-    v[4] = csub(m1, n1);
-    v[2] = cadd(m2, n2);
-    v[3] = csub(m2, n2);
-}
+template <int R, typename V2>
+struct Butterfly;
+template <typename V2>
+struct Butterfly<4, V2> {
+    static __device__ __forceinline__ void run(V2 *v)
+    {
+        V2 a = v[0], b = v[1], c = v[2], d = v[3];
+        V2 s0 = cadd(a, c), d0 = csub(a, c), s1 = cadd(b, d), d1 = cmuli(csub(b, d));
+        v[0] = cadd(s0, s1);
+        v[1] = cadd(d0, d1);
+        v[2] = csub(s0, s1);
+        v[3] = csub(d0, d1);
+    }
+};
+template <typename V2>
+struct Butterfly<2, V2> {
+    static __device__ __forceinline__ void run(V2 *v)
+    {
+        V2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+template <typename V2>
+struct Butterfly<3, V2> {
+    static __device__ __forceinline__ void run(V2 *v)
+    {
+        using R = decltype(V2::x);
+        const R h = (R)0.86602540378443864676;  // sqrt(3)/2
+        V2 a = v[0], t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+        V2 m = mk<V2>(a.x - (R)0.5 * t1.x, a.y - (R)0.5 * t1.y);
+        V2 j = mk<V2>(-h * t2.y, h * t2.x);  // i*h*t2
+        v[0] = cadd(a, t1);
+        v[1] = cadd(m, j);
+        v[2] = csub(m, j);
+    }
+};
+template <typename V2>
+struct Butterfly<5, V2> {
+    static __device__ __forceinline__ void run(V2 *v)
+    {
+        using R = decltype(V2::x);
+        const R c1 = (R)0.30901699437494742410, c2 = (R)-0.80901699437494742410;
+        const R s1 = (R)0.95105651629515357212, s2 = (R)0.58778525229247312917;
+        V2 a = v[0];
+        V2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+        V2 u1 = csub(v[1], v[4]), u2 = csub(v[2], v[3]);
+        V2 m1 = mk<V2>(a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y);
+        V2 m2 = mk<V2>(a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y);
+        // i*(s1*u1 + s2*u2) and i*(s2*u1 - s1*u2)
+        V2 n1 = mk<V2>(-(s1 * u1.y + s2 * u2.y), s1 * u1.x + s2 * u2.x);
+        V2 n2 = mk<V2>(-(s2 * u1.y - s1 * u2.y), s2 * u1.x - s1 * u2.x);
+        v[0] = cadd(a, cadd(t1, t2));
+        v[1] = cadd(m1, n1);
+        v[4] = csub(m1, n1);
+        v[2] = cadd(m2, n2);
+        v[3] = csub(m2, n2);
+    }
+};
 
 // One Stockham stage of radix R over nb transforms of length N: reads
 // x[i + t N/R], twiddles by W_{pR}^{t k} (k = i mod p), writes
 // y[(i-k) R + k + t p].
-template <int R>
-__device__ __forceinline__ void fft_stage(const double2 *a, double2 *b, int nb, int N, int p,
+template <int R, typename V2>
+__device__ __forceinline__ void fft_stage(const V2 *a, V2 *b, int nb, int N, int p,
                                           unsigned long long mNR, unsigned long long mP,
-                                          const double2 *tw)
+                                          const V2 *tw)
 {
     const unsigned NR = (unsigned)(N / R);
     const unsigned tstep = (unsigned)(N / (p * R));
@@ -96,16 +121,16 @@ __device__ __forceinline__ void fft_stage(const double2 *a, double2 *b, int nb, 
         const unsigned bt = udiv(idx, mNR);
         const unsigned i = idx - bt * NR;
         const unsigned k = i - udiv(i, mP) * (unsigned)p;
-        const double2 *x = a + bt * (unsigned)N;
-        double2 *y = b + bt * (unsigned)N;
-        double2 v[R];
+        const V2 *x = a + bt * (unsigned)N;
+        V2 *y = b + bt * (unsigned)N;
+        V2 v[R];
 #pragma unroll
         for (int t = 0; t < R; ++t) v[t] = x[i + t * NR];
         if (k > 0) {
 #pragma unroll
             for (int t = 1; t < R; ++t) v[t] = cmul(v[t], tw[t * k * tstep]);
         }
-        butterfly<R>(v);
+        Butterfly<R, V2>::run(v);
         const unsigned j = (i - k) * R + k;
 #pragma unroll
         for (int t = 0; t < R; ++t) y[j + t * (unsigned)p] = v[t];
@@ -115,7 +140,8 @@ __device__ __forceinline__ void fft_stage(const double2 *a, double2 *b, int nb, 
 // nb transforms of length N stored back to back in a (LDS); b is scratch of
 // the same size, tw the N twiddles (LDS).  Returns the buffer holding the
 // result (self-sorting Stockham, radices in fd.radix order).
-__device__ double2 *lds_fft(double2 *a, double2 *b, int nb, const FftDesc &fd, const double2 *tw)
+template <typename V2>
+__device__ V2 *lds_fft(V2 *a, V2 *b, int nb, const FftDesc &fd, const V2 *tw)
 {
     const int N = fd.N;
     int p = 1;
@@ -129,7 +155,7 @@ __device__ double2 *lds_fft(double2 *a, double2 *b, int nb, const FftDesc &fd, c
         default: fft_stage<5>(a, b, nb, N, p, mNR, mP, tw); break;
         }
         __syncthreads();
-        double2 *tmp = a; a = b; b = tmp;
+        V2 *tmp = a; a = b; b = tmp;
         p *= R;
     }
     return a;
@@ -189,19 +215,17 @@ hipError_t launch_gather(const double2 *fields, double2 *planes, int NN, int NG,
 constexpr int kPassThreads = 256;
 constexpr int kPassPer = 10;
 
-// Precision of the cache the passes write: FP64, or FP32 (the column pass
-// still transforms in FP64; the row-pass output is rounded to FP32 once).
-template <typename C2>
-__device__ __forceinline__ C2 store_as(double2 v);
+// The passes transform in the cache's precision: FP64 for FP64 caches, FP32
+// for FP32 caches (the FP32 paths' error budget is far above FP32 rounding).
+template <typename V2>
+__device__ __forceinline__ V2 cvt2(double2 v);
 template <>
-__device__ __forceinline__ double2 store_as<double2>(double2 v) { return v; }
+__device__ __forceinline__ double2 cvt2<double2>(double2 v) { return v; }
 template <>
-__device__ __forceinline__ float2 store_as<float2>(double2 v)
+__device__ __forceinline__ float2 cvt2<float2>(double2 v)
 {
     return make_float2((float)v.x, (float)v.y);
 }
-__device__ __forceinline__ double2 load_as(double2 v) { return v; }
-__device__ __forceinline__ double2 load_as(float2 v) { return make_double2(v.x, v.y); }
 
 // Fields are stored column-major: point (i, j) of an N x N field sits at
 // x = j * N + i.  (The contraction sums over all points, so the order is
@@ -222,7 +246,8 @@ __global__ __launch_bounds__(kPassThreads) void k_rowpass(const double2 *__restr
                                                  const double2 *__restrict__ tw_all,
                                                  int *__restrict__ nzflag, BatchStrides bs)
 {
-    extern __shared__ double2 lds[];
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    C2 *lds = reinterpret_cast<C2 *>(lds_raw);
     const int2 B = blk[blockIdx.x];
     const int fb = blockIdx.y;   // field of the batch
     planes += fb * bs.planes;
@@ -235,8 +260,8 @@ __global__ __launch_bounds__(kPassThreads) void k_rowpass(const double2 *__restr
     const int q0 = B.y * rpb;
     if (q0 >= nrows) return;
     const int nb = min(rpb, nrows - q0);
-    double2 *twl = lds, *a = lds + N, *b = a + rpb * N;
-    for (int t = threadIdx.x; t < N; t += blockDim.x) twl[t] = tw_all[L.tw_off + t];
+    C2 *twl = lds, *a = lds + N, *b = a + rpb * N;
+    for (int t = threadIdx.x; t < N; t += blockDim.x) twl[t] = cvt2<C2>(tw_all[L.tw_off + t]);
     const unsigned long long mN = L.fd.mN;
     // All of this thread's loads are issued before any LDS store, so they
     // are in flight together (at most kPassPer per thread).
@@ -269,12 +294,12 @@ __global__ __launch_bounds__(kPassThreads) void k_rowpass(const double2 *__restr
         const int idx = threadIdx.x + e * kPassThreads;
         if (idx < nb * N) {
             nz |= (vals[e].x != 0.0 || vals[e].y != 0.0);
-            a[idx] = vals[e];   // a[bt * N + j] with idx = bt * N + j
+            a[idx] = cvt2<C2>(vals[e]);   // a[bt * N + j] with idx = bt * N + j
         }
     }
     if (nzflag && __any(nz) && (threadIdx.x & 63) == 0) atomicOr(&nzflag[T.zid], 1);
     __syncthreads();
-    double2 *res = lds_fft(a, b, nb, L.fd, twl);
+    C2 *res = lds_fft(a, b, nb, L.fd, twl);
     // Column-major store: consecutive threads take consecutive rows of one
     // column (runs of nb points).
     C2 *dst = cache + L.off + (long long)T.fid * L.stride;
@@ -284,28 +309,24 @@ __global__ __launch_bounds__(kPassThreads) void k_rowpass(const double2 *__restr
         if (bt >= nb) continue;
         const int q = q0 + bt;
         const int i = (q <= T.rowband) ? q : N - nrows + q;
-        dst[(long long)j * N + i] = store_as<C2>(res[bt * N + j]);
+        dst[(long long)j * N + i] = res[bt * N + j];
     }
 }
 
 // 8 consecutive points of the result to the cache (and the bf16 operands),
-// as 16-byte stores; v[t] are FP64 values.
-template <typename C2>
-__device__ __forceinline__ void store8(C2 *dst, __bf16 *sp, const double2 *v);
-template <>
-__device__ __forceinline__ void store8<double2>(double2 *dst, __bf16 *, const double2 *v)
+// as 16-byte stores.
+__device__ __forceinline__ void store8(double2 *dst, __bf16 *, const double2 *v)
 {
 #pragma unroll
     for (int t = 0; t < 8; ++t) dst[t] = v[t];
 }
-template <>
-__device__ __forceinline__ void store8<float2>(float2 *dst, __bf16 *sp, const double2 *v)
+__device__ __forceinline__ void store8(float2 *dst, __bf16 *sp, const float2 *v)
 {
     float re[8], im[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
-        re[t] = (float)v[t].x;
-        im[t] = (float)v[t].y;
+        re[t] = v[t].x;
+        im[t] = v[t].y;
     }
     float4 *d4 = reinterpret_cast<float4 *>(dst);
 #pragma unroll
@@ -349,7 +370,8 @@ __global__ __launch_bounds__(kPassThreads) void k_colpass(C2 *__restrict__ cache
                                                  const double2 *__restrict__ tw_all,
                                                  BatchStrides bs)
 {
-    extern __shared__ double2 lds[];
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    C2 *lds = reinterpret_cast<C2 *>(lds_raw);
     const int2 B = blk[blockIdx.x];
     const int fb = blockIdx.y;   // field of the batch
     cache += fb * bs.cache;
@@ -363,21 +385,21 @@ __global__ __launch_bounds__(kPassThreads) void k_colpass(C2 *__restrict__ cache
     const long long base = L.off + (long long)T.fid * L.stride + (long long)j0 * N;
     const int npt = min(cpb, N - j0) * N;   // this block's points
     C2 *fld = cache + base;
-    double2 *twl = lds, *a = lds + N, *b = a + cpb * N;
-    for (int t = threadIdx.x; t < N; t += blockDim.x) twl[t] = tw_all[L.tw_off + t];
+    C2 *twl = lds, *a = lds + N, *b = a + cpb * N;
+    for (int t = threadIdx.x; t < N; t += blockDim.x) twl[t] = cvt2<C2>(tw_all[L.tw_off + t]);
     const unsigned long long mN = L.fd.mN;
     // layout a[c * N + i] = point idx of the block; all loads issued before
     // the LDS stores
-    double2 vals[kPassPer];
+    C2 vals[kPassPer];
 #pragma unroll
     for (int e = 0; e < kPassPer; ++e) {
         const int idx = threadIdx.x + e * kPassThreads;
-        double2 v = make_double2(0.0, 0.0);
+        C2 v = mk<C2>(0, 0);
         if (idx < npt) {
             const int c = (int)udiv((unsigned)idx, mN), i = idx - c * N;
             const int ia = (i > N / 2) ? (i - N) : i;
             const bool inband = (ia <= T.rowband && -ia <= T.rowband) || nrows == N;
-            if (inband) v = load_as(fld[idx]);
+            if (inband) v = fld[idx];
         }
         vals[e] = v;
     }
@@ -387,21 +409,22 @@ __global__ __launch_bounds__(kPassThreads) void k_colpass(C2 *__restrict__ cache
         if (idx < cpb * N) a[idx] = vals[e];
     }
     __syncthreads();
-    double2 *res = lds_fft(a, b, cpb, L.fd, twl);
+    C2 *res = lds_fft(a, b, cpb, L.fd, twl);
     const int ngroups = (npt + 7) >> 3;
     for (int g = threadIdx.x; g < ngroups; g += blockDim.x) {
-        double2 v[8];
+        C2 v[8];
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
             const int x = 8 * g + t;
-            v[t] = x < npt ? res[x] : make_double2(0.0, 0.0);
+            v[t] = x < npt ? res[x] : mk<C2>(0, 0);
         }
-        store8<C2>(fld + 8 * g, split ? split + (base + 8 * g) * 4 : nullptr, v);
+        store8(fld + 8 * g, split ? split + (base + 8 * g) * 4 : nullptr, v);
     }
 }
 
 // Transforms per block: about 1024 points of work for 256 threads, within
-// the 160 KiB of LDS up to N = 600 ((2 b + 1) N double2 values).  A column
+// the 160 KiB of LDS up to N = 600 ((2 b + 1) N values of the cache's
+// type).  A column
 // block's points start and end on whole 8-point groups: N is even and the
 // column batch at least 4.
 int fft_rows_per_block(int N) { return N >= 256 ? 4 : N >= 128 ? 8 : N >= 64 ? 16 : 32; }
@@ -423,8 +446,8 @@ static hipError_t launch_fft(const double2 *planes, int NP, C2 *cache, __bf16 *s
         if (L.cpb != (1 << L.cpb_log2) || L.rpb != (1 << L.rpb_log2) || (L.cpb * N) % 8 != 0 ||
             N >= 4096 || L.rpb * N > kPassThreads * kPassPer || L.cpb * N > kPassThreads * kPassPer)
             return hipErrorInvalidValue;
-        lds_r = std::max(lds_r, (2 * (size_t)L.rpb + 1) * N * sizeof(double2));
-        lds_c = std::max(lds_c, (2 * (size_t)L.cpb + 1) * N * sizeof(double2));
+        lds_r = std::max(lds_r, (2 * (size_t)L.rpb + 1) * N * sizeof(C2));
+        lds_c = std::max(lds_c, (2 * (size_t)L.cpb + 1) * N * sizeof(C2));
     }
     if (fb.nrow > 0) {
         (void)hipFuncSetAttribute((const void *)k_rowpass<C2>,

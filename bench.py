@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--no-accuracy", action="store_true", help="skip the fNL 0.1 / 1.0 fields")
     ap.add_argument("--no-tune", action="store_true",
                     help="use the default layout instead of the autotuned one (bb_tune)")
+    ap.add_argument("--layout", default="",
+                    help="fixed layout options for --precision instead of tuning, e.g. "
+                         "'grid_levels=4,waves=4,mix_fp32=0' (as bb_tune reports them)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a reference point for "
                          "HBM byte counters)")
@@ -298,7 +301,10 @@ def main():
         del src, dst
     tuned = {}
     for prec in ([a.precision] + ([] if a.skip_fp64 or a.precision == "fp64" else ["fp64"])):
-        if a.no_tune:
+        if a.layout and prec == a.precision:
+            kv = dict(x.split("=") for x in a.layout.split(","))
+            tuned[prec] = ({k: int(v) for k, v in kv.items()}, None, 0.0)
+        elif a.no_tune:
             tuned[prec] = ({}, None, 0.0)
         else:
             t0 = time.perf_counter()
