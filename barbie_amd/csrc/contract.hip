@@ -117,6 +117,92 @@ struct Stager {
     }
 };
 
+// Direct-to-LDS staging (global_load_lds_dwordx4), double-buffered, for the
+// BF16x3 kernel.  A chunk buffer holds the item's rings in groups of 16;
+// 16-byte piece k (bytes 16k..16k+15 of the ring's 256-byte chunk) of ring r
+// sits at
+//     ((r / 16) * 4 + k / 4) * 1024 + (k % 4) * 256 + (r % 16) * 16,
+// so one load instruction fills the 1 KiB block of a (group, quarter) pair:
+// lane l brings piece 4q + l/16 of ring 16 G + l%16.  Lanes that read one
+// piece of 16 different rings (mod 16) hit 64 distinct banks, and each lane's
+// 4 pieces of a quarter are 256 bytes apart (immediate offsets).  Rings past
+// the item's last are loaded from its last ring and never read.
+constexpr int kDmaGroups = 2;   // 16-ring groups per wave (rowcap / 16 / waves, rounded up)
+
+__host__ __device__ constexpr int dma_buf_stride(int nw) { return nw == 4 ? 32768 : 65536; }
+__host__ __device__ constexpr int dma_lds_bytes(int nw)
+{
+    return 2 * dma_buf_stride(nw) + kTableBytes;
+}
+
+static_assert((rowcap_for(4) + 15) / 16 <= kDmaGroups * 4 &&
+                  (rowcap_for(8) + 15) / 16 <= kDmaGroups * 8,
+              "every 16-ring group of an item has a wave");
+static_assert((rowcap_for(4) + 15) / 16 * 4096 <= dma_buf_stride(4) &&
+                  (rowcap_for(8) + 15) / 16 * 4096 <= dma_buf_stride(8),
+              "a chunk buffer holds the item's rings");
+
+__device__ __forceinline__ int dma_off(int r, int k)
+{
+    return ((r >> 4) * 4 + (k >> 2)) * 1024 + (k & 3) * 256 + (r & 15) * 16;
+}
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+template <int NW>
+struct DmaStager {
+    // This lane's source per group of the wave and quarter, for the chunk of
+    // the last issue.  They stay live (and are advanced only after the wait
+    // for the loads that read them), so no load in flight has its address
+    // registers reused.
+    const char *src[kDmaGroups][4];
+    int ngroups;   // 16-ring groups of the item
+
+    // Groups G = wave, wave + NW, ...: lane l serves ring 16 G + l%16 and
+    // pieces 4q + l/16.
+    __device__ __forceinline__ void init(const char *cache, const char *split, const int *row_slot,
+                                         const int *row_split, int nrows, long long fstride,
+                                         long long chunk_off, int wave, int lane)
+    {
+        ngroups = (nrows + 15) >> 4;
+#pragma unroll
+        for (int e = 0; e < kDmaGroups; ++e) {
+            const int G = wave + e * NW;
+            const int r = min(16 * G + (lane & 15), nrows - 1);
+            const char *b = (row_split[r] ? split : cache) + (long long)row_slot[r] * fstride * 8 +
+                            chunk_off + (lane >> 4) * 16;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) src[e][q] = b + 64 * q;
+        }
+    }
+    __device__ __forceinline__ void advance()
+    {
+#pragma unroll
+        for (int e = 0; e < kDmaGroups; ++e)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) src[e][q] += kRowBytes;
+    }
+    // Issues the loads of the current chunk into buffer `buf`.
+    __device__ __forceinline__ void issue(unsigned char *buf, int wave)
+    {
+#pragma unroll
+        for (int e = 0; e < kDmaGroups; ++e) {
+            const int G = wave + e * NW;
+            if (G < ngroups) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    __builtin_amdgcn_global_load_lds((const void *)src[e][q],
+                                                     (lds_void *)(buf + (G * 4 + q) * 1024), 16, 0,
+                                                     0);
+            }
+        }
+    }
+};
+
+// Waits for this wave's outstanding global loads (the LDS-direct ones
+// included); the other counters are left alone.
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // Flips the sign of x when the mask's top bit is set.
 __device__ __forceinline__ double flip(double x, unsigned mask)
 {
@@ -329,12 +415,11 @@ __device__ __forceinline__ void pair_operands(const unsigned char *rows, int oa1
                                               const NegUnit &nu, bf16x8 &arh, bf16x8 &arl,
                                               bf16x8 &aih, bf16x8 &ail)
 {
-    const float4 *r1p = reinterpret_cast<const float4 *>(rows + oa1);
-    const float4 *r2p = reinterpret_cast<const float4 *>(rows + oa2);
     f32x8 pr, ai;
 #pragma unroll
-    for (int h = 0; h < 4; ++h) {   // two points per float4
-        const float4 a = r1p[h], b = r2p[h];
+    for (int h = 0; h < 4; ++h) {   // two points per float4; pieces 256 bytes apart
+        const float4 a = *reinterpret_cast<const float4 *>(rows + oa1 + 256 * h);
+        const float4 b = *reinterpret_cast<const float4 *>(rows + oa2 + 256 * h);
         const float e0 = C2 ? -b.y : b.y, e1 = C2 ? -b.w : b.w;
         pr[2 * h] = a.x * b.x - a.y * e0;
         pr[2 * h + 1] = a.z * b.z - a.w * e1;
@@ -358,10 +443,10 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_contract_bf16x3(
     splitc += blockIdx.y * bs.cache * 4;
     out += blockIdx.y * bs.part;
     constexpr int TWW = kTWW32;
+    constexpr int BUF = dma_buf_stride(NW);
     extern __shared__ __align__(16) unsigned char lds[];
-    int *row_slot = reinterpret_cast<int *>(lds);
+    int *row_slot = reinterpret_cast<int *>(lds + 2 * BUF);
     int *row_split = row_slot + kRowCap;
-    unsigned char *rows = lds + kTableBytes;
 
     const WorkUnit U = units[blockIdx.x];
     const WorkItem &it = items[U.item];
@@ -376,9 +461,9 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_contract_bf16x3(
     int oa1[TWW], oa2[TWW], ob[TWW], fl[TWW];
 #pragma unroll
     for (int j = 0; j < TWW; ++j) {
-        oa1[j] = max((int)TW[j].a1[m], 0) * kPitch + kq * 64;
-        oa2[j] = (int)TW[j].a2[m] * kPitch + kq * 64;
-        ob[j] = min((int)TW[j].b0 + m, nrows - 1) * kPitch + kq * 64;
+        oa1[j] = dma_off(max((int)TW[j].a1[m], 0), 4 * kq);
+        oa2[j] = dma_off((int)TW[j].a2[m], 4 * kq);
+        ob[j] = dma_off(min((int)TW[j].b0 + m, nrows - 1), 4 * kq);
         fl[j] = uni(TW[j].flags);
     }
 
@@ -391,45 +476,56 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_contract_bf16x3(
 #pragma unroll
         for (int q = 0; q < 4; ++q) accd[j][q] = 0.0;
     }
-    Stager<64 * NW> S;
-    S.cache = reinterpret_cast<const char *>(cache + it.off);
-    S.split = reinterpret_cast<const char *>(splitc + it.off * 4);
-    S.nunits = nrows * (kRowBytes / 8);
+    DmaStager<NW> S;
     __syncthreads();
-    S.init(row_slot, row_split, nrows, it.fstride);
+    S.init(reinterpret_cast<const char *>(cache + it.off),
+           reinterpret_cast<const char *>(splitc + it.off * 4), row_slot, row_split, nrows,
+           it.fstride, cb * kRowBytes, wave, lane);
 
-    if (cb < ce) S.load(cb * kRowBytes);
+    if (cb < ce) S.issue(lds, wave);
     int since = 0;
     for (long long c = cb; c < ce; ++c) {
+        // This wave's loads of chunk c are in LDS; after the barrier, every
+        // wave's are, and every wave is done with chunk c - 1, whose buffer
+        // the loads of chunk c + 1 then overwrite.
+        wait_vmem();
         __syncthreads();
-        S.store(rows);
-        __syncthreads();
-        if (c + 1 < ce) S.load((c + 1) * kRowBytes);
-        bf16x8 arh = {}, arl = {}, aih = {}, ail = {};
-#pragma unroll
-        for (int j = 0; j < TWW; ++j) {
-            if (j >= cnt) break;   // uniform: the wave's remaining slots are empty
-            // B first, so its latency overlaps the pair products.
-            const unsigned char *bp = rows + ob[j];
-            const bf16x8 brh = *reinterpret_cast<const bf16x8 *>(bp);
-            const bf16x8 brl = *reinterpret_cast<const bf16x8 *>(bp + 16);
-            const bf16x8 bih = *reinterpret_cast<const bf16x8 *>(bp + 32);
-            const bf16x8 bil = *reinterpret_cast<const bf16x8 *>(bp + 48);
-            if (fl[j] & TW_NEWA) {
-                switch (fl[j] & (TW_CONJ2 | TW_CONJ3)) {
-                case 0: pair_operands<false, false>(rows, oa1[j], oa2[j], nu, arh, arl, aih, ail); break;
-                case TW_CONJ2: pair_operands<true, false>(rows, oa1[j], oa2[j], nu, arh, arl, aih, ail); break;
-                case TW_CONJ3: pair_operands<false, true>(rows, oa1[j], oa2[j], nu, arh, arl, aih, ail); break;
-                default: pair_operands<true, true>(rows, oa1[j], oa2[j], nu, arh, arl, aih, ail); break;
-                }
+        const int cur = (int)((c - cb) & 1);
+        // rows (read) and nxt (written by the loads of chunk c + 1) are the
+        // two buffers: restrict tells the compiler, so that its waits for
+        // LDS-direct loads do not cover reads of the other buffer.
+        auto step = [&](const unsigned char *__restrict__ rows, unsigned char *__restrict__ nxt) {
+            if (c + 1 < ce) {
+                S.advance();
+                S.issue(nxt, wave);
             }
-            acc[j] = mfma_bf16(arh, brh, acc[j]);
-            acc[j] = mfma_bf16(aih, bih, acc[j]);
-            acc[j] = mfma_bf16(arh, brl, acc[j]);
-            acc[j] = mfma_bf16(aih, bil, acc[j]);
-            acc[j] = mfma_bf16(arl, brh, acc[j]);
-            acc[j] = mfma_bf16(ail, bih, acc[j]);
-        }
+            bf16x8 arh = {}, arl = {}, aih = {}, ail = {};
+#pragma unroll
+            for (int j = 0; j < TWW; ++j) {
+                if (j >= cnt) break;   // uniform: the wave's remaining slots are empty
+                // B first, so its latency overlaps the pair products.
+                const unsigned char *bp = rows + ob[j];
+                const bf16x8 brh = *reinterpret_cast<const bf16x8 *>(bp);
+                const bf16x8 brl = *reinterpret_cast<const bf16x8 *>(bp + 256);
+                const bf16x8 bih = *reinterpret_cast<const bf16x8 *>(bp + 512);
+                const bf16x8 bil = *reinterpret_cast<const bf16x8 *>(bp + 768);
+                if (fl[j] & TW_NEWA) {
+                    switch (fl[j] & (TW_CONJ2 | TW_CONJ3)) {
+                    case 0: pair_operands<false, false>(rows, oa1[j], oa2[j], nu, arh, arl, aih, ail); break;
+                    case TW_CONJ2: pair_operands<true, false>(rows, oa1[j], oa2[j], nu, arh, arl, aih, ail); break;
+                    case TW_CONJ3: pair_operands<false, true>(rows, oa1[j], oa2[j], nu, arh, arl, aih, ail); break;
+                    default: pair_operands<true, true>(rows, oa1[j], oa2[j], nu, arh, arl, aih, ail); break;
+                    }
+                }
+                acc[j] = mfma_bf16(arh, brh, acc[j]);
+                acc[j] = mfma_bf16(aih, bih, acc[j]);
+                acc[j] = mfma_bf16(arh, brl, acc[j]);
+                acc[j] = mfma_bf16(aih, bil, acc[j]);
+                acc[j] = mfma_bf16(arl, brh, acc[j]);
+                acc[j] = mfma_bf16(ail, bih, acc[j]);
+            }
+        };
+        step(lds + cur * BUF, lds + (cur ^ 1) * BUF);
         if (++since == kFlush || c + 1 == ce) {
             since = 0;
 #pragma unroll
@@ -478,9 +574,10 @@ static hipError_t launch_nw(CPrec p, int tww, const void *cache, const void *spl
         hipLaunchKernelGGL(k, grid, dim3(64 * NW), lds, s, (const float2 *)cache, items, units,
                            tws, out, bs);
     } else {
+        const size_t lds_d = dma_lds_bytes(NW);
         (void)hipFuncSetAttribute((const void *)k_contract_bf16x3<NW>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL((k_contract_bf16x3<NW>), grid, dim3(64 * NW), lds, s,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_d);
+        hipLaunchKernelGGL((k_contract_bf16x3<NW>), grid, dim3(64 * NW), lds_d, s,
                            (const float2 *)cache, (const __bf16 *)split, items, units, tws, out,
                            bs);
     }

@@ -81,7 +81,7 @@ constexpr int kTWW32 = 8;       // tile windows per wave, BF16x3 form
 constexpr int kTWWF32 = 6;      // tile windows per wave, FP32 form
 constexpr int kMaxBlk = 48;     // ring blocks per item
 constexpr int kRowCap = 240;    // staged rings per item (8 waves; 30 per wave)
-inline int rowcap_for(int waves) { return 30 * waves; }
+__host__ __device__ constexpr int rowcap_for(int waves) { return 30 * waves; }
 
 enum : int16_t { TW_CONJ2 = 1, TW_CONJ3 = 2, TW_NEWA = 4 };
 
