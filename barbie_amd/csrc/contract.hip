@@ -129,18 +129,25 @@ struct Stager {
 // the item's last are loaded from its last ring and never read.
 constexpr int kDmaGroups = 2;   // 16-ring groups per wave (rowcap / 16 / waves, rounded up)
 
-__host__ __device__ constexpr int dma_buf_stride(int nw) { return nw == 4 ? 32768 : 65536; }
-__host__ __device__ constexpr int dma_lds_bytes(int nw)
+// Bytes between the two chunk buffers: 120 rings (4 waves), 240 (8 waves),
+// 80 (4 waves sized for 3 workgroups per CU, kTWWOcc3 tile windows).
+__host__ __device__ constexpr int dma_buf_stride(int nw, int tww)
 {
-    return 2 * dma_buf_stride(nw) + kTableBytes;
+    return tww == kTWWOcc3 ? 20480 : nw == 4 ? 32768 : 65536;
+}
+__host__ __device__ constexpr int dma_lds_bytes(int nw, int tww)
+{
+    return 2 * dma_buf_stride(nw, tww) + kTableBytes;
 }
 
 static_assert((rowcap_for(4) + 15) / 16 <= kDmaGroups * 4 &&
                   (rowcap_for(8) + 15) / 16 <= kDmaGroups * 8,
               "every 16-ring group of an item has a wave");
-static_assert((rowcap_for(4) + 15) / 16 * 4096 <= dma_buf_stride(4) &&
-                  (rowcap_for(8) + 15) / 16 * 4096 <= dma_buf_stride(8),
+static_assert((rowcap_for(4) + 15) / 16 * 4096 <= dma_buf_stride(4, kTWW32) &&
+                  (rowcap_for(8) + 15) / 16 * 4096 <= dma_buf_stride(8, kTWW32) &&
+                  (rowcap_occ3(4) + 15) / 16 * 4096 <= dma_buf_stride(4, kTWWOcc3),
               "a chunk buffer holds the item's rings");
+static_assert(3 * dma_lds_bytes(4, kTWWOcc3) <= 160 * 1024, "3 workgroups' LDS per CU");
 
 __device__ __forceinline__ int dma_off(int r, int k)
 {
@@ -151,11 +158,9 @@ typedef __attribute__((address_space(3))) void lds_void;
 
 template <int NW>
 struct DmaStager {
-    // This lane's source per group of the wave and quarter, for the chunk of
-    // the last issue.  They stay live (and are advanced only after the wait
-    // for the loads that read them), so no load in flight has its address
-    // registers reused.
-    const char *src[kDmaGroups][4];
+    // This lane's source per group of the wave, for the chunk of the last
+    // issue (advanced only after the wait for the loads that read it).
+    const char *src[kDmaGroups];
     int ngroups;   // 16-ring groups of the item
 
     // Groups G = wave, wave + NW, ...: lane l serves ring 16 G + l%16 and
@@ -169,18 +174,14 @@ struct DmaStager {
         for (int e = 0; e < kDmaGroups; ++e) {
             const int G = wave + e * NW;
             const int r = min(16 * G + (lane & 15), nrows - 1);
-            const char *b = (row_split[r] ? split : cache) + (long long)row_slot[r] * fstride * 8 +
-                            chunk_off + (lane >> 4) * 16;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) src[e][q] = b + 64 * q;
+            src[e] = (row_split[r] ? split : cache) + (long long)row_slot[r] * fstride * 8 +
+                     chunk_off + (lane >> 4) * 16;
         }
     }
     __device__ __forceinline__ void advance()
     {
 #pragma unroll
-        for (int e = 0; e < kDmaGroups; ++e)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) src[e][q] += kRowBytes;
+        for (int e = 0; e < kDmaGroups; ++e) src[e] += kRowBytes;
     }
     // Issues the loads of the current chunk into buffer `buf`.
     __device__ __forceinline__ void issue(unsigned char *buf, int wave)
@@ -191,7 +192,7 @@ struct DmaStager {
             if (G < ngroups) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    __builtin_amdgcn_global_load_lds((const void *)src[e][q],
+                    __builtin_amdgcn_global_load_lds((const void *)(src[e] + 64 * q),
                                                      (lds_void *)(buf + (G * 4 + q) * 1024), 16, 0,
                                                      0);
             }
@@ -433,8 +434,9 @@ __device__ __forceinline__ void pair_operands(const unsigned char *rows, int oa1
 
 // A staged ring of 32 points: raw rows hold float2 per point; split rows
 // hold, per group of 8 points, [Re hi x8][Re lo x8][Im hi x8][Im lo x8] bf16.
-template <int NW>
-__global__ __launch_bounds__(64 * NW, 8 / NW) void k_contract_bf16x3(
+// NW waves, TWW tile windows per wave, MINB workgroups per CU.
+template <int NW, int TWW, int MINB>
+__global__ __launch_bounds__(64 * NW, MINB) void k_contract_bf16x3(
     const float2 *__restrict__ cache, const __bf16 *__restrict__ splitc,
     const WorkItem *__restrict__ items, const WorkUnit *__restrict__ units,
     const TWin *__restrict__ tws, double *__restrict__ out, BatchStrides bs)
@@ -442,8 +444,7 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_contract_bf16x3(
     cache += blockIdx.y * bs.cache;   // field of the batch
     splitc += blockIdx.y * bs.cache * 4;
     out += blockIdx.y * bs.part;
-    constexpr int TWW = kTWW32;
-    constexpr int BUF = dma_buf_stride(NW);
+    constexpr int BUF = dma_buf_stride(NW, TWW);
     extern __shared__ __align__(16) unsigned char lds[];
     int *row_slot = reinterpret_cast<int *>(lds + 2 * BUF);
     int *row_split = row_slot + kRowCap;
@@ -560,7 +561,6 @@ static hipError_t launch_nw(CPrec p, int tww, const void *cache, const void *spl
 {
     dim3 grid((unsigned)nunits, (unsigned)nf);
     const size_t lds = kTableBytes + (size_t)(kRowCap + 1) * kPitch;   // + spare ring
-    (void)tww;
     if (p == CPrec::F64) {
         auto k = k_contract_k4<double, NW, kTWW, 8 / NW>;
         (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -573,13 +573,21 @@ static hipError_t launch_nw(CPrec p, int tww, const void *cache, const void *spl
                                   (int)lds);
         hipLaunchKernelGGL(k, grid, dim3(64 * NW), lds, s, (const float2 *)cache, items, units,
                            tws, out, bs);
+    } else if (tww == kTWWOcc3) {
+        if (NW != 4) return hipErrorInvalidValue;
+        auto k = k_contract_bf16x3<4, kTWWOcc3, 3>;
+        const size_t lds_d = dma_lds_bytes(4, kTWWOcc3);
+        (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds_d);
+        hipLaunchKernelGGL(k, grid, dim3(256), lds_d, s, (const float2 *)cache,
+                           (const __bf16 *)split, items, units, tws, out, bs);
     } else {
-        const size_t lds_d = dma_lds_bytes(NW);
-        (void)hipFuncSetAttribute((const void *)k_contract_bf16x3<NW>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_d);
-        hipLaunchKernelGGL((k_contract_bf16x3<NW>), grid, dim3(64 * NW), lds_d, s,
-                           (const float2 *)cache, (const __bf16 *)split, items, units, tws, out,
-                           bs);
+        auto k = k_contract_bf16x3<NW, kTWW32, 8 / NW>;
+        const size_t lds_d = dma_lds_bytes(NW, kTWW32);
+        (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds_d);
+        hipLaunchKernelGGL(k, grid, dim3(64 * NW), lds_d, s, (const float2 *)cache,
+                           (const __bf16 *)split, items, units, tws, out, bs);
     }
     return hipGetLastError();
 }

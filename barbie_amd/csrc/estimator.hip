@@ -223,9 +223,8 @@ namespace {
 // operands (BF16x3) instead of raw values.  pos[cid] receives tile * 256 +
 // row * 16 + column of each contraction.
 int pack_items(std::vector<PairBlock> &blocks, const Level &L, bool split_b, int tww, int waves,
-               WorkLists &W, std::vector<int32_t> &pos)
+               int rowcap, WorkLists &W, std::vector<int32_t> &pos)
 {
-    const int rowcap = rowcap_for(waves);
     struct Key { int plane, split; bool operator<(const Key &o) const { return plane != o.plane ? plane < o.plane : split < o.split; } };
     struct Range { int lo, hi; };
     size_t a = 0;
@@ -600,7 +599,19 @@ int layout_waves(const bb_options &opt)
     return kWaves;
 }
 
-int layout_tww(CPrec p) { return contract_tww(p); }
+// BF16x3 at 3 workgroups per CU: fewer tile windows and staged rings.
+bool layout_occ3(CPrec p, const bb_options &opt)
+{
+    return p == CPrec::BF16X3 && opt.occupancy == 3 && layout_waves(opt) == 4;
+}
+int layout_tww(CPrec p, const bb_options &opt)
+{
+    return layout_occ3(p, opt) ? kTWWOcc3 : contract_tww(p);
+}
+int layout_rowcap(CPrec p, const bb_options &opt)
+{
+    return layout_occ3(p, opt) ? rowcap_occ3(4) : rowcap_for(layout_waves(opt));
+}
 
 int build_layout(const PlanData &d, CPrec prec, const bb_options &opt, int waves, int tww,
                  int max_levels, Layout &L)
@@ -699,10 +710,12 @@ int build_layout(const PlanData &d, CPrec prec, const bb_options &opt, int waves
             std::vector<PairBlock> &dst = to_f ? blocks_f : blocks;
             for (PairBlock &B : sb) dst.push_back(std::move(B));
         }
-        int rc = pack_items(blocks, L.lv[l], prec == CPrec::BF16X3, tww, waves, W, L.pos);
+        int rc = pack_items(blocks, L.lv[l], prec == CPrec::BF16X3, tww, waves,
+                            layout_rowcap(prec, opt), W, L.pos);
         if (rc) return rc;
         const size_t first_f = W.items.size();
-        rc = pack_items(blocks_f, L.lv[l], false, contract_tww(CPrec::F32), waves, W, L.pos);
+        rc = pack_items(blocks_f, L.lv[l], false, contract_tww(CPrec::F32), waves,
+                        rowcap_for(waves), W, L.pos);
         if (rc) return rc;
         for (size_t i = first_f; i < W.items.size(); ++i) W.kind[i] = 1;
     }
@@ -764,11 +777,12 @@ int build_layout(const PlanData &d, CPrec prec, const bb_options &opt, int waves
 int get_layout(const bb_plan *p, CPrec prec, const bb_options &opt,
                std::shared_ptr<const Layout> &out)
 {
-    const int waves = layout_waves(opt), tww = layout_tww(prec), lev = default_levels(opt);
+    const int waves = layout_waves(opt), tww = layout_tww(prec, opt), lev = default_levels(opt);
     const std::string key = std::to_string((int)prec) + "/" + std::to_string(opt.full_grid != 0) +
                             "/" + std::to_string(opt.x_splits) + "/" + std::to_string(lev) + "/" +
                             std::to_string(waves) + "/" + std::to_string(tww) + "/" +
-                            std::to_string(prec == CPrec::BF16X3 ? opt.mix_fp32 : 0);
+                            std::to_string(prec == CPrec::BF16X3 ? opt.mix_fp32 : 0) + "/" +
+                            std::to_string(layout_occ3(prec, opt));
     bb_plan *mp = const_cast<bb_plan *>(p);
     {
         std::lock_guard<std::mutex> g(mp->mu);
@@ -943,7 +957,8 @@ int compute_num(bb_estimator *E)
     L.pos.assign(std::max(rcount, 1), 0);
     std::vector<PairBlock> blocks;
     if (rcount > 0) pair_blocks(cs, 0, 0, 0, 0, blocks);
-    int rc = pack_items(blocks, U, false, contract_tww(CPrec::F64), kWaves, W, L.pos);
+    int rc = pack_items(blocks, U, false, contract_tww(CPrec::F64), kWaves, rowcap_for(kWaves), W,
+                        L.pos);
     if (!rc) rc = make_units(W, L.chunk, 0, L.units, L.pos, L.nsp, L.part_tiles, nullptr);
     if (rc) return rc;
     L.items.swap(W.items);
@@ -1336,6 +1351,10 @@ int check_options(const bb_options &o)
         return set_error(BB_E_INVALID, "waves must be 0, 4 or 8");
     if (o.mix_fp32 < 0 || o.mix_fp32 > 90)
         return set_error(BB_E_INVALID, "mix_fp32 must be in [0, 90]");
+    if (o.occupancy != 0 && o.occupancy != 2 && o.occupancy != 3)
+        return set_error(BB_E_INVALID, "occupancy must be 0, 2 or 3");
+    if (o.occupancy == 3 && (o.precision != BB_BF16X3 || layout_waves(o) != 4))
+        return set_error(BB_E_INVALID, "occupancy 3 needs precision BB_BF16X3 and 4 waves");
     return BB_OK;
 }
 
@@ -1419,11 +1438,15 @@ int bb_tune(const bb_plan *p, const bb_options *base, const double *d_field, int
         if (base->grid_levels <= 0 || lev <= base->grid_levels) levs.push_back(lev);
     if (levs.empty()) levs.push_back(base->grid_levels);
     if (cp == CPrec::BF16X3) mixes = {0, 30, 45};
-    for (int waves : {8, 4})
+    // (waves, occupancy): occupancy 3 exists for BF16x3 with 4 waves only.
+    std::vector<std::pair<int, int>> shapes{{8, 0}, {4, 0}};
+    if (cp == CPrec::BF16X3) shapes.push_back({4, 3});
+    for (auto shape : shapes)
         for (int lev : levs)
             for (int mix : mixes) {
             bb_options o = *base;
-            o.waves = waves;
+            o.waves = shape.first;
+            o.occupancy = shape.second;
             o.grid_levels = lev;
             o.mix_fp32 = mix;
             o.max_batch = 1;

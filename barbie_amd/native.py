@@ -70,7 +70,8 @@ class BBOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("full_grid", ctypes.c_int),
                 ("x_splits", ctypes.c_int), ("grid_levels", ctypes.c_int),
                 ("max_batch", ctypes.c_int), ("waves", ctypes.c_int),
-                ("mix_fp32", ctypes.c_int), ("use_graph", ctypes.c_int)]
+                ("mix_fp32", ctypes.c_int), ("use_graph", ctypes.c_int),
+                ("occupancy", ctypes.c_int)]
 
 
 class BBServerOptions(ctypes.Structure):
@@ -267,11 +268,11 @@ class Plan:
         return [grids[i] for i in range(n.value)], cl
 
     def worklist(self, precision: str = "fp64", full_grid: bool = False, x_splits: int = 0,
-                 grid_levels: int = 0) -> dict:
+                 grid_levels: int = 0, waves: int = 0, occupancy: int = 0) -> dict:
         """Work-list figures of an Estimator with these options (host only)."""
         o = (ctypes.c_longlong * len(WORKLIST_KEYS))()
         opt = BBOptions(PRECISIONS[precision], int(full_grid), int(x_splits), int(grid_levels), 0,
-                        0)
+                        int(waves), 0, 0, int(occupancy))
         _check(lib().bb_plan_worklist(self.handle, ctypes.byref(opt), o))
         return dict(zip(WORKLIST_KEYS, list(o)))
 
@@ -284,8 +285,8 @@ class Plan:
         ms = ctypes.c_double()
         _check(lib().bb_tune(self.handle, ctypes.byref(base), ctypes.c_void_p(d_field or None),
                              int(repeats), ctypes.byref(best), ctypes.byref(ms)))
-        return ({"grid_levels": best.grid_levels, "waves": best.waves, "mix_fp32": best.mix_fp32},
-                ms.value)
+        return ({"grid_levels": best.grid_levels, "waves": best.waves, "mix_fp32": best.mix_fp32,
+                 "occupancy": best.occupancy}, ms.value)
 
     def write_outputs(self, bs, norm, biana, outdir: str, NRi: int):
         bs, norm, biana = (np.ascontiguousarray(a, dtype=np.float64) for a in (bs, norm, biana))
@@ -298,13 +299,13 @@ class Estimator:
 
     def __init__(self, plan: Plan, precision: str = "fp64", full_grid: bool = False,
                  x_splits: int = 0, grid_levels: int = 0, max_batch: int = 0, waves: int = 0,
-                 use_graph: bool = False, mix_fp32: int = 0):
+                 use_graph: bool = False, mix_fp32: int = 0, occupancy: int = 0):
         self.plan = plan
         self.precision = precision
         self.max_batch = max(1, int(max_batch))
         h = ctypes.c_void_p()
         o = BBOptions(PRECISIONS[precision], int(full_grid), int(x_splits), int(grid_levels),
-                      int(max_batch), int(waves), int(mix_fp32), int(use_graph))
+                      int(max_batch), int(waves), int(mix_fp32), int(use_graph), int(occupancy))
         _check(lib().bb_estimator_create_opts(plan.handle, ctypes.byref(o), ctypes.byref(h)))
         self.handle = h
         ng = ctypes.c_int()

@@ -84,6 +84,11 @@ typedef struct bb_options {
                        buffers, and replay the graph when the same buffers
                        come again (bb_estimate and bb_estimate_batch use the
                        estimator's own buffers, so they always replay) */
+    int occupancy;  /* BF16X3 with 4 waves only: contraction workgroups per
+                       compute unit the kernel is sized for, 2 (0: default) or
+                       3 (6 instead of 8 tile windows per wave and at most 80
+                       instead of 120 staged rings per work item, so that three
+                       workgroups' registers and LDS fit on one CU) */
 } bb_options;
 
 /* Geometry and binning of one estimate.  Replaces the constants hard-coded at

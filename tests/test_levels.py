@@ -118,3 +118,16 @@ def test_waves_option_is_validated():
     assert rc == -1
     w4 = plan.worklist("fp64")
     assert w4["waves"] == 8
+
+
+def test_occupancy_option_is_validated():
+    plan = B.Plan(medium())
+    W = len(B.native.WORKLIST_KEYS)
+    for prec, waves, occ in [("fp64", 4, 3), ("bf16x3", 8, 3), ("bf16x3", 4, 5)]:
+        o = B.native.BBOptions(B.PRECISIONS[prec], 0, 0, 0, 0, waves, 0, 0, occ)
+        out = (B.native.ctypes.c_longlong * W)()
+        assert B.lib().bb_plan_worklist(plan.handle, B.native.ctypes.byref(o), out) == -1
+    w3 = plan.worklist("bf16x3", waves=4, occupancy=3)
+    w2 = plan.worklist("bf16x3", waves=4)
+    assert w3["max_rings"] <= 80
+    assert w3["tiles"] == w2["tiles"]
