@@ -3,10 +3,13 @@
 // comparison), :343-363 (bins, zero-signal rule, accumulation) and cal_coeff
 // (funcs.c:708-745).
 //
-// Contractions arrive sorted by bin; one workgroup of four waves owns one
-// bin, each thread accumulates a strided subset of the bin's contractions in
-// registers, each wave reduces its 48 sums in a fixed tree order and the
-// four wave sums are added in wave order (deterministic).
+// Contractions arrive sorted by bin, cut into segments of at most kBinSeg
+// records within one bin (so that the largest bins do not set the time: at
+// NN=384 one bin holds 26 110 of 2.76 M records).  One workgroup of four
+// waves owns one segment: each thread accumulates a strided subset of the
+// segment's contractions in registers, each wave reduces its 48 sums in a
+// fixed tree order and the four wave sums are added in wave order.  A second
+// kernel adds each bin's segment sums in segment order (deterministic).
 #include <hip/hip_runtime.h>
 
 #include "device.hpp"
@@ -67,24 +70,19 @@ __device__ __forceinline__ void cal_coeff(double px, double pz, double *bs, doub
 
 constexpr int kBinThreads = 256;   // four waves per bin
 
-__global__ __launch_bounds__(kBinThreads) void k_binning(const int32_t *__restrict__ seg,
+__global__ __launch_bounds__(kBinThreads) void k_binning(const int32_t *__restrict__ sg,
                                                 const BinRec *__restrict__ recs,
                                                 const double *__restrict__ S,
                                                 const double *__restrict__ num,
                                                 const int *__restrict__ nzflag,
                                                 const double *__restrict__ avg, BinParams bp,
-                                                double *__restrict__ out_bs,
-                                                double *__restrict__ out_norm,
-                                                double *__restrict__ out_biana,
-                                                BatchStrides st)
+                                                double *__restrict__ binpart, BatchStrides st)
 {
-    const int b = blockIdx.x;
-    const int fb = blockIdx.y;   // field of the batch
+    const int sgi = blockIdx.x;   // segment
+    const int fb = blockIdx.y;    // field of the batch
     S += fb * st.part;
     nzflag += fb * st.nz;
-    if (out_bs) out_bs += fb * st.bins;
-    if (out_norm) out_norm += fb * st.bins;
-    if (out_biana) out_biana += fb * st.bins;
+    binpart += fb * st.binpart;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     double abs_[16], anm[16], abi[16];
 #pragma unroll
@@ -92,8 +90,8 @@ __global__ __launch_bounds__(kBinThreads) void k_binning(const int32_t *__restri
 
     // Software-pipelined over the thread's records: the record and partial
     // sums of the next one are loaded before the current one is binned.
-    const int end = seg[b + 1];
-    int c = seg[b] + tid;
+    const int end = sg[sgi + 1];
+    int c = sg[sgi] + tid;
     BinRec Rn{};
     double Sn = 0.0;
     if (c < end) {
@@ -169,21 +167,47 @@ __global__ __launch_bounds__(kBinThreads) void k_binning(const int32_t *__restri
     if (tid < 48) {
         double v = part[0][tid];
         for (int w = 1; w < kBinThreads / 64; ++w) v += part[w][tid];
-        const int k = tid & 15;
-        double *o = tid < 16 ? out_bs : tid < 32 ? out_norm : out_biana;
-        if (o) o[(long long)b * 16 + k] = v;
+        binpart[(long long)sgi * 48 + tid] = v;
     }
 }
 
-hipError_t launch_binning(const int32_t *seg, const BinRec *recs, const double *S,
-                          const double *num, const int *nzflag, const double *avg,
-                          const BinParams &bp, double *bins_bs, double *bins_norm,
-                          double *bins_biana, const BatchStrides &st, int nf, hipStream_t s)
+// Bin sums: thread (bin, t < 48) adds the bin's segment sums in order;
+// bins without records get zeros.
+__global__ __launch_bounds__(256) void k_bin_reduce(const int32_t *__restrict__ bin_sg, int nbins,
+                                                    const double *__restrict__ binpart,
+                                                    double *__restrict__ out_bs,
+                                                    double *__restrict__ out_norm,
+                                                    double *__restrict__ out_biana,
+                                                    BatchStrides st)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nbins * 48) return;
+    const int fb = blockIdx.y;
+    binpart += fb * st.binpart;
+    const int b = i / 48, t = i - b * 48;
+    double v = 0.0;
+    for (int sgi = bin_sg[b]; sgi < bin_sg[b + 1]; ++sgi) v += binpart[(long long)sgi * 48 + t];
+    double *o = t < 16 ? out_bs : t < 32 ? out_norm : out_biana;
+    if (o) o[fb * st.bins + (long long)b * 16 + (t & 15)] = v;
+}
+
+hipError_t launch_binning(const int32_t *sg, int nseg, const int32_t *bin_sg, const BinRec *recs,
+                          const double *S, const double *num, const int *nzflag, const double *avg,
+                          const BinParams &bp, double *binpart, double *bins_bs,
+                          double *bins_norm, double *bins_biana, const BatchStrides &st, int nf,
+                          hipStream_t s)
 {
     if (bp.nbins <= 0) return hipSuccess;
     if (nf < 1 || nf > 65535) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_binning, dim3(bp.nbins, nf), dim3(kBinThreads), 0, s, seg, recs, S, num,
-                       nzflag, avg, bp, bins_bs, bins_norm, bins_biana, st);
+    if (nseg > 0) {
+        hipLaunchKernelGGL(k_binning, dim3(nseg, nf), dim3(kBinThreads), 0, s, sg, recs, S, num,
+                           nzflag, avg, bp, binpart, st);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    const int n = bp.nbins * 48;
+    hipLaunchKernelGGL(k_bin_reduce, dim3((n + 255) / 256, nf), dim3(256), 0, s, bin_sg, bp.nbins,
+                       binpart, bins_bs, bins_norm, bins_biana, st);
     return hipGetLastError();
 }
 

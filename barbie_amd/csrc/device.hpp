@@ -147,6 +147,7 @@ struct BatchStrides {
     long long nz = 0;       // nonzero flags
     long long part = 0;     // partial sums (doubles)
     long long bins = 0;     // bin outputs (doubles)
+    long long binpart = 0;  // per-segment bin sums (doubles)
 };
 
 // Blocks of the row and column passes: {task, batch index} pairs in device
@@ -190,9 +191,14 @@ hipError_t launch_contract(CPrec p, int waves, int tww, const void *cache, const
                            const TWin *tws, double *out, const BatchStrides &bs, int nf,
                            hipStream_t s);
 
-hipError_t launch_binning(const int32_t *seg, const BinRec *recs, const double *S,
-                          const double *num, const int *nzflag, const double *avg,
-                          const BinParams &bp, double *bins_bs, double *bins_norm,
-                          double *bins_biana, const BatchStrides &bs, int nf, hipStream_t s);
+// Records (sorted by bin) are split into segments of at most kBinSeg records
+// within one bin: sg[s]..sg[s+1] are segment s's records, bin_sg[b]..
+// bin_sg[b+1] the segments of bin b.  binpart holds 48 sums per segment.
+constexpr int kBinSeg = 1024;
+hipError_t launch_binning(const int32_t *sg, int nseg, const int32_t *bin_sg, const BinRec *recs,
+                          const double *S, const double *num, const int *nzflag, const double *avg,
+                          const BinParams &bp, double *binpart, double *bins_bs,
+                          double *bins_norm, double *bins_biana, const BatchStrides &bs, int nf,
+                          hipStream_t s);
 
 }  // namespace bb

@@ -207,6 +207,7 @@ struct Layout {
     std::vector<int16_t> nsp;          // contraction -> partial sums
     std::vector<BinRec> recs;          // bin-sorted
     std::vector<int32_t> seg;
+    std::vector<int32_t> sg, bin_sg;   // binning segments (record ranges) and each bin's first
     BinParams bp{};
     int max_rows = 0;
     long long staged_rows = 0, staged_bytes = 0, useful_pts = 0;
@@ -764,6 +765,15 @@ int build_layout(const PlanData &d, CPrec prec, const bb_options &opt, int waves
     for (int b = 0; b < d.nbins; ++b)
         std::sort(L.recs.begin() + L.seg[b], L.recs.begin() + L.seg[b + 1],
                   [](const BinRec &x, const BinRec &y) { return x.pos < y.pos; });
+    // Binning segments: each bin's records in pieces of at most kBinSeg.
+    L.sg.assign(1, 0);
+    L.bin_sg.assign(d.nbins + 1, 0);
+    for (int b = 0; b < d.nbins; ++b) {
+        L.bin_sg[b] = (int32_t)L.sg.size() - 1;
+        for (int32_t r = L.seg[b]; r < L.seg[b + 1]; r += kBinSeg)
+            L.sg.push_back(std::min(r + kBinSeg, L.seg[b + 1]));
+    }
+    L.bin_sg[d.nbins] = (int32_t)L.sg.size() - 1;
     L.bp.nbins = d.nbins;
     L.bp.n_mtbin = d.g.n_mtbin;
     L.bp.NN = NN;
@@ -831,7 +841,8 @@ struct bb_estimator {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     double *d_part = nullptr;         // partial tiles
     BinRec *d_recs = nullptr;
-    int32_t *d_seg = nullptr;
+    int32_t *d_sg = nullptr, *d_bin_sg = nullptr;
+    double *d_binpart = nullptr;      // [segment][48] per field
     double *d_num = nullptr;
     std::vector<double> num_host;
     int *d_nz = nullptr;              // [Nmax * Nr]
@@ -1063,7 +1074,8 @@ size_t layout_bytes(const PlanData &d, const Layout &L, size_t F)
     b += tw * 16 + nt * sizeof(FieldTask) + L.lv.size() * sizeof(LevelDesc);
     b += L.items.size() * sizeof(WorkItem) + L.tws.size() * sizeof(TWin) +
          L.units.size() * sizeof(WorkUnit);
-    b += L.recs.size() * sizeof(BinRec) + L.seg.size() * 4;
+    b += L.recs.size() * sizeof(BinRec) + (L.sg.size() + L.bin_sg.size()) * 4;
+    b += (L.sg.size() - 1) * 48 * 8 * (size_t)F;
     b += d.tri.size() / 3 * 8 + Nr * 8;
     return b;
 }
@@ -1129,6 +1141,7 @@ int estimator_init(bb_estimator *E)
     E->bst.nz = (long long)Nmax * Nr;
     E->bst.part = (long long)nslots1;
     E->bst.bins = (long long)d.nbins * BB_NPOLES;
+    E->bst.binpart = (long long)(L.sg.size() - 1) * 48;
     if ((rc = dev_alloc(&E->d_planes, (size_t)Nmax * top2 * F, E->owned))) return rc;
     if (E->cprec != CPrec::F64) {
         float2 *c;
@@ -1160,7 +1173,10 @@ int estimator_init(bb_estimator *E)
     }
     if ((rc = dev_alloc(&E->d_part, nslots, E->owned))) return rc;
     if ((rc = dev_alloc(&E->d_recs, L.recs.size(), E->owned))) return rc;
-    if ((rc = dev_alloc(&E->d_seg, L.seg.size(), E->owned))) return rc;
+    if ((rc = dev_alloc(&E->d_sg, L.sg.size(), E->owned))) return rc;
+    if ((rc = dev_alloc(&E->d_bin_sg, L.bin_sg.size(), E->owned))) return rc;
+    if ((rc = dev_alloc(&E->d_binpart, std::max<size_t>(L.sg.size() - 1, 1) * 48 * F, E->owned)))
+        return rc;
     if ((rc = dev_alloc(&E->d_nz, (size_t)Nmax * Nr * F, E->owned))) return rc;
     if ((rc = dev_alloc(&E->d_avg, Nr, E->owned))) return rc;
     if ((rc = dev_alloc(&E->d_bins, (size_t)3 * d.nbins * BB_NPOLES * F, E->owned))) return rc;
@@ -1169,7 +1185,9 @@ int estimator_init(bb_estimator *E)
     if (!L.tws.empty() && (rc = upload(E->d_tws, L.tws.data(), sizeof(TWin) * L.tws.size(), E->stream))) return rc;
     if (!L.units.empty() && (rc = upload(E->d_units, L.units.data(), sizeof(WorkUnit) * L.units.size(), E->stream))) return rc;
     if (!L.recs.empty() && (rc = upload(E->d_recs, L.recs.data(), sizeof(BinRec) * L.recs.size(), E->stream))) return rc;
-    if ((rc = upload(E->d_seg, L.seg.data(), sizeof(int32_t) * L.seg.size(), E->stream))) return rc;
+    if ((rc = upload(E->d_sg, L.sg.data(), sizeof(int32_t) * L.sg.size(), E->stream))) return rc;
+    if ((rc = upload(E->d_bin_sg, L.bin_sg.data(), sizeof(int32_t) * L.bin_sg.size(), E->stream)))
+        return rc;
     if ((rc = upload(E->d_avg, d.avg.data(), sizeof(double) * Nr, E->stream))) return rc;
     HIPCHK(hipMemsetAsync(E->d_part, 0, sizeof(double) * nslots, E->stream));
     // Padding points between fields are read by the contraction and never
@@ -1211,8 +1229,9 @@ int enqueue_kernels(bb_estimator *E, int nf, const double *d_fields, double *d_b
                            E->d_units, (int)L.units.size(), E->d_tws, E->d_part, E->bst, nf, s));
     if (mixed) HIPCHK(hipStreamWaitEvent(s, E->ev_join, 0));
     if (timed) HIPCHK(hipEventRecord(E->ev[3], s));
-    HIPCHK(launch_binning(E->d_seg, E->d_recs, E->d_part, E->d_num, E->d_nz, E->d_avg, L.bp,
-                          d_bs, d_norm, d_biana, E->bst, nf, s));
+    HIPCHK(launch_binning(E->d_sg, (int)L.sg.size() - 1, E->d_bin_sg, E->d_recs, E->d_part,
+                          E->d_num, E->d_nz, E->d_avg, L.bp, E->d_binpart, d_bs, d_norm, d_biana,
+                          E->bst, nf, s));
     return BB_OK;
 }
 
@@ -1698,6 +1717,11 @@ static void worklist_of(const Layout &L, long long *o)
     o[12] = L.useful_pts;
     o[13] = (long long)L.items.size() * L.waves * L.tww;
     o[14] = L.total_cost;
+    o[15] = L.cache_pts;
+    long long band_rows = 0;
+    for (const Level &V : L.lv)
+        for (const FieldTask &T : V.tasks) band_rows += std::min(V.NG, 2 * T.rowband + 1);
+    o[16] = band_rows;
 }
 
 int bb_estimator_worklist(const bb_estimator *e, long long *o)

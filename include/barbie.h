@@ -235,10 +235,14 @@ int bb_estimator_field(bb_estimator *e, int level, int plane, int ring, double *
  * [8] waves per workgroup, [9] wave balance x1000 (tile windows / (waves x
  * busiest wave), summed over items), [10] grid levels, [11] partial tiles
  * (output tiles x their x-splits), [12] sum over contractions of NG^2 of
- * their level (points summed).  The environment variables
+ * their level (points summed), [13] tile-window slots (items x waves x
+ * windows per wave), [14] cost of the busiest waves (windows x chunks, summed
+ * over items), [15] points of all plane-ring fields of all levels (the
+ * cache), [16] band rows the row pass transforms (summed over fields).  The
+ * environment variables
  * BARBIE_CONTRACT_WAVES (4 or 8) and BARBIE_GRID_LEVELS (overrides
  * grid_levels) are read when the estimator is created. */
-#define BB_WORKLIST_N 15
+#define BB_WORKLIST_N 17
 int bb_estimator_worklist(const bb_estimator *e, long long *out);
 /* The same work-list figures for an estimator with these options, computed
  * on the host without a device ([13]: tile window slots executed, i.e.
