@@ -130,10 +130,10 @@ struct Stager {
 constexpr int kDmaGroups = 2;   // 16-ring groups per wave (rowcap / 16 / waves, rounded up)
 
 // Bytes between the two chunk buffers: 120 rings (4 waves), 240 (8 waves),
-// 80 (4 waves sized for 3 workgroups per CU, kTWWOcc3 tile windows).
+// 96 (4 waves sized for 3 workgroups per CU, kTWWOcc3 tile windows).
 __host__ __device__ constexpr int dma_buf_stride(int nw, int tww)
 {
-    return tww == kTWWOcc3 ? 20480 : nw == 4 ? 32768 : 65536;
+    return tww == kTWWOcc3 ? 24576 : nw == 4 ? 32768 : 65536;
 }
 __host__ __device__ constexpr int dma_lds_bytes(int nw, int tww)
 {
