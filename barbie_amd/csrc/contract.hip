@@ -126,11 +126,13 @@ struct Stager {
 // lane l brings piece 4q + l/16 of ring 16 G + l%16.  Lanes that read one
 // piece of 16 different rings (mod 16) hit 64 distinct banks, and each lane's
 // 4 pieces of a quarter are 256 bytes apart (immediate offsets).  Rings past
-// the item's last are loaded from its last ring and never read.
+// the item's last are not loaded; slot nrows (the zero ring) is cleared once
+// and read by empty tile rows and columns.
 constexpr int kDmaGroups = 2;   // 16-ring groups per wave (rowcap / 16 / waves, rounded up)
 
 // Bytes between the two chunk buffers: 120 rings (4 waves), 240 (8 waves),
-// 96 (4 waves sized for 3 workgroups per CU, kTWWOcc3 tile windows).
+// 95 (4 waves sized for 3 workgroups per CU, kTWWOcc3 tile windows), each
+// plus the zero ring.
 __host__ __device__ constexpr int dma_buf_stride(int nw, int tww)
 {
     return tww == kTWWOcc3 ? 24576 : nw == 4 ? 32768 : 65536;
@@ -143,10 +145,10 @@ __host__ __device__ constexpr int dma_lds_bytes(int nw, int tww)
 static_assert((rowcap_for(4) + 15) / 16 <= kDmaGroups * 4 &&
                   (rowcap_for(8) + 15) / 16 <= kDmaGroups * 8,
               "every 16-ring group of an item has a wave");
-static_assert((rowcap_for(4) + 15) / 16 * 4096 <= dma_buf_stride(4, kTWW32) &&
-                  (rowcap_for(8) + 15) / 16 * 4096 <= dma_buf_stride(8, kTWW32) &&
-                  (rowcap_occ3(4) + 15) / 16 * 4096 <= dma_buf_stride(4, kTWWOcc3),
-              "a chunk buffer holds the item's rings");
+static_assert((rowcap_for(4) + 16) / 16 * 4096 <= dma_buf_stride(4, kTWW32) &&
+                  (rowcap_for(8) + 16) / 16 * 4096 <= dma_buf_stride(8, kTWW32) &&
+                  (rowcap_occ3(4) + 16) / 16 * 4096 <= dma_buf_stride(4, kTWWOcc3),
+              "a chunk buffer holds the item's rings and the zero ring");
 static_assert(3 * dma_lds_bytes(4, kTWWOcc3) <= 160 * 1024, "3 workgroups' LDS per CU");
 
 __device__ __forceinline__ int dma_off(int r, int k)
@@ -162,6 +164,7 @@ struct DmaStager {
     // issue (advanced only after the wait for the loads that read it).
     const char *src[kDmaGroups];
     int ngroups;   // 16-ring groups of the item
+    int live;      // bit e: this lane's ring of group e is one of the item's
 
     // Groups G = wave, wave + NW, ...: lane l serves ring 16 G + l%16 and
     // pieces 4q + l/16.
@@ -170,9 +173,11 @@ struct DmaStager {
                                          long long chunk_off, int wave, int lane)
     {
         ngroups = (nrows + 15) >> 4;
+        live = 0;
 #pragma unroll
         for (int e = 0; e < kDmaGroups; ++e) {
             const int G = wave + e * NW;
+            if (16 * G + (lane & 15) < nrows) live |= 1 << e;
             const int r = min(16 * G + (lane & 15), nrows - 1);
             src[e] = (row_split[r] ? split : cache) + (long long)row_slot[r] * fstride * 8 +
                      chunk_off + (lane >> 4) * 16;
@@ -189,7 +194,7 @@ struct DmaStager {
 #pragma unroll
         for (int e = 0; e < kDmaGroups; ++e) {
             const int G = wave + e * NW;
-            if (G < ngroups) {
+            if (G < ngroups && (live >> e & 1)) {   // lanes past the item's rings stay idle
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
                     __builtin_amdgcn_global_load_lds((const void *)(src[e] + 64 * q),
@@ -462,9 +467,13 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_contract_bf16x3(
     int oa1[TWW], oa2[TWW], ob[TWW], fl[TWW];
 #pragma unroll
     for (int j = 0; j < TWW; ++j) {
-        oa1[j] = dma_off(max((int)TW[j].a1[m], 0), 4 * kq);
-        oa2[j] = dma_off((int)TW[j].a2[m], 4 * kq);
-        ob[j] = dma_off(min((int)TW[j].b0 + m, nrows - 1), 4 * kq);
+        // Empty rows and the columns past the window's rings read the zero
+        // ring (slot nrows): their products are zeros, which cost the same
+        // instructions but less power than arbitrary data.
+        const bool row_ok = TW[j].a1[m] >= 0, col_ok = m < TW[j].nb;
+        oa1[j] = dma_off(row_ok ? (int)TW[j].a1[m] : nrows, 4 * kq);
+        oa2[j] = dma_off(row_ok ? (int)TW[j].a2[m] : nrows, 4 * kq);
+        ob[j] = dma_off(col_ok ? (int)TW[j].b0 + m : nrows, 4 * kq);
         fl[j] = uni(TW[j].flags);
     }
 
@@ -477,6 +486,10 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_contract_bf16x3(
 #pragma unroll
         for (int q = 0; q < 4; ++q) accd[j][q] = 0.0;
     }
+    // The zero ring: slot nrows of both buffers (no load writes it).
+    if (threadIdx.x < 32)
+        *reinterpret_cast<float4 *>(lds + (threadIdx.x >> 4) * BUF + dma_off(nrows, threadIdx.x & 15)) =
+            make_float4(0.f, 0.f, 0.f, 0.f);
     DmaStager<NW> S;
     __syncthreads();
     S.init(reinterpret_cast<const char *>(cache + it.off),

@@ -83,7 +83,8 @@ constexpr int kMaxBlk = 48;     // ring blocks per item
 constexpr int kRowCap = 240;    // staged rings per item (8 waves; 30 per wave)
 __host__ __device__ constexpr int rowcap_for(int waves) { return 30 * waves; }
 // Staged rings per item of the BF16x3 kernel sized for 3 workgroups per CU.
-__host__ __device__ constexpr int rowcap_occ3(int waves) { return 24 * waves; }
+// (One ring slot of the 96 in the LDS buffer is kept for the zero ring.)
+__host__ __device__ constexpr int rowcap_occ3(int waves) { return 24 * waves - 1; }
 constexpr int kTWWOcc3 = 6;   // tile windows per wave, BF16x3 at 3 workgroups per CU
 
 enum : int16_t { TW_CONJ2 = 1, TW_CONJ3 = 2, TW_NEWA = 4 };
