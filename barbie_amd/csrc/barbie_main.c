@@ -17,6 +17,11 @@
  *   BARBIE_NMAX (100), BARBIE_NR (40), BARBIE_NMTBIN (5),
  *   BARBIE_RING_RULE (reference | uniform), BARBIE_RING_WIDTH (1.0),
  *   BARBIE_LAST_RING_TO_NMAX (1), BARBIE_PRECISION (fp64 | fp32 | bf16x3).
+ * BARBIE_WRITE_CACHE=1 also writes the reference's on-disk caches
+ * (closed_rings_<NN>, rcountFor_<NN>, dataPlaneRing_<NN>_<p>_<r>_R<NRi>;
+ * funcs.c:328-415, 537-566) into ResFiles_N<NN>/data_files, which the
+ * reference computes on its first run; this build keeps them in HBM and
+ * writes them only on request.
  * Invalid arguments exit with status 2 and a message (the reference does not
  * validate them, docs/REVIEW.md:10); missing input exits with status 3;
  * device errors with status 4.
@@ -169,6 +174,16 @@ int main(int argc, char *argv[])
     if (fp) {
         fwrite(bins, sizeof(double), 3 * nb, fp);
         fclose(fp);
+    }
+    const char *wc = getenv("BARBIE_WRITE_CACHE");
+    if (wc && atoi(wc) != 0) {
+        char cdir[600];
+        snprintf(cdir, sizeof cdir, "%s/data_files", dir);
+        mkdir(cdir, 0777);
+        if (bb_export_cache(est, field, cdir, NRi)) {
+            fprintf(stderr, "barbie: %s\n", bb_last_error());
+            return 3;
+        }
     }
     bb_estimator_destroy(est);
     bb_plan_destroy(plan);

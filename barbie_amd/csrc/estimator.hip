@@ -1047,6 +1047,151 @@ int compute_num(bb_estimator *E)
     return BB_OK;
 }
 
+// The reference's on-disk caches (funcs.c:328-452, 537-566) from a field:
+// closed_rings_<NN> (rcount x {r1, r2, r3, Num} doubles, calCR order),
+// rcountFor_<NN> (the count as text), and for every plane p < Nmax and ring
+// r < Nr dataPlaneRing_<NN>_<p>_<r>_R<NRi>: D_{p,r} on the NN grid, 2 NN^2
+// doubles, row-major interleaved complex (wrapid_fast's layout).  The fields
+// are built on the device in FP64 by the estimator's transforms, nb at a
+// time, and written as they arrive.  A text file cache_config_<NN>_R<NRi>.txt
+// beside them records the geometry and its 64-bit FNV-1a hash, the key the
+// reference's file names lack.
+int export_cache(bb_estimator *E, const double *field_r2c, const std::string &dir, int NRi)
+{
+    const PlanData &d = E->plan->d;
+    const int NN = d.g.NN, Nmax = d.g.Nmax, Nr = d.g.Nr;
+    const int rcount = (int)(d.tri.size() / 3);
+    const std::string nn = std::to_string(NN);
+    auto path = [&](const std::string &f) { return dir + "/" + f; };
+    // Ring triples and triangle counts.
+    {
+        FILE *f = std::fopen(path("closed_rings_" + nn).c_str(), "wb");
+        if (!f) return set_error(BB_E_IO, "cannot create " + path("closed_rings_" + nn));
+        bool ok = true;
+        for (int rp = 0; rp < rcount && ok; ++rp) {
+            const double v[4] = {(double)d.tri[3 * rp], (double)d.tri[3 * rp + 1],
+                                 (double)d.tri[3 * rp + 2], E->num_host[rp]};
+            ok = std::fwrite(v, sizeof(double), 4, f) == 4;
+        }
+        ok = (std::fclose(f) == 0) && ok;
+        FILE *g = std::fopen(path("rcountFor_" + nn).c_str(), "w");
+        if (!g) return set_error(BB_E_IO, "cannot create " + path("rcountFor_" + nn));
+        ok = std::fprintf(g, "%d", rcount) > 0 && ok;
+        ok = (std::fclose(g) == 0) && ok;
+        if (!ok) return set_error(BB_E_IO, "write failed in " + dir);
+    }
+    // Geometry key.
+    {
+        unsigned long long h = 1469598103934665603ULL;
+        auto mix = [&](const void *p, size_t n) {
+            const unsigned char *b = (const unsigned char *)p;
+            for (size_t k = 0; k < n; ++k) h = (h ^ b[k]) * 1099511628211ULL;
+        };
+        const int ints[4] = {NN, Nmax, Nr, d.g.n_mtbin};
+        mix(ints, sizeof ints);
+        mix(d.low.data(), d.low.size() * sizeof(double));
+        mix(d.high.data(), d.high.size() * sizeof(double));
+        const std::string cf = path("cache_config_" + nn + "_R" + std::to_string(NRi) + ".txt");
+        FILE *f = std::fopen(cf.c_str(), "w");
+        if (!f) return set_error(BB_E_IO, "cannot create " + cf);
+        std::fprintf(f, "NN %d\nNmax %d\nNr %d\nn_mtbin %d\nNRi %d\nkey %016llx\n", NN, Nmax, Nr,
+                     d.g.n_mtbin, NRi, h);
+        for (int r = 0; r < Nr; ++r)
+            std::fprintf(f, "ring %d %.17g %.17g\n", r, d.low[r], d.high[r]);
+        if (std::fclose(f) != 0) return set_error(BB_E_IO, "write failed: " + cf);
+    }
+    // Plane-ring fields on the NN grid, FP64.
+    const std::vector<int> K = ring_kx(d);
+    Level V;
+    V.NG = NN;
+    V.npts = (long long)NN * NN;
+    V.stride = (V.npts + 31) / 32 * 32;
+    LevelDesc D;
+    if (int rc = level_desc(V, 0, D)) return rc;
+    const int nfield = Nmax * Nr;
+    const int nb = std::max(1, std::min(nfield, (int)((256LL << 20) / (V.stride * 16))));
+    const std::vector<double2> tw = twiddles(NN);
+    const size_t fsz = (size_t)NN * NN * (NN / 2 + 1);
+    std::vector<void *> tmp;
+    double2 *d_field, *d_planes, *d_cache, *d_tw;
+    FieldTask *d_tasks;
+    LevelDesc *d_ld;
+    int2 *d_blk;
+    const size_t maxblk = (size_t)nb * (NN / std::max(1, std::min(D.rpb, D.cpb)) + 2) * 2;
+    int rc = dev_alloc(&d_field, fsz, tmp);
+    if (!rc) rc = dev_alloc(&d_planes, (size_t)Nmax * NN * NN, tmp);
+    if (!rc) rc = dev_alloc(&d_cache, (size_t)nb * V.stride, tmp);
+    if (!rc) rc = dev_alloc(&d_tw, (size_t)NN, tmp);
+    if (!rc) rc = dev_alloc(&d_tasks, (size_t)nb, tmp);
+    if (!rc) rc = dev_alloc(&d_ld, 1, tmp);
+    if (!rc) rc = dev_alloc(&d_blk, maxblk, tmp);
+    hipStream_t s = E->stream;
+    if (!rc) rc = upload(d_field, field_r2c, fsz * sizeof(double2), s);
+    if (!rc) rc = upload(d_tw, tw.data(), sizeof(double2) * NN, s);
+    if (!rc) rc = upload(d_ld, &D, sizeof D, s);
+    if (!rc) {
+        hipError_t e = launch_gather(d_field, d_planes, NN, NN, NN / 2, Nmax, 1, s);
+        if (e != hipSuccess) rc = set_error(BB_E_DEVICE, hipGetErrorString(e));
+    }
+    std::vector<double2> host((size_t)nb * V.stride);
+    std::vector<double> row((size_t)2 * NN * NN);
+    for (int f0 = 0; f0 < nfield && !rc; f0 += nb) {
+        const int n = std::min(nb, nfield - f0);
+        std::vector<FieldTask> tasks;
+        for (int k = 0; k < n; ++k) {
+            const int p = (f0 + k) / Nr, r = (f0 + k) % Nr;
+            const int band = d.nhi[r] > 0 ? std::min(NN / 2, K[r]) : -1;
+            tasks.push_back(FieldTask{p, r, k, band, d.nlo[r], d.nhi[r], 0, 0});
+        }
+        std::vector<int2> brow, bcol;
+        fft_blocks(tasks.data(), n, &D, brow, bcol);
+        if (brow.size() + bcol.size() > maxblk) {
+            rc = set_error(BB_E_INVALID, "internal: export block list");
+            break;
+        }
+        const size_t nrow = brow.size();
+        brow.insert(brow.end(), bcol.begin(), bcol.end());
+        rc = upload(d_tasks, tasks.data(), sizeof(FieldTask) * n, s);
+        if (!rc && !brow.empty()) rc = upload(d_blk, brow.data(), sizeof(int2) * brow.size(), s);
+        if (!rc) {
+            FftBlocks fb;
+            fb.row = d_blk;
+            fb.nrow = (int)nrow;
+            fb.col = d_blk + nrow;
+            fb.ncol = (int)bcol.size();
+            const BatchStrides one;
+            hipError_t e = launch_fft_f64(d_planes, NN, d_cache, d_tasks, fb, d_ld, &D, 1, d_tw,
+                                          nullptr, one, 1, s);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(host.data(), d_cache, sizeof(double2) * n * V.stride,
+                                   hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) rc = set_error(BB_E_DEVICE, hipGetErrorString(e));
+        }
+        for (int k = 0; k < n && !rc; ++k) {
+            const int p = (f0 + k) / Nr, r = (f0 + k) % Nr;
+            const double2 *src = host.data() + (size_t)k * V.stride;   // column-major
+            for (int i = 0; i < NN; ++i)
+                for (int j = 0; j < NN; ++j) {
+                    row[2 * ((size_t)i * NN + j)] = src[(size_t)j * NN + i].x;
+                    row[2 * ((size_t)i * NN + j) + 1] = src[(size_t)j * NN + i].y;
+                }
+            const std::string fn = path("dataPlaneRing_" + nn + "_" + std::to_string(p) + "_" +
+                                        std::to_string(r) + "_R" + std::to_string(NRi));
+            FILE *f = std::fopen(fn.c_str(), "wb");
+            if (!f) {
+                rc = set_error(BB_E_IO, "cannot create " + fn);
+                break;
+            }
+            const bool ok = std::fwrite(row.data(), sizeof(double), row.size(), f) == row.size();
+            if ((std::fclose(f) != 0) || !ok) rc = set_error(BB_E_IO, "write failed: " + fn);
+        }
+    }
+    (void)hipStreamSynchronize(s);
+    for (void *q : tmp) (void)hipFree(q);
+    return rc;
+}
+
 // Device bytes held by an estimator of layout L (steady state).
 // Device bytes held by an estimator of layout L with F fields per batch:
 // per-field buffers times F, shared work lists once, and the staging of one
@@ -1698,6 +1843,13 @@ int bb_estimator_field(bb_estimator *e, int level, int plane, int ring, double *
             }
     }
     return BB_OK;
+}
+
+int bb_export_cache(bb_estimator *e, const double *field_r2c, const char *dir, int NRi)
+{
+    if (!e || !field_r2c || !dir) return set_error(BB_E_INVALID, "null argument");
+    if (int rc = estimate_finish(e)) return rc;
+    return export_cache(e, field_r2c, dir, NRi);
 }
 
 static void worklist_of(const Layout &L, long long *o)

@@ -33,7 +33,7 @@ EXPORTED = [
     "bb_estimate_batch", "bb_estimate_batch_device", "bb_estimate_batch_async",
     "bb_estimator_query", "bb_estimator_wait", "bb_estimator_stream", "bb_estimator_set_stream",
     "bb_estimator_contraction_sums", "bb_estimator_triangle_counts", "bb_estimator_field",
-    "bb_estimator_timings", "bb_estimator_worklist", "bb_write_outputs",
+    "bb_estimator_timings", "bb_estimator_worklist", "bb_write_outputs", "bb_export_cache",
     "bb_server_create", "bb_server_destroy", "bb_submit", "bb_poll", "bb_wait", "bb_server_pause",
     "bb_server_prepare",
     "bb_server_stats",
@@ -151,6 +151,7 @@ def lib() -> ctypes.CDLL:
                                           ctypes.POINTER(ctypes.c_int), _ip, _ip]
         L.bb_estimator_timings.argtypes = [ctypes.c_void_p, _dp]
         L.bb_estimator_worklist.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)]
+        L.bb_export_cache.argtypes = [ctypes.c_void_p, _dp, ctypes.c_char_p, ctypes.c_int]
         L.bb_write_outputs.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_char_p,
                                        ctypes.c_int]
         _lib = L
@@ -369,6 +370,12 @@ class Estimator:
 
     def wait(self):
         _check(lib().bb_estimator_wait(self.handle))
+
+    def export_cache(self, field: np.ndarray, directory: str, NRi: int):
+        """bb_export_cache: the reference's cache files of `field` (host
+        [NN][NN][NN+2] doubles) in `directory`."""
+        f = np.ascontiguousarray(field, dtype=np.float64)
+        _check(lib().bb_export_cache(self.handle, _p(f), directory.encode(), int(NRi)))
 
     def contraction_sums(self) -> np.ndarray:
         S = np.zeros(self.plan.stats["ncontr"])

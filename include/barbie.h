@@ -44,8 +44,9 @@ enum {
     BB_RINGS_EXPLICIT = 2   /* caller-supplied low/high edges */
 };
 
-/* Arithmetic of the contraction stage (the FFTs always run in FP64; the
- * per-bin sums, triangle counts and Y_lm projection are FP64 in every mode). */
+/* Arithmetic of the contraction stage (the FFTs run in FP64 for BB_FP64 and
+ * in FP32 for the others; the per-bin sums, triangle counts and Y_lm
+ * projection are FP64 in every mode). */
 typedef enum {
     BB_FP64 = 0,    /* FP64 fields, FP64 MFMA contraction (the FP64 GPU oracle) */
     BB_FP32 = 1,    /* FP32 fields, FP32 MFMA contraction, FP64 across LDS chunks */
@@ -312,6 +313,18 @@ int bb_server_stats(bb_server *s, long long *out);
  * "%d\t%d\t%d\t%e\t%e\t%f\n" format. */
 int bb_write_outputs(const bb_plan *p, const double *bin_bs, const double *bin_norm,
                      const double *bin_biana, const char *dir, int NRi);
+
+/* The reference's on-disk caches of a field, written into dir (which must
+ * exist), in place of calCR's and wrapid_fast's files (funcs.c:328-415,
+ * 537-566; read by read_rings, funcs.c:429-452, and barbie.c:312-320):
+ * closed_rings_<NN> (rcount x {r1, r2, r3, Num} doubles), rcountFor_<NN>
+ * (text), and dataPlaneRing_<NN>_<p>_<r>_R<NRi> for every p < Nmax, r < Nr
+ * (D_{p,r} on the NN grid, 2*NN*NN doubles, row-major interleaved complex,
+ * FP64).  Also cache_config_<NN>_R<NRi>.txt: the geometry and a 64-bit
+ * FNV-1a key of it.  field_r2c: host [NN][NN][NN+2] doubles.  The fields are
+ * built on the device (FP64, the estimator's transforms on the NN grid) in
+ * batches of about 256 MiB. */
+int bb_export_cache(bb_estimator *e, const double *field_r2c, const char *dir, int NRi);
 
 #ifdef __cplusplus
 }
