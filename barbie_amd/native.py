@@ -16,6 +16,7 @@ import numpy as np
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "libbarbie_amd.so")
 BIN_PATH = os.path.join(PKG_DIR, "bin", "barbie")
+CALPOW_PATH = os.path.join(PKG_DIR, "bin", "calpow_mom")
 
 NPOLES = 16
 BB_FP64, BB_FP32, BB_BF16X3 = 0, 1, 2
@@ -34,6 +35,7 @@ EXPORTED = [
     "bb_estimator_query", "bb_estimator_wait", "bb_estimator_stream", "bb_estimator_set_stream",
     "bb_estimator_contraction_sums", "bb_estimator_triangle_counts", "bb_estimator_field",
     "bb_estimator_timings", "bb_estimator_worklist", "bb_write_outputs", "bb_export_cache",
+    "bb_power_multipoles",
     "bb_server_create", "bb_server_destroy", "bb_submit", "bb_poll", "bb_wait", "bb_server_pause",
     "bb_server_prepare",
     "bb_server_stats",
@@ -152,6 +154,8 @@ def lib() -> ctypes.CDLL:
         L.bb_estimator_timings.argtypes = [ctypes.c_void_p, _dp]
         L.bb_estimator_worklist.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)]
         L.bb_export_cache.argtypes = [ctypes.c_void_p, _dp, ctypes.c_char_p, ctypes.c_int]
+        L.bb_power_multipoles.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_double, _dp]
         L.bb_write_outputs.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_char_p,
                                        ctypes.c_int]
         _lib = L
@@ -418,6 +422,20 @@ def normalise(geom: Geometry, bs, norm, biana):
 SERVER_STATS_KEYS = ("submitted", "completed", "failed", "batches", "batched_fields",
                      "lanes_created", "lanes_evicted", "peak_bytes", "queued", "running",
                      "lanes", "bytes", "budget")
+
+
+def power_multipoles(field, NN: int, Nbin: int = 10, LL: float = 0.56) -> np.ndarray:
+    """bb_power_multipoles (calpow_mom on the GPU): per bin {kmean, P0, P2,
+    P4, P6, num}, shape (Nbin, 6).  `field` is a host array [NN][NN][NN+2] or
+    a device address (int, e.g. a torch tensor's data_ptr())."""
+    out = np.zeros((Nbin, 6))
+    if isinstance(field, int):
+        src, dev = ctypes.c_void_p(field), 1
+    else:
+        f = np.ascontiguousarray(field, dtype=np.float64)
+        src, dev = f.ctypes.data_as(ctypes.c_void_p), 0
+    _check(lib().bb_power_multipoles(src, dev, int(NN), int(Nbin), float(LL), _p(out)))
+    return out
 
 
 class Server:

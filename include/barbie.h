@@ -326,6 +326,17 @@ int bb_write_outputs(const bb_plan *p, const double *bin_bs, const double *bin_n
  * batches of about 256 MiB. */
 int bb_export_cache(bb_estimator *e, const double *field_r2c, const char *dir, int NRi);
 
+/* Power-spectrum multipoles of a field: the program calpow_mom
+ * (calpow_mom.c:22-117) and its calpow_mom function (funcs.c:860-1097) on the
+ * current device.  field_r2c: [NN][NN][NN+2] doubles, host memory, or device
+ * memory when on_device != 0.  Nbin logarithmic |k| bins (1..10; the
+ * reference uses 10), cell size LL (the reference uses 0.56).  out: Nbin rows
+ * of {kmean, P0, P2, P4, P6, num}, normalised as funcs.c:1055-1071 does (the
+ * columns of psoutN<NN>_R<NRi>.dat, calpow_mom.c:104-113).  Mode bins match
+ * the reference exactly; the Legendre weights are float, as there. */
+int bb_power_multipoles(const double *field_r2c, int on_device, int NN, int Nbin, double LL,
+                        double *out);
+
 #ifdef __cplusplus
 }
 #endif

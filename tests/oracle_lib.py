@@ -44,6 +44,7 @@ def lib():
         L.orc_estimate.restype = ctypes.c_long
         L.orc_write_outputs.argtypes = [ctypes.POINTER(OrcParams), _dp, _dp, _dp, ctypes.c_char_p,
                                         ctypes.c_int]
+        L.orc_calpow_mom.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double, _dp]
         _lib = L
     return _lib
 
@@ -133,3 +134,11 @@ def write_outputs(params, bs, nm, bi, outdir, NRi):
     P = OrcParams(**params)
     assert lib().orc_write_outputs(ctypes.byref(P), _p(bs), _p(nm), _p(bi),
                                    outdir.encode(), NRi) == 0
+
+
+def calpow_mom(field, NN, Nbin=10, LL=0.56):
+    """funcs.c:860-1097: per bin {kmean, P0, P2, P4, P6, num}, shape (Nbin, 6)."""
+    out = np.zeros((Nbin, 6))
+    f = np.ascontiguousarray(field, dtype=np.float64)
+    assert lib().orc_calpow_mom(NN, _p(f), Nbin, float(LL), _p(out)) == 0
+    return out
