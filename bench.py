@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the fNL 0.1 / 1.0 fields")
+    ap.add_argument("--no-power", action="store_true",
+                    help="skip the power-spectrum multipoles (calpow_mom) leg")
     ap.add_argument("--no-tune", action="store_true",
                     help="use the default layout instead of the autotuned one (bb_tune)")
     ap.add_argument("--layout", default="",
@@ -233,6 +235,35 @@ def serving_leg(B, torch, precision, n_sat=200, n_load=300, seed=7):
     return out
 
 
+def power_leg(B, torch, field, d_field, NN, reps=20):
+    """Power-spectrum multipoles (calpow_mom) of the same field, device
+    resident: wall time per call (launches, the field pass, the sum over
+    workgroups and the copy of 40 sums back; the geometry pass of the first
+    call per (NN, bins) is outside the timed calls) and the HBM rate of the
+    algorithmic bytes, 16 B per summed mode; the CPU oracle of the same call
+    on one host thread beside it."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    ptr = d_field.data_ptr()
+    B.power_multipoles(ptr, NN)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        got = B.power_multipoles(ptr, NN)
+    dt = (time.perf_counter() - t0) / reps
+    nbytes = 16.0 * float(got[:, 5].sum())
+    t1 = time.perf_counter()
+    ref = O.calpow_mom(field, NN)
+    dcpu = time.perf_counter() - t1
+    return {"ms_per_field": dt * 1e3, "fields_per_s": 1.0 / dt, "summed_modes": float(got[:, 5].sum()),
+            "bytes_algorithmic": nbytes, "hbm_GBps": nbytes / dt / 1e9, "hbm_frac": nbytes / dt / 8e12,
+            "num_equal_to_oracle": bool(np.array_equal(got[:, 5], ref[:, 5])),
+            "max_rel_diff_P0_vs_oracle": float(np.max(np.abs(got[:, 1] - ref[:, 1]) / np.abs(ref[:, 1]))),
+            "cpu_oracle_ms": dcpu * 1e3, "cpu_cores": 1,
+            "note": "wall time per bb_power_multipoles call on a device field"}
+
+
 def cpu_baseline(B, plan, field, target_s):
     """CPU oracle on the p1 prefix whose contraction count makes about
     target_s of single-thread work, extrapolated by contraction count; the
@@ -377,6 +408,8 @@ def main():
             acc = accuracy_leg(B, torch, plan, g, a.precision)
             acc["0.3"] = max(out["rel_err_vs_fp64_per_multipole"])
             out["max_rel_err_vs_fp64_by_fnl"] = acc
+    if not a.no_power:
+        out["power_multipoles"] = power_leg(B, torch, field, d_field, g.NN)
     del d_field
     torch.cuda.empty_cache()
     if not a.no_batched:

@@ -333,7 +333,10 @@ int bb_export_cache(bb_estimator *e, const double *field_r2c, const char *dir, i
  * reference uses 10), cell size LL (the reference uses 0.56).  out: Nbin rows
  * of {kmean, P0, P2, P4, P6, num}, normalised as funcs.c:1055-1071 does (the
  * columns of psoutN<NN>_R<NRi>.dat, calpow_mom.c:104-113).  Mode bins match
- * the reference exactly; the Legendre weights are float, as there. */
+ * the reference exactly; the Legendre weights are float, as there.  The work
+ * runs on a stream of the library's own, so a device field must be complete
+ * before the call.  The first call on a device keeps 0.7 MB of partial sums
+ * and that stream until the process ends; calls are serialised. */
 int bb_power_multipoles(const double *field_r2c, int on_device, int NN, int Nbin, double LL,
                         double *out);
 

@@ -2,10 +2,13 @@
 
 Mode counts per bin are bit-exact (the bins are decided as the reference
 decides them); kmean, the monopole and P2 within 1e-12; P4 and P6 within
-5e-6 of the bin's monopole: their float Legendre weights use powf, and the
-device's powf differs from the C library's in the last bit for about 0.07%
-of arguments (measured differences up to 1.2e-6 of P0 at NN=64 and 128).  Host and device inputs give the same bits; the calpow_mom executable
-keeps the reference's files."""
+5e-8 of the bin's monopole.  Their float Legendre weights need powf(mu, 4)
+and powf(mu, 6): the kernel forms the correctly rounded powers, the oracle
+calls the C library's powf (the reference's), which differs in the last bit
+for about 0.07% of float arguments (a CPU restatement with the kernel's
+powers differs from the oracle by at most 6e-9 of P0 at NN=64 to 192).
+Host and device inputs give the same bits; the calpow_mom executable keeps
+the reference's files."""
 import os
 import subprocess
 
@@ -25,10 +28,11 @@ def compare(got, ref):
     assert np.allclose(got[pop, 1], ref[pop, 1], rtol=1e-12)
     assert np.all(np.abs(got[pop, 2] - ref[pop, 2]) <= 1e-12 * np.abs(ref[pop, 1])), 2
     for c in (3, 4):
-        assert np.all(np.abs(got[pop, c] - ref[pop, c]) <= 5e-6 * np.abs(ref[pop, 1])), c
+        assert np.all(np.abs(got[pop, c] - ref[pop, c]) <= 5e-8 * np.abs(ref[pop, 1])), c
 
 
-@pytest.mark.parametrize("NN,fnl,beta", [(24, 0.3, 1.0), (40, 1.0, 0.5), (64, 0.3, 1.0)])
+@pytest.mark.parametrize("NN,fnl,beta", [(24, 0.3, 1.0), (40, 1.0, 0.5), (64, 0.3, 1.0),
+                                         (150, 0.3, 1.0), (192, 1.0, 0.8)])
 def test_power_multipoles_match_oracle(NN, fnl, beta):
     f = B.make_field(NN, NRi=2, fnl=fnl, beta=beta)
     got = B.power_multipoles(f, NN)
