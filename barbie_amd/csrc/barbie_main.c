@@ -22,6 +22,9 @@
  * funcs.c:328-415, 537-566) into ResFiles_N<NN>/data_files, which the
  * reference computes on its first run; this build keeps them in HBM and
  * writes them only on request.
+ * BARBIE_CHECK_DENOMINATORS=1 writes the multipole files with each
+ * multipole's own denominator checked (docs/REVIEW.md:11): entries without a
+ * positive finite denominator are written as "nan" and counted on stderr.
  * Invalid arguments exit with status 2 and a message (the reference does not
  * validate them, docs/REVIEW.md:10); missing input exits with status 3;
  * device errors with status 4.
@@ -164,7 +167,16 @@ int main(int argc, char *argv[])
     }
     printf("Total time %f\n", t_done - t0);
 
-    rc = bb_write_outputs(plan, bins, bins + nb, bins + 2 * nb, dir, NRi);
+    const char *chk = getenv("BARBIE_CHECK_DENOMINATORS");
+    if (chk && atoi(chk)) {
+        long long unsupported = 0;
+        rc = bb_write_outputs_checked(plan, bins, bins + nb, bins + 2 * nb, dir, NRi, &unsupported);
+        if (!rc && unsupported)
+            fprintf(stderr, "barbie: %lld (bin, multipole) entries have no positive finite "
+                            "denominator; written as nan\n", unsupported);
+    } else {
+        rc = bb_write_outputs(plan, bins, bins + nb, bins + 2 * nb, dir, NRi);
+    }
     if (rc) {
         fprintf(stderr, "barbie: %s\n", bb_last_error());
         return 3;

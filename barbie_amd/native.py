@@ -34,7 +34,7 @@ EXPORTED = [
     "bb_estimate_batch", "bb_estimate_batch_device", "bb_estimate_batch_async",
     "bb_estimator_query", "bb_estimator_wait", "bb_estimator_stream", "bb_estimator_set_stream",
     "bb_estimator_contraction_sums", "bb_estimator_triangle_counts", "bb_estimator_field",
-    "bb_estimator_timings", "bb_estimator_worklist", "bb_write_outputs", "bb_export_cache",
+    "bb_estimator_timings", "bb_estimator_worklist", "bb_write_outputs", "bb_write_outputs_checked", "bb_export_cache",
     "bb_power_multipoles",
     "bb_server_create", "bb_server_destroy", "bb_submit", "bb_poll", "bb_wait", "bb_server_pause",
     "bb_server_prepare",
@@ -158,6 +158,8 @@ def lib() -> ctypes.CDLL:
                                           ctypes.c_double, _dp]
         L.bb_write_outputs.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_char_p,
                                        ctypes.c_int]
+        L.bb_write_outputs_checked.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_char_p,
+                                               ctypes.c_int, ctypes.POINTER(ctypes.c_longlong)]
         _lib = L
     return _lib
 
@@ -293,10 +295,19 @@ class Plan:
         return ({"grid_levels": best.grid_levels, "waves": best.waves, "mix_fp32": best.mix_fp32,
                  "occupancy": best.occupancy}, ms.value)
 
-    def write_outputs(self, bs, norm, biana, outdir: str, NRi: int):
+    def write_outputs(self, bs, norm, biana, outdir: str, NRi: int, checked: bool = False):
+        """The 16 multipole files (barbie.c:383-397).  checked: multipoles
+        whose own denominator is not positive and finite are written as
+        "nan"; returns their count (0 when not checked)."""
         bs, norm, biana = (np.ascontiguousarray(a, dtype=np.float64) for a in (bs, norm, biana))
-        _check(lib().bb_write_outputs(self.handle, _p(bs), _p(norm), _p(biana),
-                                      outdir.encode(), NRi))
+        if not checked:
+            _check(lib().bb_write_outputs(self.handle, _p(bs), _p(norm), _p(biana),
+                                          outdir.encode(), NRi))
+            return 0
+        n = ctypes.c_longlong()
+        _check(lib().bb_write_outputs_checked(self.handle, _p(bs), _p(norm), _p(biana),
+                                              outdir.encode(), NRi, ctypes.byref(n)))
+        return n.value
 
 
 class Estimator:

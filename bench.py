@@ -95,6 +95,16 @@ def time_path(B, torch, plan, precision, d_field, steps, warmup, full_grid=False
     return (t1 - t0) * 1e3 / steps, st, bins, levels, wl
 
 
+def unsupported_entries(bins, nbins):
+    """(bin, multipole) entries of the written rows (monopole denominator
+    > .1, barbie.c:392) whose own denominator is not positive: what
+    bb_write_outputs_checked writes as nan."""
+    import numpy as np
+    nm = bins[nbins * 16:2 * nbins * 16].reshape(-1, 16)
+    rows = nm[:, 0] > .1
+    return int((rows[:, None] & ~(nm > 0)).sum())
+
+
 def per_multipole_err(bins, ref, nb):
     import numpy as np
     a = bins[:nb].reshape(-1, 16)
@@ -386,6 +396,7 @@ def main():
         "tuned": {p: {"options": o, "tune_ms_per_estimate": m, "tune_seconds": ts}
                   for p, (o, m, ts) in tuned.items()},
         "worklist": wl,
+        "multipoles_without_denominator": unsupported_entries(bins, g.nbins),
         "roofline": {"bound": "mfma", "kernel": "contraction",
                      "achieved": w_ref / t_contr / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                      "frac": w_ref / t_contr / peak, "traffic": traffic,

@@ -313,6 +313,16 @@ int bb_server_stats(bb_server *s, long long *out);
  * "%d\t%d\t%d\t%e\t%e\t%f\n" format. */
 int bb_write_outputs(const bb_plan *p, const double *bin_bs, const double *bin_norm,
                      const double *bin_biana, const char *dir, int NRi);
+/* Same files with the denominator checks the reference lacks
+ * (barbie.c:388-393 divides every multipole by its own bin_normalization but
+ * filters rows on the monopole's only): in a written row, a multipole whose
+ * own denominator is not positive and finite, or whose B or Biana is not
+ * finite, is written as "nan" in both value columns (the reference prints
+ * whatever 0/0 or x/0 gives there).  *n_unsupported (may be NULL): the
+ * number of such (bin, multipole) entries over the 16 files. */
+int bb_write_outputs_checked(const bb_plan *p, const double *bin_bs, const double *bin_norm,
+                             const double *bin_biana, const char *dir, int NRi,
+                             long long *n_unsupported);
 
 /* The reference's on-disk caches of a field, written into dir (which must
  * exist), in place of calCR's and wrapid_fast's files (funcs.c:328-415,

@@ -158,3 +158,42 @@ def test_bins_follow_the_law_of_cosines(gf):
     assert ok.mean() > 0.9
     assert np.array_equal(flat[ok], c[ok, 7])
     assert np.all((MU >= 0.5 - 1e-12) & (MU <= 1.0 + 1e-12))
+
+
+def test_checked_outputs_mark_missing_denominators(tmp_path):
+    """bb_write_outputs_checked: rows as the reference writes them
+    (monopole denominator > .1), every multipole whose own denominator is
+    zero, negative or not finite written as "nan" and counted; the other
+    entries identical to the unchecked files."""
+    g = tiny()
+    plan = B.Plan(g)
+    rng = np.random.default_rng(1)
+    nb = g.nbins * 16
+    bs, nm, bi = rng.standard_normal(nb), np.abs(rng.standard_normal(nb)) + 0.2, rng.standard_normal(nb)
+    nm4 = nm.reshape(-1, 16)
+    nm4[::3, 2] = 0.0
+    nm4[1::5, 7] = np.inf
+    nm4[2::7, 11] = -1.0
+    nm4[4::9, 0] = 0.05          # row dropped, as the reference drops it
+    (tmp_path / "a").mkdir()
+    (tmp_path / "b").mkdir()
+    assert plan.write_outputs(bs, nm, bi, str(tmp_path / "a"), 3) == 0
+    n = plan.write_outputs(bs, nm, bi, str(tmp_path / "b"), 3, checked=True)
+    rows = nm4[:, 0] > .1
+    expect = 0
+    for ii in range(16):
+        d = nm4[:, ii]
+        bad = rows & ~((d > 0) & np.isfinite(d))
+        expect += int(bad.sum())
+        name = f"BS_test_ii{ii}_N16_R3.txt"
+        la = (tmp_path / "a" / name).read_text().splitlines()
+        lb = (tmp_path / "b" / name).read_text().splitlines()
+        assert len(la) == len(lb) == int(rows.sum())
+        for row, x, y in zip(np.nonzero(rows)[0], la, lb):
+            if bad[row]:
+                fb = y.split("\t")
+                assert fb[3] == "nan" and fb[4] == "nan", y
+                assert fb[:3] == x.split("\t")[:3] and fb[5] == x.split("\t")[5]
+            else:
+                assert x == y
+    assert n == expect and n > 0
