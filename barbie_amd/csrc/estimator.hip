@@ -662,7 +662,7 @@ int build_layout(const PlanData &d, CPrec prec, const bb_options &opt, int waves
                 if (V.rlo[p] < 0) continue;
                 for (int r = V.rlo[p]; r <= V.rhi[p]; ++r) {
                     const int band = d.nhi[r] > 0 ? std::min(V.NG / 2, K[r]) : -1;
-                    V.tasks.push_back(FieldTask{p, r, V.nfields, band, d.nlo[r], d.nhi[r], p * Nr + r, l});
+                    V.tasks.push_back(FieldTask{p, r, V.nfields, band, d.nlo[r], d.nhi[r], p * Nr + r, l, 3});
                     V.max_rows = std::max(V.max_rows, std::min(V.NG, 2 * band + 1));
                     V.nfields++;
                 }
@@ -710,6 +710,22 @@ int build_layout(const PlanData &d, CPrec prec, const bb_options &opt, int waves
             if (to_f) cost_f += c;
             std::vector<PairBlock> &dst = to_f ? blocks_f : blocks;
             for (PairBlock &B : sb) dst.push_back(std::move(B));
+        }
+        // What each field of the level is read as: the BF16x3 kernel reads
+        // pair fields as FP32 values and third fields as bf16 operands; the
+        // FP32 kernel of mixed layouts reads all three as FP32 values.  The
+        // column pass writes only those forms.
+        if (prec == CPrec::BF16X3) {
+            std::vector<uint8_t> role((size_t)Nmax * Nr, 0);
+            for (int f = 0; f < 2; ++f)
+                for (const PairBlock &B : f ? blocks_f : blocks) {
+                    for (int m = 0; m < B.nrow; ++m) {
+                        role[(size_t)B.pa * Nr + B.r1[m]] |= 1;
+                        role[(size_t)B.pb * Nr + B.r2[m]] |= 1;
+                    }
+                    for (const auto &c : B.cs) role[(size_t)B.pc * Nr + c[1]] |= f ? 1 : 2;
+                }
+            for (FieldTask &t : L.lv[l].tasks) t.forms = role[(size_t)t.plane * Nr + t.ring];
         }
         int rc = pack_items(blocks, L.lv[l], prec == CPrec::BF16X3, tww, waves,
                             layout_rowcap(prec, opt), W, L.pos);
@@ -955,7 +971,7 @@ int compute_num(bb_estimator *E)
     U.nfields = Nr;
     for (int r = 0; r < Nr; ++r) {
         const int band = d.nhi[r] > 0 ? std::min(NG / 2, K[r]) : -1;
-        U.tasks.push_back(FieldTask{-1, r, r, band, d.nlo[r], d.nhi[r], 0, 0});
+        U.tasks.push_back(FieldTask{-1, r, r, band, d.nlo[r], d.nhi[r], 0, 0, 3});
         U.max_rows = std::max(U.max_rows, std::min(NG, 2 * band + 1));
     }
     Layout L;
@@ -1141,7 +1157,7 @@ int export_cache(bb_estimator *E, const double *field_r2c, const std::string &di
         for (int k = 0; k < n; ++k) {
             const int p = (f0 + k) / Nr, r = (f0 + k) % Nr;
             const int band = d.nhi[r] > 0 ? std::min(NN / 2, K[r]) : -1;
-            tasks.push_back(FieldTask{p, r, k, band, d.nlo[r], d.nhi[r], 0, 0});
+            tasks.push_back(FieldTask{p, r, k, band, d.nlo[r], d.nhi[r], 0, 0, 3});
         }
         std::vector<int2> brow, bcol;
         fft_blocks(tasks.data(), n, &D, brow, bcol);

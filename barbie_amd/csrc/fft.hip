@@ -313,10 +313,11 @@ __global__ __launch_bounds__(kPassThreads) void k_rowpass(const double2 *__restr
     }
 }
 
-// 8 consecutive points of the result to the cache (and the bf16 operands),
-// as 16-byte stores.
+// 8 consecutive points of the result to the cache (dst, unless null) and
+// the bf16 operands (sp, unless null), as 16-byte stores.
 __device__ __forceinline__ void store8(double2 *dst, __bf16 *, const double2 *v)
 {
+    if (!dst) return;
 #pragma unroll
     for (int t = 0; t < 8; ++t) dst[t] = v[t];
 }
@@ -328,9 +329,12 @@ __device__ __forceinline__ void store8(float2 *dst, __bf16 *sp, const float2 *v)
         re[t] = v[t].x;
         im[t] = v[t].y;
     }
-    float4 *d4 = reinterpret_cast<float4 *>(dst);
+    if (dst) {
+        float4 *d4 = reinterpret_cast<float4 *>(dst);
 #pragma unroll
-    for (int t = 0; t < 4; ++t) d4[t] = make_float4(re[2 * t], im[2 * t], re[2 * t + 1], im[2 * t + 1]);
+        for (int t = 0; t < 4; ++t)
+            d4[t] = make_float4(re[2 * t], im[2 * t], re[2 * t + 1], im[2 * t + 1]);
+    }
     if (sp) {
         // [Re hi x8][Re lo x8][Im hi x8][Im lo x8], hi = bf16(v), lo = bf16(v - hi)
         __bf16 g[32];
@@ -359,8 +363,10 @@ __device__ __forceinline__ void store8(float2 *dst, __bf16 *sp, const float2 *v)
 // set (FP32 caches) also as bf16 hi/lo operands at the same point offsets:
 // for every 8 consecutive points, 32 bf16 values [Re hi x8][Re lo x8]
 // [Im hi x8][Im lo x8], with hi = bf16(v) and lo = bf16(v - hi) of the FP32
-// value v.  Past the field's N*N points (up to a multiple of 8, inside the
-// padded stride) zeros are written.
+// value v.  The task's `forms` selects which of the two are written (the
+// contraction reads a field in one form or both).  Past the field's N*N
+// points (up to a multiple of 8, inside the padded stride) zeros are
+// written.
 template <typename C2>
 __global__ __launch_bounds__(kPassThreads) void k_colpass(C2 *__restrict__ cache,
                                                  __bf16 *__restrict__ split,
@@ -411,6 +417,7 @@ __global__ __launch_bounds__(kPassThreads) void k_colpass(C2 *__restrict__ cache
     __syncthreads();
     C2 *res = lds_fft(a, b, cpb, L.fd, twl);
     const int ngroups = (npt + 7) >> 3;
+    const bool own = T.forms & 1, ops = split && (T.forms & 2);
     for (int g = threadIdx.x; g < ngroups; g += blockDim.x) {
         C2 v[8];
 #pragma unroll
@@ -418,7 +425,7 @@ __global__ __launch_bounds__(kPassThreads) void k_colpass(C2 *__restrict__ cache
             const int x = 8 * g + t;
             v[t] = x < npt ? res[x] : mk<C2>(0, 0);
         }
-        store8(fld + 8 * g, split ? split + (base + 8 * g) * 4 : nullptr, v);
+        store8(own ? fld + 8 * g : nullptr, ops ? split + (base + 8 * g) * 4 : nullptr, v);
     }
 }
 
