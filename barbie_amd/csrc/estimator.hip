@@ -91,41 +91,49 @@ int dev_alloc(T **out, size_t count, std::vector<void *> &owned)
 
 // ---------------------------------------------------------------------------
 // Work lists.  One contraction of a plane group: rings and its index.
-struct CItem { int r1, r2, r3, cid; };
+struct CItem { int r1, r2, r3, cid, pa, pb; };
 
 // A pair block: up to 16 (r1, r2) rows of one plane group whose r3 rings
 // span at most kWinRings, covered by nwin windows starting at lo.
 constexpr int kWinRings = 16 * 3;
+constexpr bool kRepack = true;   // pack the partly filled blocks of a plane set together
 struct PairBlock {
-    int pa, pb, pc;           // planes of D1, D2, D3 (p1, |p2|, |p3|)
+    int pc;                   // plane of D3 (|p3|)
     int flags;                // TW_CONJ2 | TW_CONJ3
+    int pa[16], pb[16];       // per row: planes of D1, D2 (p1, |p2|)
     int r1[16], r2[16];
     int nrow;
     int lo, hi, nwin;
     std::vector<std::array<int, 3>> cs;   // (row, r3, cid)
 };
 
-void pair_blocks(std::vector<CItem> &cs, int pa, int pb, int pc, int flags,
-                 std::vector<PairBlock> &out)
+// cs: contractions of one third plane pc and one conjugation pattern; each
+// carries the planes (pa, pb) of its pair, so that the rows of one block may
+// come from different plane pairs (they share the B operand, plane pc).
+void pair_blocks(std::vector<CItem> &cs, int pc, int flags, std::vector<PairBlock> &out)
 {
     std::sort(cs.begin(), cs.end(), [](const CItem &a, const CItem &b) {
+        if (a.pa != b.pa) return a.pa < b.pa;
+        if (a.pb != b.pb) return a.pb < b.pb;
         if (a.r1 != b.r1) return a.r1 < b.r1;
         if (a.r2 != b.r2) return a.r2 < b.r2;
         return a.r3 < b.r3;
     });
-    struct Pair { int r1, r2, lo, hi; size_t b, e; };
+    struct Pair { int pa, pb, r1, r2, lo, hi; size_t b, e; };
     std::vector<Pair> pairs;
     for (size_t k = 0; k < cs.size();) {
         size_t e = k;
-        while (e < cs.size() && cs[e].r1 == cs[k].r1 && cs[e].r2 == cs[k].r2 &&
-               cs[e].r3 - cs[k].r3 < kWinRings)
+        while (e < cs.size() && cs[e].pa == cs[k].pa && cs[e].pb == cs[k].pb &&
+               cs[e].r1 == cs[k].r1 && cs[e].r2 == cs[k].r2 && cs[e].r3 - cs[k].r3 < kWinRings)
             ++e;
-        pairs.push_back({cs[k].r1, cs[k].r2, cs[k].r3, cs[e - 1].r3, k, e});
+        pairs.push_back({cs[k].pa, cs[k].pb, cs[k].r1, cs[k].r2, cs[k].r3, cs[e - 1].r3, k, e});
         k = e;
     }
     std::sort(pairs.begin(), pairs.end(), [](const Pair &a, const Pair &b) {
         if (a.lo != b.lo) return a.lo < b.lo;
         if (a.hi != b.hi) return a.hi < b.hi;
+        if (a.pa != b.pa) return a.pa < b.pa;
+        if (a.pb != b.pb) return a.pb < b.pb;
         if (a.r1 != b.r1) return a.r1 < b.r1;
         return a.r2 < b.r2;
     });
@@ -140,12 +148,14 @@ void pair_blocks(std::vector<CItem> &cs, int pa, int pb, int pc, int flags,
             ++e;
         }
         PairBlock B;
-        B.pa = pa; B.pb = pb; B.pc = pc; B.flags = flags;
+        B.pc = pc; B.flags = flags;
         B.nrow = (int)(e - q);
         B.lo = lo; B.hi = hi; B.nwin = (hi - lo) / 16 + 1;
-        for (int m = 0; m < 16; ++m) { B.r1[m] = -1; B.r2[m] = -1; }
+        for (int m = 0; m < 16; ++m) { B.r1[m] = -1; B.r2[m] = -1; B.pa[m] = 0; B.pb[m] = 0; }
         for (size_t u = q; u < e; ++u) {
             const int m = (int)(u - q);
+            B.pa[m] = pairs[u].pa;
+            B.pb[m] = pairs[u].pb;
             B.r1[m] = pairs[u].r1;
             B.r2[m] = pairs[u].r2;
             for (size_t k = pairs[u].b; k < pairs[u].e; ++k) B.cs.push_back({m, cs[k].r3, cs[k].cid});
@@ -243,8 +253,8 @@ int pack_items(std::vector<PairBlock> &blocks, const Level &L, bool split_b, int
                 else { it->second.lo = std::min(it->second.lo, lo); it->second.hi = std::max(it->second.hi, hi); }
             };
             for (int m = 0; m < B.nrow; ++m) {
-                add({B.pa, 0}, B.r1[m], B.r1[m]);
-                add({B.pb, 0}, B.r2[m], B.r2[m]);
+                add({B.pa[m], 0}, B.r1[m], B.r1[m]);
+                add({B.pb[m], 0}, B.r2[m], B.r2[m]);
             }
             add({B.pc, split_b ? 1 : 0}, B.lo, B.hi);
             int rows = 0;
@@ -298,15 +308,14 @@ int pack_items(std::vector<PairBlock> &blocks, const Level &L, bool split_b, int
             int j = 0;
             for (size_t bi : wave_blocks[w]) {
                 const PairBlock &B = blocks[bi];
-                const int ra = row0[{B.pa, 0}], rb = row0[{B.pb, 0}];
-                const int rc = row0[{B.pc, split_b ? 1 : 0}];
+                const int rc = row0.at({B.pc, split_b ? 1 : 0});
                 const int tile0 = W.ntw;
                 for (int v = 0; v < B.nwin; ++v) {
                     TWin &T = W.tws[it.tw0 + w * kTWW + j];
                     std::memset(&T, 0, sizeof T);
                     for (int m = 0; m < 16; ++m) {
-                        T.a1[m] = (int16_t)(B.r1[m] >= 0 ? ra + B.r1[m] : -1);
-                        T.a2[m] = (int16_t)(B.r2[m] >= 0 ? rb + B.r2[m] : 0);
+                        T.a1[m] = (int16_t)(B.r1[m] >= 0 ? row0.at({B.pa[m], 0}) + B.r1[m] : -1);
+                        T.a2[m] = (int16_t)(B.r2[m] >= 0 ? row0.at({B.pb[m], 0}) + B.r2[m] : 0);
                     }
                     const int ws = B.lo + 16 * v;
                     T.b0 = (int16_t)(rc + ws);
@@ -683,7 +692,7 @@ int build_layout(const PlanData &d, CPrec prec, const bb_options &opt, int waves
         for (size_t c = 0; c < d.contr.size(); ++c) {
             if (lvl[c] != l) continue;
             const Contraction &k = d.contr[c];
-            groups[{k.p1, k.p2}].push_back({k.r1, k.r2, k.r3, (int)c});
+            groups[{k.p1, k.p2}].push_back({k.r1, k.r2, k.r3, (int)c, k.p1, std::abs(k.p2)});
         }
         std::map<std::array<int, 3>, std::vector<std::pair<int, int>>> by_set;
         for (auto &kv : groups) {
@@ -694,14 +703,30 @@ int build_layout(const PlanData &d, CPrec prec, const bb_options &opt, int waves
         }
         // Mixed layouts: whole plane sets go to the FP32 kernel while its share
         // of the cost (tile windows x points) stays within `mix` percent.
+        // Blocks per plane pair; the rows of a plane pair's last, partly
+        // filled blocks are packed again with those of the other plane pairs
+        // of the set that share the third plane and conjugation pattern (the
+        // B operand), so that fewer A tile rows are empty.
         std::vector<PairBlock> blocks, blocks_f;
         for (auto &kv : by_set) {
             std::vector<PairBlock> sb;
+            std::map<std::pair<int, int>, std::vector<CItem>> rest;
             for (auto &gk : kv.second) {
                 const int p1 = gk.first, p2 = gk.second, p3 = -p1 - p2;
                 const int flags = (p2 < 0 ? TW_CONJ2 : 0) | (p3 < 0 ? TW_CONJ3 : 0);
-                pair_blocks(groups[gk], p1, std::abs(p2), std::abs(p3), flags, sb);
+                std::vector<PairBlock> gb;
+                pair_blocks(groups[gk], std::abs(p3), flags, gb);
+                for (PairBlock &B : gb) {
+                    if (B.nrow == 16 || !kRepack) {
+                        sb.push_back(std::move(B));
+                        continue;
+                    }
+                    std::vector<CItem> &dst = rest[{std::abs(p3), flags}];
+                    for (const auto &c : B.cs)
+                        dst.push_back({B.r1[c[0]], B.r2[c[0]], c[1], c[2], B.pa[c[0]], B.pb[c[0]]});
+                }
             }
+            for (auto &rk : rest) pair_blocks(rk.second, rk.first.first, rk.first.second, sb);
             long long c = 0;
             for (const PairBlock &B : sb) c += B.nwin;
             c *= L.lv[l].npts;
@@ -720,8 +745,8 @@ int build_layout(const PlanData &d, CPrec prec, const bb_options &opt, int waves
             for (int f = 0; f < 2; ++f)
                 for (const PairBlock &B : f ? blocks_f : blocks) {
                     for (int m = 0; m < B.nrow; ++m) {
-                        role[(size_t)B.pa * Nr + B.r1[m]] |= 1;
-                        role[(size_t)B.pb * Nr + B.r2[m]] |= 1;
+                        role[(size_t)B.pa[m] * Nr + B.r1[m]] |= 1;
+                        role[(size_t)B.pb[m] * Nr + B.r2[m]] |= 1;
                     }
                     for (const auto &c : B.cs) role[(size_t)B.pc * Nr + c[1]] |= f ? 1 : 2;
                 }
@@ -979,11 +1004,11 @@ int compute_num(bb_estimator *E)
     L.cache_pts = (long long)Nr * U.stride;
     std::vector<CItem> cs;
     for (int rp = 0; rp < rcount; ++rp)
-        cs.push_back({d.tri[3 * rp], d.tri[3 * rp + 1], d.tri[3 * rp + 2], rp});
+        cs.push_back({d.tri[3 * rp], d.tri[3 * rp + 1], d.tri[3 * rp + 2], rp, 0, 0});
     WorkLists W;
     L.pos.assign(std::max(rcount, 1), 0);
     std::vector<PairBlock> blocks;
-    if (rcount > 0) pair_blocks(cs, 0, 0, 0, 0, blocks);
+    if (rcount > 0) pair_blocks(cs, 0, 0, blocks);
     int rc = pack_items(blocks, U, false, contract_tww(CPrec::F64), kWaves, rowcap_for(kWaves), W,
                         L.pos);
     if (!rc) rc = make_units(W, L.chunk, 0, L.units, L.pos, L.nsp, L.part_tiles, nullptr);
