@@ -353,79 +353,110 @@ __device__ __forceinline__ void store8(float2 *dst, __bf16 *sp, const float2 *v)
     }
 }
 
-// Column pass over the tasks of every level, in place, one block per entry
-// of `blk`: block {t, x} transforms columns [x * cpb, x * cpb + cpb) of task
-// t's field along i.  Those columns are the contiguous points
-// [j0 * N, (j0 + cpb) * N) of the field (column-major); j0 * N and cpb * N
-// are multiples of 8.
-// Rows outside the ring's band were not written by the row pass and are
-// taken as zero.  The result is written 8 points at a time; with `split`
-// set (FP32 caches) also as bf16 hi/lo operands at the same point offsets:
-// for every 8 consecutive points, 32 bf16 values [Re hi x8][Re lo x8]
-// [Im hi x8][Im lo x8], with hi = bf16(v) and lo = bf16(v - hi) of the FP32
-// value v.  The task's `forms` selects which of the two are written (the
-// contraction reads a field in one form or both).  Past the field's N*N
-// points (up to a multiple of 8, inside the padded stride) zeros are
-// written.
+// Column pass over the tasks of every level, in place.  Entry {t, x} of
+// `blk` transforms columns [x * cpb, x * cpb + cpb) of task t's field along
+// i; those columns are the contiguous points [j0 * N, (j0 + cpb) * N) of the
+// field (column-major), j0 * N and cpb * N multiples of 8.  Rows outside the
+// ring's band were not written by the row pass and are taken as zero.
+//
+// The grid is one round of resident workgroups; workgroup g takes entries
+// g, g + G, ...  While it transforms one entry in LDS and stores it, the
+// loads of its next entry are in flight in registers.  Every level's
+// twiddles (ntw values, the table of all levels) stay in LDS for the whole
+// launch.
+//
+// The result is written 8 points at a time; with `split` set (FP32 caches)
+// also as bf16 hi/lo operands at the same point offsets: for every 8
+// consecutive points, 32 bf16 values [Re hi x8][Re lo x8][Im hi x8]
+// [Im lo x8], with hi = bf16(v) and lo = bf16(v - hi) of the FP32 value v.
+// The task's `forms` selects which of the two are written (the contraction
+// reads a field in one form or both).  Past the field's N*N points (up to a
+// multiple of 8, inside the padded stride) zeros are written.
 template <typename C2>
-__global__ __launch_bounds__(kPassThreads) void k_colpass(C2 *__restrict__ cache,
-                                                 __bf16 *__restrict__ split,
-                                                 const FieldTask *__restrict__ tasks,
-                                                 const int2 *__restrict__ blk,
-                                                 const LevelDesc *__restrict__ lv,
-                                                 const double2 *__restrict__ tw_all,
-                                                 BatchStrides bs)
+__device__ __forceinline__ void colpass_load(const C2 *__restrict__ cache,
+                                             const FieldTask *__restrict__ tasks,
+                                             const int2 *__restrict__ blk, int nblk,
+                                             const LevelDesc *__restrict__ lv, int bi,
+                                             C2 (&vals)[kPassPer])
 {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    C2 *lds = reinterpret_cast<C2 *>(lds_raw);
-    const int2 B = blk[blockIdx.x];
-    const int fb = blockIdx.y;   // field of the batch
-    cache += fb * bs.cache;
-    if (split) split += fb * bs.cache * 4;
+#pragma unroll
+    for (int e = 0; e < kPassPer; ++e) vals[e] = mk<C2>(0, 0);
+    if (bi >= nblk) return;
+    const int2 B = blk[bi];
     const FieldTask T = tasks[B.x];
     const LevelDesc &L = lv[T.level];
     const int N = L.fd.N, cpb = L.cpb;
     const int j0 = B.y * cpb;
     if (j0 >= N) return;
     const int nrows = min(N, 2 * T.rowband + 1);
-    const long long base = L.off + (long long)T.fid * L.stride + (long long)j0 * N;
-    const int npt = min(cpb, N - j0) * N;   // this block's points
-    C2 *fld = cache + base;
-    C2 *twl = lds, *a = lds + N, *b = a + cpb * N;
-    for (int t = threadIdx.x; t < N; t += blockDim.x) twl[t] = cvt2<C2>(tw_all[L.tw_off + t]);
+    const C2 *fld = cache + L.off + (long long)T.fid * L.stride + (long long)j0 * N;
+    const int npt = min(cpb, N - j0) * N;
     const unsigned long long mN = L.fd.mN;
-    // layout a[c * N + i] = point idx of the block; all loads issued before
-    // the LDS stores
-    C2 vals[kPassPer];
 #pragma unroll
     for (int e = 0; e < kPassPer; ++e) {
         const int idx = threadIdx.x + e * kPassThreads;
-        C2 v = mk<C2>(0, 0);
         if (idx < npt) {
             const int c = (int)udiv((unsigned)idx, mN), i = idx - c * N;
             const int ia = (i > N / 2) ? (i - N) : i;
             const bool inband = (ia <= T.rowband && -ia <= T.rowband) || nrows == N;
-            if (inband) v = fld[idx];
+            if (inband) vals[e] = fld[idx];
         }
-        vals[e] = v;
     }
+}
+
+template <typename C2>
+__global__ __launch_bounds__(kPassThreads) void k_colpass(C2 *__restrict__ cache,
+                                                 __bf16 *__restrict__ split,
+                                                 const FieldTask *__restrict__ tasks,
+                                                 const int2 *__restrict__ blk, int nblk,
+                                                 const LevelDesc *__restrict__ lv,
+                                                 const double2 *__restrict__ tw_all, int ntw,
+                                                 BatchStrides bs)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    C2 *twl_all = reinterpret_cast<C2 *>(lds_raw);
+    C2 *abuf = twl_all + ((ntw + 1) & ~1);   // 16-byte aligned
+    const int fb = blockIdx.y;   // field of the batch
+    cache += fb * bs.cache;
+    if (split) split += fb * bs.cache * 4;
+    for (int t = threadIdx.x; t < ntw; t += blockDim.x) twl_all[t] = cvt2<C2>(tw_all[t]);
+    C2 cur[kPassPer], nxt[kPassPer];
+    int bi = blockIdx.x;
+    colpass_load<C2>(cache, tasks, blk, nblk, lv, bi, cur);
+    for (; bi < nblk; bi += gridDim.x) {
+        colpass_load<C2>(cache, tasks, blk, nblk, lv, bi + gridDim.x, nxt);
+        const int2 B = blk[bi];
+        const FieldTask T = tasks[B.x];
+        const LevelDesc &L = lv[T.level];
+        const int N = L.fd.N, cpb = L.cpb;
+        const int j0 = B.y * cpb;
+        if (j0 < N) {   // uniform over the workgroup
+            const long long base = L.off + (long long)T.fid * L.stride + (long long)j0 * N;
+            const int npt = min(cpb, N - j0) * N;
+            C2 *fld = cache + base;
+            C2 *a = abuf, *b = abuf + cpb * N;
 #pragma unroll
-    for (int e = 0; e < kPassPer; ++e) {
-        const int idx = threadIdx.x + e * kPassThreads;
-        if (idx < cpb * N) a[idx] = vals[e];
-    }
-    __syncthreads();
-    C2 *res = lds_fft(a, b, cpb, L.fd, twl);
-    const int ngroups = (npt + 7) >> 3;
-    const bool own = T.forms & 1, ops = split && (T.forms & 2);
-    for (int g = threadIdx.x; g < ngroups; g += blockDim.x) {
-        C2 v[8];
+            for (int e = 0; e < kPassPer; ++e) {
+                const int idx = threadIdx.x + e * kPassThreads;
+                if (idx < cpb * N) a[idx] = cur[e];
+            }
+            __syncthreads();
+            C2 *res = lds_fft(a, b, cpb, L.fd, twl_all + L.tw_off);
+            const int ngroups = (npt + 7) >> 3;
+            const bool own = T.forms & 1, ops = split && (T.forms & 2);
+            for (int g = threadIdx.x; g < ngroups; g += blockDim.x) {
+                C2 v[8];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const int x = 8 * g + t;
-            v[t] = x < npt ? res[x] : mk<C2>(0, 0);
+                for (int t = 0; t < 8; ++t) {
+                    const int x = 8 * g + t;
+                    v[t] = x < npt ? res[x] : mk<C2>(0, 0);
+                }
+                store8(own ? fld + 8 * g : nullptr, ops ? split + (base + 8 * g) * 4 : nullptr, v);
+            }
+            __syncthreads();   // the next entry overwrites a and b
         }
-        store8(own ? fld + 8 * g : nullptr, ops ? split + (base + 8 * g) * 4 : nullptr, v);
+#pragma unroll
+        for (int e = 0; e < kPassPer; ++e) cur[e] = nxt[e];
     }
 }
 
@@ -445,6 +476,7 @@ static hipError_t launch_fft(const double2 *planes, int NP, C2 *cache, __bf16 *s
 {
     if (nf < 1 || nf > 65535) return hipErrorInvalidValue;
     size_t lds_r = 0, lds_c = 0;
+    int ntw = 0;   // twiddles of all levels (the table is the levels' back to back)
     for (int l = 0; l < nlev; ++l) {
         const LevelDesc &L = h_lv[l];
         const int N = L.fd.N;
@@ -454,8 +486,10 @@ static hipError_t launch_fft(const double2 *planes, int NP, C2 *cache, __bf16 *s
             N >= 4096 || L.rpb * N > kPassThreads * kPassPer || L.cpb * N > kPassThreads * kPassPer)
             return hipErrorInvalidValue;
         lds_r = std::max(lds_r, (2 * (size_t)L.rpb + 1) * N * sizeof(C2));
-        lds_c = std::max(lds_c, (2 * (size_t)L.cpb + 1) * N * sizeof(C2));
+        lds_c = std::max(lds_c, 2 * (size_t)L.cpb * N * sizeof(C2));
+        ntw = std::max(ntw, L.tw_off + N);
     }
+    lds_c += (size_t)((ntw + 1) & ~1) * sizeof(C2);
     if (fb.nrow > 0) {
         (void)hipFuncSetAttribute((const void *)k_rowpass<C2>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_r);
@@ -467,8 +501,17 @@ static hipError_t launch_fft(const double2 *planes, int NP, C2 *cache, __bf16 *s
     if (fb.ncol > 0) {
         (void)hipFuncSetAttribute((const void *)k_colpass<C2>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_c);
-        hipLaunchKernelGGL(k_colpass<C2>, dim3(fb.ncol, nf), dim3(kPassThreads), lds_c, s, cache, split,
-                           tasks, fb.col, d_lv, tw, bs);
+        // One round of resident workgroups, shared by the fields of the batch.
+        int dev = 0, ncu = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_colpass<C2>, kPassThreads,
+                                                         lds_c) != hipSuccess)
+            return hipErrorInvalidValue;
+        const long long round = (long long)std::max(1, ncu) * std::max(1, per_cu);
+        const int gx = (int)std::max(1LL, std::min<long long>(fb.ncol, round / nf));
+        hipLaunchKernelGGL(k_colpass<C2>, dim3(gx, nf), dim3(kPassThreads), lds_c, s, cache, split,
+                           tasks, fb.col, fb.ncol, d_lv, tw, ntw, bs);
     }
     return hipGetLastError();
 }
