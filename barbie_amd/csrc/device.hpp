@@ -197,7 +197,7 @@ hipError_t launch_contract(CPrec p, int waves, int tww, const void *cache, const
 // Records (sorted by bin) are split into segments of at most kBinSeg records
 // within one bin: sg[s]..sg[s+1] are segment s's records, bin_sg[b]..
 // bin_sg[b+1] the segments of bin b.  binpart holds 48 sums per segment.
-constexpr int kBinSeg = 1024;
+constexpr int kBinSeg = 2048;
 hipError_t launch_binning(const int32_t *sg, int nseg, const int32_t *bin_sg, const BinRec *recs,
                           const double *S, const double *num, const int *nzflag, const double *avg,
                           const BinParams &bp, double *binpart, double *bins_bs,
