@@ -231,6 +231,41 @@ def test_cli_end_to_end(tmp_path):
         "Total time")
 
 
+def test_cli_split_precision_with_checked_denominators(tmp_path):
+    """`barbie` with BARBIE_PRECISION=bf16x3 and BARBIE_CHECK_DENOMINATORS=1:
+    the FP64 dump within the BF16x3 bar (1e-4 per multipole, norm-wise) of
+    the oracle, Num to 1e-12, and in the 16 files exactly the written rows'
+    multipoles whose own oracle denominator is not positive read "nan"."""
+    o = oracle_run(medium, NRi=1)
+    g = o["g"]
+    B.write_field(str(tmp_path / f"field_N{g.NN}_R3.txt"), o["field"])
+    env = dict(os.environ, BARBIE_NMAX=str(g.Nmax), BARBIE_NR=str(g.Nr),
+               BARBIE_PRECISION="bf16x3", BARBIE_CHECK_DENOMINATORS="1")
+    r = subprocess.run([B.BIN_PATH, str(g.NN), "3", str(g.UL_k1), str(g.LL_k1), str(g.BW_k1)],
+                       cwd=tmp_path, env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    outdir = tmp_path / f"ResFiles_N{g.NN}"
+    dump = np.fromfile(outdir / f"BS_bins_N{g.NN}_R3.f64").reshape(3, -1, 16)
+    obs, onm = o["bs"].reshape(-1, 16), o["nm"].reshape(-1, 16)
+    for ii in range(16):
+        assert np.linalg.norm(dump[0][:, ii] - obs[:, ii]) <= 1e-4 * np.linalg.norm(obs[:, ii])
+    assert rel_err(dump[1], onm) <= 1e-12
+    rows = onm[:, 0] > .1
+    expect = 0
+    for ii in range(16):
+        lines = (outdir / f"BS_test_ii{ii}_N{g.NN}_R3.txt").read_text().splitlines()
+        assert len(lines) == int(rows.sum())
+        bad = ~(onm[rows, ii] > 0)
+        expect += int(bad.sum())
+        for line, isbad in zip(lines, bad):
+            f = line.split("\t")
+            assert (f[3] == "nan" and f[4] == "nan") == bool(isbad), line
+    if expect:
+        assert f"{expect} (bin, multipole) entries" in r.stderr
+    else:
+        assert "nan" not in r.stderr
+
+
 def test_tuned_layout_matches_oracle():
     """bb_tune picks among layouts that all give the reference's sums."""
     o = oracle_run(medium, NRi=1)
