@@ -1,6 +1,8 @@
-"""Properties of the estimator at the reference configuration (NN=384,
-Nmax=100, Nr=40, UL_k1=51; barbie.c:61-84), where the CPU oracle would take
-about an hour per field.  Each follows from the estimator's definition
+"""Properties of the estimator at full size, where the CPU oracle would take
+an hour or more per field: the reference configuration (NN=384, Nmax=100,
+Nr=40, UL_k1=51; barbie.c:61-84) and the largest served one (NN=512,
+Nmax=134, Nr=54 uniform rings, UL_k1=73).  Each follows from the
+estimator's definition
 (S = sum_x Re D1 D2 D3, barbie.c:326-337, over closed triangles of
 isotropic rings) and holds at any size:
 
@@ -32,23 +34,29 @@ import barbie_amd as B
 
 pytestmark = pytest.mark.gpu
 
-NN = 384
+GEOMS = {
+    "reference384": dict(NN=384, UL_k1=51, LL_k1=1, BW_k1=1),
+    "uniform512": dict(NN=512, UL_k1=73, LL_k1=1, BW_k1=1, Nmax=134, Nr=54, n_mtbin=5,
+                       ring_rule=B.RINGS_UNIFORM, ring_width=134 / 54, last_ring_to_nmax=1),
+}
 _EST = {}
 
 
-def estimator(prec):
-    if prec not in _EST:
-        g = B.Geometry(NN=NN, UL_k1=51, LL_k1=1, BW_k1=1)
-        opts = dict(grid_levels=4, waves=4, occupancy=3) if prec == "bf16x3" else {}
-        _EST[prec] = (g, B.Estimator(B.Plan(g), prec, **opts))
-    return _EST[prec]
+def estimator(geom, prec):
+    if (geom, prec) not in _EST:
+        g = B.Geometry(**GEOMS[geom])
+        # the tuned BF16x3 layout of the reference geometry; defaults otherwise
+        opts = (dict(grid_levels=4, waves=4, occupancy=3)
+                if prec == "bf16x3" and geom == "reference384" else {})
+        _EST[(geom, prec)] = (g, B.Estimator(B.Plan(g), prec, **opts))
+    return _EST[(geom, prec)]
 
 
-def bins(prec, d):
+def bins(case, prec, d):
     """The three bin arrays of the field tensor d (its producer is waited
     for: the estimator reads it on its own stream)."""
     import torch
-    g, est = estimator(prec)
+    g, est = estimator(case[0], prec)
     nb = g.nbins * 16
     out = torch.zeros(3 * nb, dtype=torch.float64, device="cuda")
     p = out.data_ptr()
@@ -58,14 +66,20 @@ def bins(prec, d):
     return o[:nb].reshape(-1, 16), o[nb:2 * nb].reshape(-1, 16), o[2 * nb:].reshape(-1, 16)
 
 
-@pytest.fixture(scope="module")
-def field():
-    return B.make_field_device(NN, NRi=3, fnl=0.3)
+@pytest.fixture(scope="module", params=list(GEOMS))
+def case(request):
+    """(geometry name, device field); the previous geometry's estimators are
+    released first."""
+    for k in [k for k in _EST if k[0] != request.param]:
+        _EST.pop(k)[1].close()
+    NN = GEOMS[request.param]["NN"]
+    return request.param, B.make_field_device(NN, NRi=3, fnl=0.3)
 
 
 def shifted_x(d):
     """The r2c buffer of the real-space field moved by one cell along x."""
     import torch
+    NN = d.shape[0]
     i = torch.arange(NN, dtype=torch.float64, device=d.device)
     ph = -2.0 * math.pi * i / NN
     c, s = torch.cos(ph)[:, None, None], torch.sin(ph)[:, None, None]
@@ -86,10 +100,11 @@ def per_multipole(a, b):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
-def test_bins_are_cubic_in_the_field(field, prec):
-    a = bins(prec, field)
-    b = bins(prec, 2.0 * field)
-    n = bins(prec, -field)
+def test_bins_are_cubic_in_the_field(case, prec):
+    field = case[1]
+    a = bins(case, prec, field)
+    b = bins(case, prec, 2.0 * field)
+    n = bins(case, prec, -field)
     assert np.abs(a[0]).max() > 0
     if prec == "fp64":
         assert np.array_equal(b[0], 8.0 * a[0])
@@ -103,27 +118,31 @@ def test_bins_are_cubic_in_the_field(field, prec):
         assert np.array_equal(b[k], a[k]) and np.array_equal(n[k], a[k])
 
 
-def test_bins_are_translation_invariant(field):
-    a = bins("fp64", field)
-    t = bins("fp64", shifted_x(field))
+def test_bins_are_translation_invariant(case):
+    field = case[1]
+    a = bins(case, "fp64", field)
+    t = bins(case, "fp64", shifted_x(field))
     for k in range(3):
         scale = np.abs(a[k]).max()
         assert np.abs(t[k] - a[k]).max() <= 1e-10 * scale, k
-    tb = bins("bf16x3", shifted_x(field))
+    tb = bins(case, "bf16x3", shifted_x(field))
     assert max(per_multipole(tb[0], a[0])) <= 1e-5
     assert np.abs(tb[1] - a[1]).max() <= 1e-12 * np.abs(a[1]).max()
 
 
-def test_bins_are_symmetric_in_x_and_y(field):
-    a = bins("fp64", field)
-    s = bins("fp64", swapped_xy(field))
+def test_bins_are_symmetric_in_x_and_y(case):
+    field = case[1]
+    a = bins(case, "fp64", field)
+    s = bins(case, "fp64", swapped_xy(field))
     for k in range(3):
         scale = np.abs(a[k]).max()
         assert np.abs(s[k] - a[k]).max() <= 1e-10 * scale, k
 
 
-def test_power_multipoles_scale_and_symmetry(field):
+def test_power_multipoles_scale_and_symmetry(case):
     import torch
+    field = case[1]
+    NN = field.shape[0]
     doubled, swapped = 2.0 * field, swapped_xy(field)
     torch.cuda.synchronize()   # the library reads device fields on its own stream
     a = B.power_multipoles(field.data_ptr(), NN)
