@@ -21,7 +21,8 @@
 //            split once, by the FFT pass), FP32 accumulation over kFlush
 //            chunks, FP64 across.
 // Rows of a tile whose pair slot is empty and columns past the window's end
-// read arbitrary staged rings; MFMA keeps rows and columns independent, and
+// read arbitrary staged rings in the FP64 and FP32 kernels and a ring of
+// zeros in the BF16X3 kernel; MFMA keeps rows and columns independent, and
 // the outputs of those slots are never read.
 //
 // Lane layouts (checked on gfx950): for the K=4 forms lane l holds
@@ -30,10 +31,12 @@
 // register j is D[4j + l/16][l%16] for f64 and D[4(l/16) + j][l%16] for the
 // f32-accumulating forms.
 //
-// Staging is software-pipelined: each thread holds its share of the next
-// chunk in registers while the waves work on the current one from LDS.  Every
-// staged ring is 256 bytes of one chunk (plus 16 bytes of padding, so that 16
-// lanes reading 16 different rings at one offset hit distinct banks).
+// Staging is software-pipelined.  FP64 and FP32 kernels: each thread holds
+// its share of the next chunk in registers while the waves work on the
+// current one from LDS; every staged ring is 256 bytes of one chunk (plus 16
+// bytes of padding, so that 16 lanes reading 16 different rings at one offset
+// hit distinct banks).  BF16X3 kernel: LDS-direct loads fill a second buffer
+// while the waves work on the first (DmaStager, dma_off).
 #include <hip/hip_runtime.h>
 
 #include "device.hpp"
