@@ -12,8 +12,10 @@
 //    row, apply the ring mask (integer radius bounds), transform along j,
 //    write the row into the field's cache slot;
 //  - column pass: transform along i in place (rows outside the ring's band
-//    are zero), write the field to the cache (and, for FP32 caches, the
-//    bf16 hi/lo operands).
+//    are zero), write the field to the cache and, for FP32 caches, as bf16
+//    hi/lo operands, each in the forms the contraction reads it in; one
+//    round of workgroups loops over the column groups, loading the next
+//    group during the current one's transform.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
