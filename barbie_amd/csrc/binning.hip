@@ -70,6 +70,31 @@ __device__ __forceinline__ void cal_coeff(double px, double pz, double *bs, doub
 
 constexpr int kBinThreads = 256;   // four waves per bin
 
+// v of the lane that the DPP control CTRL selects (within a row of 16
+// lanes), for a double: both halves moved.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the wave, the same bits in every lane: pairwise sums within
+// pairs, quads (quad permutes), half rows and rows (mirrors) by DPP, then
+// across the four rows by two lane exchanges.  Each step adds a lane's value
+// and its partner's, so both end with the same sum.
+__device__ __forceinline__ double wave_sum(double v)
+{
+    v += dpp_f64<0xB1>(v);    // quad_perm [1, 0, 3, 2]
+    v += dpp_f64<0x4E>(v);    // quad_perm [2, 3, 0, 1]
+    v += dpp_f64<0x141>(v);   // row_half_mirror
+    v += dpp_f64<0x140>(v);   // row_mirror
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
 __global__ __launch_bounds__(kBinThreads) void k_binning(const int32_t *__restrict__ sg,
                                                 const BinRec *__restrict__ recs,
                                                 const double *__restrict__ S,
@@ -146,17 +171,11 @@ __global__ __launch_bounds__(kBinThreads) void k_binning(const int32_t *__restri
             anm[k] += Num * nm[k];
         }
     }
-    // Wave sums (butterfly), then the four waves' sums in wave order.
+    // Wave sums, then the four waves' sums in wave order.
     __shared__ double part[kBinThreads / 64][48];
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-        double v0 = abs_[k], v1 = anm[k], v2 = abi[k];
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            v0 += __shfl_xor(v0, off, 64);
-            v1 += __shfl_xor(v1, off, 64);
-            v2 += __shfl_xor(v2, off, 64);
-        }
+        const double v0 = wave_sum(abs_[k]), v1 = wave_sum(anm[k]), v2 = wave_sum(abi[k]);
         if (lane == 0) {
             part[wave][k] = v0;
             part[wave][16 + k] = v1;
