@@ -70,30 +70,6 @@ __device__ __forceinline__ void cal_coeff(double px, double pz, double *bs, doub
 
 constexpr int kBinThreads = 256;   // four waves per bin
 
-// v of the lane that the DPP control CTRL selects (within a row of 16
-// lanes), for a double: both halves moved.
-template <int CTRL>
-__device__ __forceinline__ double dpp_f64(double v)
-{
-    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
-    return __hiloint2double(hi, lo);
-}
-
-// Sum over the wave, the same bits in every lane: pairwise sums within
-// pairs, quads (quad permutes), half rows and rows (mirrors) by DPP, then
-// across the four rows by two lane exchanges.  Each step adds a lane's value
-// and its partner's, so both end with the same sum.
-__device__ __forceinline__ double wave_sum(double v)
-{
-    v += dpp_f64<0xB1>(v);    // quad_perm [1, 0, 3, 2]
-    v += dpp_f64<0x4E>(v);    // quad_perm [2, 3, 0, 1]
-    v += dpp_f64<0x141>(v);   // row_half_mirror
-    v += dpp_f64<0x140>(v);   // row_mirror
-    v += __shfl_xor(v, 16, 64);
-    v += __shfl_xor(v, 32, 64);
-    return v;
-}
 
 __global__ __launch_bounds__(kBinThreads) void k_binning(const int32_t *__restrict__ sg,
                                                 const BinRec *__restrict__ recs,

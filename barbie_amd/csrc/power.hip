@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "barbie.h"
+#include "device.hpp"
 #include "plan.hpp"
 
 namespace {
@@ -207,27 +208,22 @@ __global__ __launch_bounds__(kPsThreads) void k_power(const double2 *__restrict_
             probe(t + (2 + slot) * stride, q);
         }
     }
-    // Workgroup sums, a bin at a time: a fixed tree over the threads.
-    __shared__ double red[kPsThreads][NA];
-    for (int s = 0; s < P.Nbin; ++s) {
+    // Workgroup sums: each wave's (DPP within rows, lane exchanges across
+    // rows; the same bits in every lane), then the waves' in wave order.
+    __shared__ double red[kPsThreads / 64][kPsMaxBins * NA];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int s = 0; s < kPsMaxBins; ++s)
 #pragma unroll
         for (int q = 0; q < NA; ++q) {
-            double v = 0.0;
-#pragma unroll
-            for (int u = 0; u < kPsMaxBins; ++u)
-                if (u == s) v = acc[u][q];
-            red[threadIdx.x][q] = v;
+            const double v = bb::wave_sum(acc[s][q]);
+            if (lane == 0) red[wave][s * NA + q] = v;
         }
-        __syncthreads();
-        for (int w = kPsThreads / 2; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w)
-#pragma unroll
-                for (int q = 0; q < NA; ++q) red[threadIdx.x][q] += red[threadIdx.x + w][q];
-            __syncthreads();
-        }
-        if (threadIdx.x < NA)
-            part[((long long)blockIdx.x * P.Nbin + s) * NA + threadIdx.x] = red[0][threadIdx.x];
-        __syncthreads();
+    __syncthreads();
+    for (int t = threadIdx.x; t < P.Nbin * NA; t += kPsThreads) {
+        double v = red[0][t];
+        for (int w = 1; w < kPsThreads / 64; ++w) v += red[w][t];
+        part[(long long)blockIdx.x * P.Nbin * NA + t] = v;
     }
 }
 
