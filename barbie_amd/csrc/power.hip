@@ -12,9 +12,9 @@
 // consecutive threads along k.  Triples whose integer a^2 + b^2 + k^2 lies
 // beyond the last bin's edge (about half of the octant) are skipped before
 // any floating-point work.  Each thread keeps (a, b, k) by carries instead
-// of divisions, accumulates per bin in registers (branches per bin),
-// workgroups reduce in a fixed tree order, and a second kernel adds the
-// workgroups by a fixed tree (deterministic).
+// of divisions and accumulates per bin in registers (branches per bin);
+// workgroups reduce by wave sums (DPP) added in wave order, and a second
+// kernel adds the workgroups the same way (deterministic).
 //
 // The per-mode arithmetic keeps the reference's types and roundings: m, mu
 // and the P_l are `float`, formed as C evaluates the reference's expressions
@@ -228,22 +228,24 @@ __global__ __launch_bounds__(kPsThreads) void k_power(const double2 *__restrict_
 }
 
 // Sum over workgroups: one workgroup per (bin, accumulator); thread t adds
-// the partials t, t + 256, ... in order, then a fixed tree over the threads.
+// the partials t, t + 256, ... in order, then wave sums and the waves' in
+// order.
 __global__ __launch_bounds__(kPsThreads) void k_power_sum(const double *__restrict__ part,
                                                           int nblocks, int n,
                                                           double *__restrict__ out)
 {
-    __shared__ double red[kPsThreads];
+    __shared__ double red[kPsThreads / 64];
     const int o = blockIdx.x, t = threadIdx.x;
     double v = 0.0;
     for (int g = t; g < nblocks; g += kPsThreads) v += part[(long long)g * n + o];
-    red[t] = v;
+    v = bb::wave_sum(v);
+    if ((t & 63) == 0) red[t >> 6] = v;
     __syncthreads();
-    for (int w = kPsThreads / 2; w > 0; w >>= 1) {
-        if (t < w) red[t] += red[t + w];
-        __syncthreads();
+    if (t == 0) {
+        double sum = red[0];
+        for (int w = 1; w < kPsThreads / 64; ++w) sum += red[w];
+        out[o] = sum;
     }
-    if (t == 0) out[o] = red[0];
 }
 
 // Per-device partial-sum buffers and stream, made on a device's first call
