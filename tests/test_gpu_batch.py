@@ -99,3 +99,33 @@ def test_graph_replays_equal_eager(prec):
     for rep in range(3):
         bs, nm, bi = host.estimate(hf)
         assert np.array_equal(bs.ravel(), ref[1][0]) and np.array_equal(nm.ravel(), ref[1][1])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+@pytest.mark.parametrize("NN", [192, 256])
+def test_batch_equals_single_at_batching_sizes(NN, prec):
+    """The batching workload's geometries (Nmax = NN/4, Nr = round(NN/10),
+    UL = floor(NN/7)): four device fields in one batch get the bins each
+    gets alone, bit for bit."""
+    import torch
+    g = B.Geometry(NN=NN, UL_k1=NN // 7, LL_k1=1, BW_k1=1, Nmax=NN // 4,
+                   Nr=int(round(NN / 10)), n_mtbin=5)
+    plan = B.Plan(g)
+    nb = g.nbins * 16
+    fields = torch.stack([B.make_field_device(NN, NRi=k, fnl=0.3) for k in range(1, 5)])
+    out = torch.zeros(3, 4 * nb, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    batched = B.Estimator(plan, prec, max_batch=4)
+    batched.estimate_batch_device(4, fields.data_ptr(), out[0].data_ptr(), out[1].data_ptr(),
+                                  out[2].data_ptr())
+    got = out.cpu().numpy().reshape(3, 4, nb)
+    single = B.Estimator(plan, prec)
+    one = torch.zeros(3 * nb, dtype=torch.float64, device="cuda")
+    p = one.data_ptr()
+    for k in range(4):
+        torch.cuda.synchronize()
+        single.estimate_device(fields[k].data_ptr(), p, p + 8 * nb, p + 16 * nb)
+        alone = one.cpu().numpy().reshape(3, nb)
+        for a in range(3):
+            assert np.array_equal(got[a, k], alone[a]), (k, a)
+    assert np.abs(got[0]).max() > 0
