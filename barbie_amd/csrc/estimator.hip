@@ -1780,9 +1780,9 @@ int bb_estimate_batch_async(bb_estimator *e, int nfields, const double *d_fields
                             double *d_norm, double *d_biana)
 {
     if (!e || !d_fields) return set_error(BB_E_INVALID, "null argument");
-    int rc = estimate_finish(e);
-    if (rc) return rc;
-    rc = estimate_enqueue(e, nfields, d_fields, d_bs, d_norm, d_biana);
+    // A pending batch is not waited for: the new one runs after it on the
+    // estimator's stream (the stage events then describe the newest).
+    const int rc = estimate_enqueue(e, nfields, d_fields, d_bs, d_norm, d_biana);
     if (rc) {
         (void)hipStreamSynchronize(e->stream);
         e->pending = false;

@@ -206,8 +206,10 @@ int bb_estimate_batch_device(bb_estimator *e, int nfields, const double *d_field
                              double *d_bin_bs, double *d_bin_norm, double *d_bin_biana);
 /* Asynchronous form (device pointers): enqueues the batch on the estimator's
  * stream and returns; bb_estimator_query sets *done without blocking,
- * bb_estimator_wait blocks.  Device buffers must stay valid until done.  Any
- * other call on the estimator waits for a pending batch first. */
+ * bb_estimator_wait blocks.  Device buffers must stay valid until done.
+ * Batches enqueued while others are pending run after them in order (the
+ * estimator's stage times then describe the last one); any other call on
+ * the estimator waits for all pending batches first. */
 int bb_estimate_batch_async(bb_estimator *e, int nfields, const double *d_fields_r2c,
                             double *d_bin_bs, double *d_bin_norm, double *d_bin_biana);
 int bb_estimator_query(bb_estimator *e, int *done);

@@ -58,6 +58,33 @@ def test_async_batch_matches_sync():
         assert np.array_equal(got[k], r.reshape(2, nb))
 
 
+def test_async_batches_queue_in_order():
+    """Asynchronous batches enqueued back to back, without waiting, run in
+    order on the estimator's stream: each output equals the synchronous
+    result of its own field (bit for bit), including a batch that reuses the
+    input buffer of an earlier one after it is overwritten on the host's
+    stream only once everything has finished."""
+    torch = pytest.importorskip("torch")
+    g = config1()
+    plan = B.Plan(g)
+    fields = np.stack([B.make_field(g.NN, NRi=k) for k in (7, 8, 9)])
+    single = B.Estimator(plan, "bf16x3")
+    ref = [single.estimate(f) for f in fields]
+    est = B.Estimator(plan, "bf16x3")
+    d_f = torch.from_numpy(fields).to("cuda")
+    nb = g.nbins * 16
+    out = torch.zeros(3, 3, nb, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    for k in range(3):
+        est.estimate_batch_device(1, d_f[k].data_ptr(), out[0, k].data_ptr(),
+                                  out[1, k].data_ptr(), out[2, k].data_ptr(), wait=False)
+    est.wait()
+    got = out.cpu().numpy()
+    for k in range(3):
+        for a in range(3):
+            assert np.array_equal(got[a, k], ref[k][a].reshape(nb)), (k, a)
+
+
 @pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
 def test_graph_replays_equal_eager(prec):
     """use_graph: the first call runs eagerly and captures the launch
