@@ -140,8 +140,10 @@ struct BinParams {
 
 // Launchers (fft.hip, contract.hip, binning.hip).
 // nf fields of NN * NN * (NN/2+1) complex values each, back to back.
+// zero (may be null): nzero ints the gather also sets to 0 (the fields'
+// nonzero flags, which the row pass then raises).
 hipError_t launch_gather(const double2 *fields, double2 *planes, int NN, int NG, int kmax,
-                         int Nmax, int nf, hipStream_t s);
+                         int Nmax, int nf, hipStream_t s, int *zero = nullptr, int nzero = 0);
 // Per-field strides of a batched launch (blockIdx.y, or blockIdx.z for the
 // gather, selects the field): elements between consecutive fields' buffers.
 struct BatchStrides {

@@ -29,12 +29,6 @@ using namespace bb;
 
 namespace {
 
-__global__ void k_fill_int(int *p, int n, int v)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
 bool make_fft_desc(int N, FftDesc &fd)
 {
     fd.N = N;
@@ -1394,10 +1388,9 @@ int enqueue_kernels(bb_estimator *E, int nf, const double *d_fields, double *d_b
     const Layout &L = *E->L;
     hipStream_t s = E->stream;
     const int nz = d.g.Nmax * d.g.Nr * nf;
-    hipLaunchKernelGGL(k_fill_int, dim3((nz + 255) / 256), dim3(256), 0, s, E->d_nz, nz, 0);
-    HIPCHK(hipGetLastError());
+    // The gather also clears the nonzero flags that the row pass raises.
     HIPCHK(launch_gather((const double2 *)d_fields, E->d_planes, L.NN, L.NGtop, L.kmax,
-                         d.g.Nmax, nf, s));
+                         d.g.Nmax, nf, s, E->d_nz, nz));
     if (timed) HIPCHK(hipEventRecord(E->ev[1], s));
     int rc = run_fields(E, E->d_planes, L.NGtop, E->d_nz, nf);
     if (rc) return rc;

@@ -173,9 +173,16 @@ __device__ V2 *lds_fft(V2 *a, V2 *b, int nb, const FftDesc &fd, const V2 *tw)
 // LDS keep both the read (along p) and the write (along j) coalesced.
 __global__ __launch_bounds__(256) void k_gather(const double2 *__restrict__ fields,
                                                 double2 *__restrict__ planes_all, int NN, int NG,
-                                                int kmax, int Nmax)
+                                                int kmax, int Nmax, int *__restrict__ zero,
+                                                int nzero)
 {
     __shared__ double2 tile[32][33];
+    if (zero) {   // the flags to clear, one thread each over the whole grid
+        const long long gid =
+            ((long long)(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * 256 +
+            threadIdx.x;
+        if (gid < nzero) zero[gid] = 0;
+    }
     const int H = NN / 2 + 1;
     const int f = blockIdx.z / NG, i = blockIdx.z - f * NG;   // field of the batch, row
     const double2 *field = fields + (long long)f * NN * NN * H;
@@ -204,11 +211,13 @@ __global__ __launch_bounds__(256) void k_gather(const double2 *__restrict__ fiel
 }
 
 hipError_t launch_gather(const double2 *fields, double2 *planes, int NN, int NG, int kmax,
-                         int Nmax, int nf, hipStream_t s)
+                         int Nmax, int nf, hipStream_t s, int *zero, int nzero)
 {
     if ((long long)NG * nf > 65535) return hipErrorInvalidValue;
     dim3 grid((NG + 31) / 32, (Nmax + 31) / 32, NG * nf);
-    hipLaunchKernelGGL(k_gather, grid, dim3(256), 0, s, fields, planes, NN, NG, kmax, Nmax);
+    if ((long long)grid.x * grid.y * grid.z * 256 < nzero) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gather, grid, dim3(256), 0, s, fields, planes, NN, NG, kmax, Nmax, zero,
+                       nzero);
     return hipGetLastError();
 }
 
