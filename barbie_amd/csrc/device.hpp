@@ -5,6 +5,8 @@
 #include <cstdint>
 #include <vector>
 
+#include "barbie.h"
+
 namespace bb {
 
 constexpr int kMaxStages = 16;
@@ -230,5 +232,10 @@ __device__ __forceinline__ double wave_sum(double v)
     v += __shfl_xor(v, 32, 64);
     return v;
 }
+
+// bb_tune with a switch for the mixed BF16x3 + FP32 candidates (allow_mix
+// false: BF16x3 candidates use the BF16x3 kernel only).
+int tune_layout(const bb_plan *p, const bb_options *base, const double *d_field, int repeats,
+                bool allow_mix, bb_options *best, double *best_ms);
 
 }  // namespace bb

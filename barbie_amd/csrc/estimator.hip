@@ -434,6 +434,12 @@ int check_lists(const Layout &L, size_t ncontr)
             seen[q] = 1;
         }
     std::vector<int> item_sp(L.items.size(), 0);
+    // Tile windows per wave the kernel that runs an item executes: the
+    // layout's for its own units, the FP32 kernel's for units_f.
+    std::vector<int> item_tww(L.items.size(), L.tww);
+    for (const WorkUnit &u : L.units_f)
+        if (u.item >= 0 && (size_t)u.item < L.items.size())
+            item_tww[u.item] = std::min(item_tww[u.item], contract_tww(CPrec::F32));
     std::vector<WorkUnit> all(L.units);
     all.insert(all.end(), L.units_f.begin(), L.units_f.end());
     for (const WorkUnit &u : all) {
@@ -462,7 +468,8 @@ int check_lists(const Layout &L, size_t ncontr)
         if (rows != it.nrows || it.nrows > kRowCap || it.nblk > kMaxBlk || item_sp[i] < 1)
             return set_error(BB_E_INVALID, "internal: work item row blocks are inconsistent");
         for (int w = 0; w < kWaves; ++w) {
-            if (it.cnt[w] < 0 || it.cnt[w] > kTWW)
+            const int limit = w < L.waves ? std::min(item_tww[i], (int)kTWW) : 0;
+            if (it.cnt[w] < 0 || it.cnt[w] > limit)
                 return set_error(BB_E_INVALID, "internal: too many tile windows in a wave");
             for (int j = it.cnt[w]; j < kTWW; ++j)
                 if (L.tws[(size_t)it.tw0 + w * kTWW + j].out != -1)
@@ -963,8 +970,12 @@ int run_fields(bb_estimator *E, const double2 *planes, int NP, int *nz, int nf)
 }
 
 // Triangle counts Num on the GPU (calCR, funcs.c:328-424): unit fields
-// I_r = IFFT2(mask_r) in FP64 on the top grid, then Num = sum_x I1 I2 I3 for
+// I_r = IFFT2(mask_r) in FP64 on the NN grid, then Num = sum_x I1 I2 I3 for
 // every ring triple with the FP64 contraction kernel.  Cached in the plan.
+// The NN grid, as in the reference, and not the estimator's grid levels:
+// calCR counts every ring triple, also those of rings beyond UL_k1 that no
+// contraction reads, and for those a smaller grid would alias (their
+// K1 + K2 + K3 exceeds the levels' sizes).
 int compute_num(bb_estimator *E)
 {
     bb_plan *mp = const_cast<bb_plan *>(E->plan);
@@ -976,7 +987,7 @@ int compute_num(bb_estimator *E)
         }
     }
     const PlanData &d = E->plan->d;
-    const int Nr = d.g.Nr, NN = d.g.NN, NG = E->L->NGtop;
+    const int Nr = d.g.Nr, NN = d.g.NN, NG = NN;
     const int rcount = (int)(d.tri.size() / 3);
     const std::vector<int> K = ring_kx(d);
     Level U;
@@ -995,6 +1006,7 @@ int compute_num(bb_estimator *E)
     }
     Layout L;
     L.chunk = contract_chunk(CPrec::F64);
+    L.tww = contract_tww(CPrec::F64);
     L.cache_pts = (long long)Nr * U.stride;
     std::vector<CItem> cs;
     for (int rp = 0; rp < rcount; ++rp)
@@ -1604,6 +1616,16 @@ int bb_plan_grid_levels(const bb_plan *p, const bb_options *opt, int *nlevels, i
 int bb_tune(const bb_plan *p, const bb_options *base, const double *d_field, int repeats,
             bb_options *best, double *best_ms)
 {
+    return bb::tune_layout(p, base, d_field, repeats, true, best, best_ms);
+}
+
+}  // extern "C"
+
+// bb_tune; allow_mix = false keeps BF16x3 candidates to the BF16x3 kernel
+// alone (mix_fp32 = 0), for callers whose streams are CU-masked.
+int bb::tune_layout(const bb_plan *p, const bb_options *base, const double *d_field, int repeats,
+                    bool allow_mix, bb_options *best, double *best_ms)
+{
     if (!p || !base || !best) return set_error(BB_E_INVALID, "null argument");
     CPrec cp;
     int rc = precision_of(base->precision, cp);
@@ -1635,7 +1657,7 @@ int bb_tune(const bb_plan *p, const bb_options *base, const double *d_field, int
     for (int lev : (cp == CPrec::BF16X3 ? std::vector<int>{4, 6} : std::vector<int>{4, 5, 6, 8}))
         if (base->grid_levels <= 0 || lev <= base->grid_levels) levs.push_back(lev);
     if (levs.empty()) levs.push_back(base->grid_levels);
-    if (cp == CPrec::BF16X3) mixes = {0, 30, 45};
+    if (cp == CPrec::BF16X3) mixes = allow_mix ? std::vector<int>{0, 30, 45} : std::vector<int>{0};
     // (waves, occupancy): occupancy 3 exists for BF16x3 with 4 waves only.
     std::vector<std::pair<int, int>> shapes{{8, 0}, {4, 0}};
     if (cp == CPrec::BF16X3) shapes.push_back({4, 3});
@@ -1672,6 +1694,8 @@ int bb_tune(const bb_plan *p, const bb_options *base, const double *d_field, int
     if (best_ms) *best_ms = best_t;
     return BB_OK;
 }
+
+extern "C" {
 
 int bb_estimator_footprint(const bb_plan *p, bb_precision prec, size_t *bytes)
 {

@@ -210,15 +210,27 @@ __global__ __launch_bounds__(256) void k_gather(const double2 *__restrict__ fiel
     }
 }
 
+// grid.z (NG rows per field) is limited to 65535: batches of more fields
+// than fit are gathered by several launches, the first of which clears the
+// flags.
 hipError_t launch_gather(const double2 *fields, double2 *planes, int NN, int NG, int kmax,
                          int Nmax, int nf, hipStream_t s, int *zero, int nzero)
 {
-    if ((long long)NG * nf > 65535) return hipErrorInvalidValue;
-    dim3 grid((NG + 31) / 32, (Nmax + 31) / 32, NG * nf);
-    if ((long long)grid.x * grid.y * grid.z * 256 < nzero) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_gather, grid, dim3(256), 0, s, fields, planes, NN, NG, kmax, Nmax, zero,
-                       nzero);
-    return hipGetLastError();
+    if (NG < 1 || NG > 65535 || nf < 1) return hipErrorInvalidValue;
+    const int per = 65535 / NG;
+    const long long fstride = (long long)NN * NN * (NN / 2 + 1), pstride = (long long)Nmax * NG * NG;
+    for (int f0 = 0; f0 < nf; f0 += per) {
+        const int n = std::min(per, nf - f0);
+        dim3 grid((NG + 31) / 32, (Nmax + 31) / 32, NG * n);
+        int *z = f0 == 0 ? zero : nullptr;
+        const int nz = f0 == 0 ? nzero : 0;
+        if ((long long)grid.x * grid.y * grid.z * 256 < nz) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_gather, grid, dim3(256), 0, s, fields + f0 * fstride,
+                           planes + f0 * pstride, NN, NG, kmax, Nmax, z, nz);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 // Threads per block of both passes, and the most points a thread loads

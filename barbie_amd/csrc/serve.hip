@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "barbie.h"
+#include "device.hpp"
 #include "plan.hpp"
 
 using namespace bb;
@@ -170,12 +171,6 @@ void bb_server::free_lane(Lane *L)
 int bb_server::make_lane(Model *m, Lane **out)
 {
     *out = nullptr;
-    if (opt.tune && !m->tuned) {
-        // First lane of this geometry: time the candidate layouts once.
-        bb_options best;
-        if (bb_tune(m->plan, &m->eo, nullptr, 2, &best, nullptr) == BB_OK) m->eo = best;
-        m->tuned = true;
-    }
     size_t need = 0;
     int rc = lane_bytes(m, opt, &need);
     if (rc) return rc;
@@ -257,6 +252,11 @@ bool bb_server::complete_lanes()
         L->reqs.clear();
         L->busy = false;
         L->last_used = t;
+        // The server stream goes back to the pool: the idle lane returns to
+        // its own stream, so that destroying it later (eviction) waits for
+        // nothing another lane runs.
+        (void)bb_estimator_set_stream(L->est, nullptr);
+        L->stream = nullptr;
         if (L->slot >= 0) stream_busy[L->slot] = 0;
         L->slot = -1;
         any = true;
@@ -519,6 +519,27 @@ int get_model(bb_server *S, const bb_geometry *g, Model **out)
     std::memset(&nm->eo, 0, sizeof nm->eo);
     nm->eo.precision = S->opt.precision;
     nm->eo.max_batch = S->opt.max_batch;
+    if (S->opt.tune) {
+        // Time the candidate layouts once per geometry, here in the caller's
+        // thread (on the server's device) and outside the server lock, so
+        // that submit, poll and wait of other requests never block on it
+        // (lanes that run meanwhile can slow the candidates;
+        // bb_server_prepare before traffic avoids that).  With reserved CUs
+        // every lane runs on a CU-masked server stream, and the mixed BF16x3
+        // layout, which puts its FP32 share on a stream of its own across
+        // all CUs, is not a candidate.
+        int prev = 0;
+        const bool swap = hipGetDevice(&prev) == hipSuccess && prev != S->device;
+        if (swap) (void)hipSetDevice(S->device);
+        bb_options best;
+        if (bb::tune_layout(nm->plan, &nm->eo, nullptr, 2, S->reserved <= 0, &best, nullptr) ==
+            BB_OK)
+            nm->eo = best;
+        if (swap) (void)hipSetDevice(prev);
+        (void)hipGetLastError();
+        nm->tuned = true;
+    }
+    if (S->reserved > 0) nm->eo.mix_fp32 = 0;
     std::lock_guard<std::mutex> lk(S->mu);
     auto ins = S->models.emplace(key, std::move(nm));
     *out = ins.first->second.get();

@@ -76,6 +76,7 @@ def time_path(B, torch, plan, precision, d_field, steps, warmup, full_grid=False
     levels = est.levels
     nb = plan.geom.nbins * 16
     out = torch.zeros(3 * nb, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()   # the estimator runs on its own stream
     ptr = out.data_ptr()
     args = (d_field.data_ptr(), ptr, ptr + 8 * nb, ptr + 16 * nb)
     for _ in range(warmup):
@@ -125,6 +126,7 @@ def estimate_once(B, torch, plan, precision, d_field):
     est = B.Estimator(plan, precision)
     nb = plan.geom.nbins * 16
     out = torch.zeros(3 * nb, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()   # the estimator runs on its own stream
     ptr = out.data_ptr()
     est.estimate_device(d_field.data_ptr(), ptr, ptr + 8 * nb, ptr + 16 * nb)
     r = out.cpu().numpy()
@@ -160,6 +162,7 @@ def batched_leg(B, torch, precision, nfields=64, distinct=8, reps=3):
         del base
         est = B.Estimator(plan, precision, max_batch=nfields)
         out = torch.zeros(3, nfields, nb, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()   # the estimator runs on its own stream
         ptrs = (fields.data_ptr(), out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr())
         est.estimate_batch_device(nfields, *ptrs)
         torch.cuda.synchronize()

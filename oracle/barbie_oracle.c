@@ -224,13 +224,18 @@ static double input_ps(double kk) { return 1.0 * pow(kk, -2.0); }
  * p1 with per-thread bins merged in thread order (this changes the
  * summation order; nthreads = 1 keeps the reference's order).
  * contr_out (optional): per contraction, in the reference's enumeration
- * order, 8 doubles {p1, r1, p2, r2, p3, r3, Bispec_real, flat bin index}.
+ * order, 8 doubles {p1, r1, p2, r2, p3, r3, Bispec_real, flat bin index}
+ * (threaded runs place each ring triple's records at offsets counted by the
+ * serial pre-pass, so the order and the values are those of a serial run).
  * Returns the number of contractions (K1 inside (LL_k1, UL_k1)) visited, or
  * a negative error. */
 typedef struct {
     int NN, Nr;
     const double *field, *low, *high;
     double **cache;     /* [Nmax*Nr] lazily built fields, 2*NN*NN doubles */
+    const char *dir;    /* on-disk mode: dataPlaneRing_* files are read from here */
+    int NRi;
+    double *buf[3];     /* on-disk mode: buffer1..3 of barbie.c:274-324 */
 } orc_cache;
 
 static const double *cache_get(orc_cache *c, int p, int r)
@@ -245,11 +250,90 @@ static const double *cache_get(orc_cache *c, int p, int r)
     return *slot;
 }
 
+/* A field of the contraction.  In-memory mode: the lazily built cache.
+ * On-disk mode: the reference's access pattern (barbie.c:274-277 for
+ * buffer1 once per ring triple, :315-324 for buffer2 and buffer3 once per
+ * contraction): open, read 2 NN^2 doubles, close, into buffer `slot`. */
+static const double *field_at(orc_cache *c, int p, int r, int slot)
+{
+    if (!c->dir) return cache_get(c, p, r);
+    char name[1024];
+    snprintf(name, sizeof name, "%s/dataPlaneRing_%d_%d_%d_R%d", c->dir, c->NN, p, r, c->NRi);
+    FILE *f = fopen(name, "rb");
+    if (!f) return NULL;
+    const size_t n = 2 * (size_t)c->NN * c->NN;
+    const size_t got = fread(c->buf[slot], sizeof(double), n, f);
+    fclose(f);
+    return got == n ? c->buf[slot] : NULL;
+}
+
+/* wrapid_fast (funcs.c:537-566): D_{p,r} of every plane p < Nmax and ring
+ * r < Nr (those with want[p*Nr+r] != 0 when want is not NULL) written to
+ * <dir>/dataPlaneRing_<NN>_<p>_<r>_R<NRi> as 2 NN^2 doubles, row-major
+ * interleaved complex.  The transforms run on nthreads threads (the
+ * reference runs FFTW with 4, barbie.c:68,151-153).  Returns the number of
+ * files written, or a negative error. */
+long orc_write_plane_cache(int NN, const double *field, int Nmax, int Nr, const double *low,
+                           const double *high, const char *dir, int NRi, int nthreads,
+                           const unsigned char *want)
+{
+    long written = 0, bad = 0;
+    if (nthreads < 1) nthreads = 1;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1) reduction(+:written,bad)
+#endif
+    for (long k = 0; k < (long)Nmax * Nr; ++k) {
+        if (want && !want[k]) continue;
+        const int p = (int)(k / Nr), r = (int)(k % Nr);
+        double *d = (double *)malloc(sizeof(double) * 2 * (size_t)NN * NN);
+        if (!d) { bad++; continue; }
+        orc_df(NN, field, p, low[r], high[r], d);
+        char name[1024];
+        snprintf(name, sizeof name, "%s/dataPlaneRing_%d_%d_%d_R%d", dir, NN, p, r, NRi);
+        FILE *f = fopen(name, "wb");
+        if (!f || fwrite(d, sizeof(double), 2 * (size_t)NN * NN, f) != 2 * (size_t)NN * NN) bad++;
+        else written++;
+        if (f) fclose(f);
+        free(d);
+    }
+    return bad ? -1 : written;
+}
+
+static long estimate_impl(const orc_params *P, const double *low, const double *high,
+                          const double *avg, int rcount, const double *rdata,
+                          const double *field, const char *dir, int NRi, int p1_begin,
+                          int p1_end, int nthreads, double *bin_bs, double *bin_norm,
+                          double *bin_biana, double *contr_out, long contr_cap);
+
 long orc_estimate(const orc_params *P, const double *low, const double *high,
                   const double *avg, int rcount, const double *rdata,
                   const double *field, int p1_begin, int p1_end, int nthreads,
                   double *bin_bs, double *bin_norm, double *bin_biana,
                   double *contr_out, long contr_cap)
+{
+    return estimate_impl(P, low, high, avg, rcount, rdata, field, NULL, 0, p1_begin, p1_end,
+                         nthreads, bin_bs, bin_norm, bin_biana, contr_out, contr_cap);
+}
+
+/* The same loop reading its fields from the reference's on-disk cache
+ * (orc_write_plane_cache) exactly as barbie.c:274-324 does: one thread, one
+ * file read per field use.  Used to time the reference's own access pattern;
+ * the sums are those of orc_estimate. */
+long orc_estimate_disk(const orc_params *P, const double *low, const double *high,
+                       const double *avg, int rcount, const double *rdata, const char *dir,
+                       int NRi, int p1_begin, int p1_end, double *bin_bs, double *bin_norm,
+                       double *bin_biana, double *contr_out, long contr_cap)
+{
+    if (!dir) return -1;
+    return estimate_impl(P, low, high, avg, rcount, rdata, NULL, dir, NRi, p1_begin, p1_end, 1,
+                         bin_bs, bin_norm, bin_biana, contr_out, contr_cap);
+}
+
+static long estimate_impl(const orc_params *P, const double *low, const double *high,
+                          const double *avg, int rcount, const double *rdata,
+                          const double *field, const char *dir, int NRi, int p1_begin,
+                          int p1_end, int nthreads, double *bin_bs, double *bin_norm,
+                          double *bin_biana, double *contr_out, long contr_cap)
 {
     const int NN = P->NN, Nmax = P->Nmax, Nr = P->Nr, n_mtbin = P->n_mtbin;
     const int UL_k1 = P->UL_k1, LL_k1 = P->LL_k1, BW_k1 = P->BW_k1;
@@ -262,9 +346,22 @@ long orc_estimate(const orc_params *P, const double *low, const double *high,
     if (nthreads < 1) nthreads = 1;
 
     orc_cache C;
+    memset(&C, 0, sizeof C);
     C.NN = NN; C.Nr = Nr; C.field = field; C.low = low; C.high = high;
+    C.dir = dir; C.NRi = NRi;
     C.cache = (double **)calloc((size_t)Nmax * Nr, sizeof(double *));
     if (!C.cache) return -2;
+    if (dir) {
+        nthreads = 1;
+        for (int k = 0; k < 3; ++k) {
+            C.buf[k] = (double *)malloc(sizeof(double) * 2 * (size_t)NN * NN);
+            if (!C.buf[k]) {
+                for (int q = 0; q < k; ++q) free(C.buf[q]);
+                free(C.cache);
+                return -2;
+            }
+        }
+    }
 
     long count = 0;
     long bad = 0;
@@ -275,10 +372,21 @@ long orc_estimate(const orc_params *P, const double *low, const double *high,
         if (!tbins) { free(C.cache); return -2; }
     }
 
+    long *roff = NULL;   /* threaded runs: first record index of each ring triple */
+    char *need = NULL;   /* threaded runs: fields a p1 reads */
+    if (nthreads > 1) {
+        need = (char *)calloc((size_t)Nmax * Nr, 1);
+        if (contr_out) roff = (long *)calloc((size_t)rcount + 1, sizeof(long));
+        if (!need || (contr_out && !roff)) {
+            free(need); free(roff); free(tbins); free(C.cache);
+            return -2;
+        }
+    }
+
     for (int p1 = p1_begin; p1 < p1_end; ++p1) {
         /* Fields used by this p1 are built up front so the parallel loop
          * below only reads the cache. */
-        for (int rp = 0; rp < rcount; ++rp) {
+        for (int rp = 0; rp < rcount && !dir; ++rp) {
             int r1 = (int)rdata[rp * 4 + 0];
             (void)cache_get(&C, p1, r1);
         }
@@ -292,6 +400,7 @@ long orc_estimate(const orc_params *P, const double *low, const double *high,
                 else if (r1 == r2 && r2 != r3) { p2_min = -p1; p2_max = p1; }
                 else if (r1 != r2 && r2 == r3) { p2_min = (int)(-p1 / 2); p2_max = Nmax; }
                 else { p2_min = (p1 == 0) ? 0 : -Nmax + 1; p2_max = Nmax; }
+                long nrp = 0;
                 for (int p2 = p2_min; p2 <= p2_max; ++p2) {
                     int p3 = -p1 - p2;
                     int s2 = p2 < 0 ? -1 : 1, s3 = p3 < 0 ? -1 : 1;
@@ -301,12 +410,25 @@ long orc_estimate(const orc_params *P, const double *low, const double *high,
                                           {sqrt(avg[r3] * avg[r3] + p3 * p3), p3}};
                         orc_sort(K);
                         if (K[0][0] > (LL_k1) && K[0][0] < UL_k1) {
-                            (void)cache_get(&C, s2 * p2, r2);
-                            (void)cache_get(&C, s3 * p3, r3);
+                            need[(long)(s2 * p2) * Nr + r2] = 1;
+                            need[(long)(s3 * p3) * Nr + r3] = 1;
+                            ++nrp;
                         }
                     }
                 }
+                if (roff) roff[rp + 1] = roff[rp] + nrp;
             }
+            /* the marked fields are independent: build them in parallel */
+            long nslot = (long)Nmax * Nr, nbuilt = 0;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1) reduction(+:nbuilt)
+#endif
+            for (long k = 0; k < nslot; ++k)
+                if (need[k] && !C.cache[k]) {
+                    nbuilt += cache_get(&C, (int)(k / Nr), (int)(k % Nr)) != NULL;
+                    need[k] = 0;
+                }
+            (void)nbuilt;
         }
 
 #ifdef _OPENMP
@@ -333,7 +455,9 @@ long orc_estimate(const orc_params *P, const double *low, const double *high,
             else if (r1 != r2 && r2 == r3) { p2_min = (int)(-p1 / 2); p2_max = Nmax; }
             else { p2_min = (p1 == 0) ? 0 : -Nmax + 1; p2_max = Nmax; }
 
-            const double *buffer1 = cache_get(&C, p1, r1);
+            const double *buffer1 = field_at(&C, p1, r1, 0);
+            if (!buffer1) { bad++; continue; }
+            long trec = roff ? rec + roff[rp] : 0;   /* this triple's records (threaded) */
             for (int p2 = p2_min; p2 <= p2_max; ++p2) {
                 int p3 = -p1 - p2;
                 int sign_2 = p2 < 0 ? -1 : 1;
@@ -362,9 +486,10 @@ long orc_estimate(const orc_params *P, const double *low, const double *high,
                         const double *buffer2 = C.cache[(long)(sign_2 * p2) * Nr + r2];
                         const double *buffer3 = C.cache[(long)(sign_3 * p3) * Nr + r3];
                         if (nthreads == 1) {
-                            buffer2 = cache_get(&C, sign_2 * p2, r2);
-                            buffer3 = cache_get(&C, sign_3 * p3, r3);
+                            buffer2 = field_at(&C, sign_2 * p2, r2, 1);
+                            buffer3 = field_at(&C, sign_3 * p3, r3, 2);
                         }
+                        if (!buffer2 || !buffer3) { bad++; continue; }
                         double Bispec_real = 0.;
                         for (long index = 0; index < NN2; ++index) {
                             Bispec_real += buffer1[2 * index] * buffer2[2 * index] * buffer3[2 * index]
@@ -393,20 +518,24 @@ long orc_estimate(const orc_params *P, const double *low, const double *high,
                         } else {
                             bad++;   /* the reference would index out of bounds */
                         }
-                        if (contr_out && nthreads == 1) {
-                            if (rec < contr_cap) {
-                                double *o = contr_out + 8 * rec;
+                        if (contr_out) {
+                            long *ri = roff ? &trec : &rec;
+                            if (*ri < contr_cap) {
+                                double *o = contr_out + 8 * *ri;
                                 o[0] = p1; o[1] = r1; o[2] = p2; o[3] = r2; o[4] = p3; o[5] = r3;
                                 o[6] = Bispec_real; o[7] = (double)flat;
                             }
-                            rec++;
+                            ++*ri;
                         }
                         count++;
                     }
                 }
             }
         }
+        if (roff) rec += roff[rcount];
     }
+    free(roff);
+    free(need);
     if (nthreads > 1) {
         for (int t = 0; t < nthreads; ++t)
             for (long k = 0; k < nbin * NPOLES; ++k) {
@@ -418,6 +547,7 @@ long orc_estimate(const orc_params *P, const double *low, const double *high,
     }
     for (long k = 0; k < (long)Nmax * Nr; ++k) free(C.cache[k]);
     free(C.cache);
+    for (int k = 0; k < 3; ++k) free(C.buf[k]);
     if (bad) return -1000 - bad;
     return count;
 }

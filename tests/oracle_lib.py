@@ -45,6 +45,14 @@ def lib():
         L.orc_write_outputs.argtypes = [ctypes.POINTER(OrcParams), _dp, _dp, _dp, ctypes.c_char_p,
                                         ctypes.c_int]
         L.orc_calpow_mom.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double, _dp]
+        L.orc_write_plane_cache.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp,
+                                            _dp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_void_p]
+        L.orc_write_plane_cache.restype = ctypes.c_long
+        L.orc_estimate_disk.argtypes = [ctypes.POINTER(OrcParams), _dp, _dp, _dp, ctypes.c_int,
+                                        _dp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_long]
+        L.orc_estimate_disk.restype = ctypes.c_long
         _lib = L
     return _lib
 
@@ -142,3 +150,44 @@ def calpow_mom(field, NN, Nbin=10, LL=0.56):
     f = np.ascontiguousarray(field, dtype=np.float64)
     assert lib().orc_calpow_mom(NN, _p(f), Nbin, float(LL), _p(out)) == 0
     return out
+
+
+def write_plane_cache(field, NN, Nmax, lo, hi, directory, NRi, nthreads=4, want=None):
+    """wrapid_fast (funcs.c:537-566): dataPlaneRing_<NN>_<p>_<r>_R<NRi> files
+    in `directory` for every (p, r), or those with want[p, r] true."""
+    f = np.ascontiguousarray(field, dtype=np.float64)
+    lo = np.ascontiguousarray(lo, dtype=np.float64)
+    hi = np.ascontiguousarray(hi, dtype=np.float64)
+    w = None
+    if want is not None:
+        w = np.ascontiguousarray(want, dtype=np.uint8).reshape(-1)
+        assert w.size == Nmax * lo.size
+    n = lib().orc_write_plane_cache(NN, _p(f), Nmax, lo.size, _p(lo), _p(hi), directory.encode(),
+                                    NRi, nthreads, None if w is None else w.ctypes.data)
+    assert n >= 0, n
+    return n
+
+
+def estimate_disk(params, lo, hi, av, rdata, directory, NRi, p1_begin=0, p1_end=None,
+                  record=False, cap=None):
+    """The oracle loop reading its fields from the on-disk cache as
+    barbie.c:274-324 does (one thread).  Returns (bs, norm, biana, count, contr)."""
+    P = OrcParams(**params)
+    nb = int((P.UL_k1 - P.LL_k1) / (1.0 * P.BW_k1)) * P.n_mtbin * P.n_mtbin
+    bs, nm, bi = np.zeros(nb * 16), np.zeros(nb * 16), np.zeros(nb * 16)
+    if p1_end is None:
+        p1_end = P.Nmax
+    rd = np.ascontiguousarray(rdata, dtype=np.float64).reshape(-1)
+    contr, cptr = None, None
+    if record:
+        cap = cap or 2_000_000
+        contr = np.zeros((cap, 8))
+        cptr = _p(contr)
+    n = lib().orc_estimate_disk(ctypes.byref(P), _p(lo), _p(hi), _p(av), rd.size // 4, _p(rd),
+                                directory.encode(), NRi, p1_begin, p1_end, _p(bs), _p(nm), _p(bi),
+                                cptr, cap if record else 0)
+    if n < 0:
+        raise RuntimeError(f"orc_estimate_disk failed: {n}")
+    if record:
+        contr = contr[:n].copy()
+    return bs, nm, bi, n, contr

@@ -47,7 +47,7 @@ def test_async_batch_matches_sync():
     d_f = torch.from_numpy(fields).to("cuda")
     nb = g.nbins * 16
     out = torch.zeros(3, 2, nb, dtype=torch.float64, device="cuda")
-    torch.cuda.synchronize()
+    torch.cuda.synchronize()   # the estimator runs on its own stream
     est.estimate_batch_device(2, d_f.data_ptr(), out[0].data_ptr(), out[1].data_ptr(),
                               out[2].data_ptr(), wait=False)
     while not est.done():
@@ -74,7 +74,7 @@ def test_async_batches_queue_in_order():
     d_f = torch.from_numpy(fields).to("cuda")
     nb = g.nbins * 16
     out = torch.zeros(3, 3, nb, dtype=torch.float64, device="cuda")
-    torch.cuda.synchronize()
+    torch.cuda.synchronize()   # the estimator runs on its own stream
     for k in range(3):
         est.estimate_batch_device(1, d_f[k].data_ptr(), out[0, k].data_ptr(),
                                   out[1, k].data_ptr(), out[2, k].data_ptr(), wait=False)
@@ -101,15 +101,18 @@ def test_graph_replays_equal_eager(prec):
     ref = {}
     for k, f in ((1, f1), (2, f2)):
         o = torch.zeros(3, nb, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()   # the estimator runs on its own stream
         eager.estimate_device(f.data_ptr(), o[0].data_ptr(), o[1].data_ptr(), o[2].data_ptr())
         ref[k] = o.cpu().numpy()
     est = B.Estimator(plan, prec, use_graph=True)
     out = torch.zeros(3, nb, dtype=torch.float64, device="cuda")
     other = torch.zeros(3, nb, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()   # the estimator runs on its own stream
     args = (f1.data_ptr(), out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr())
     for rnd in range(2):
         for rep in range(4):
             out.zero_()
+            torch.cuda.synchronize()   # the estimator runs on its own stream
             est.estimate_device(*args)      # rep 0 eager + capture, then replays
             assert np.array_equal(out.cpu().numpy(), ref[1]), (rnd, rep)
             t = est.timings()
@@ -141,13 +144,14 @@ def test_batch_equals_single_at_batching_sizes(NN, prec):
     nb = g.nbins * 16
     fields = torch.stack([B.make_field_device(NN, NRi=k, fnl=0.3) for k in range(1, 5)])
     out = torch.zeros(3, 4 * nb, dtype=torch.float64, device="cuda")
-    torch.cuda.synchronize()
+    torch.cuda.synchronize()   # the estimator runs on its own stream
     batched = B.Estimator(plan, prec, max_batch=4)
     batched.estimate_batch_device(4, fields.data_ptr(), out[0].data_ptr(), out[1].data_ptr(),
                                   out[2].data_ptr())
     got = out.cpu().numpy().reshape(3, 4, nb)
     single = B.Estimator(plan, prec)
     one = torch.zeros(3 * nb, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()   # the estimator runs on its own stream
     p = one.data_ptr()
     for k in range(4):
         torch.cuda.synchronize()
@@ -156,3 +160,29 @@ def test_batch_equals_single_at_batching_sizes(NN, prec):
         for a in range(3):
             assert np.array_equal(got[a, k], alone[a]), (k, a)
     assert np.abs(got[0]).max() > 0
+
+
+def test_batch_of_1024_fields_on_the_nn_grid():
+    """max_batch 1024 at NN=64 on the NN grid: the plane gather's launch
+    rows (64 per field) exceed one launch's 65535, so the batch is gathered by
+    two launches; every field's bins equal the single-field estimate's."""
+    torch = pytest.importorskip("torch")
+    g = B.Geometry(NN=64, UL_k1=10, LL_k1=1, BW_k1=1, Nmax=16, Nr=8, ring_rule=B.RINGS_UNIFORM,
+                   ring_width=1.0, last_ring_to_nmax=0)
+    plan = B.Plan(g)
+    nf = 1024
+    f = [B.make_field(g.NN, NRi=k) for k in (1, 2, 3)]
+    ref = [B.Estimator(plan, "fp64", full_grid=True).estimate(x) for x in f]
+    est = B.Estimator(plan, "fp64", full_grid=True, max_batch=nf)
+    assert est.grid == 64 and 64 * nf > 65535
+    d_f = torch.from_numpy(np.stack([f[k % 3] for k in range(nf)])).to("cuda")
+    nb = g.nbins * 16
+    out = torch.zeros(3, nf, nb, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()   # the estimator runs on its own stream
+    est.estimate_batch_device(nf, d_f.data_ptr(), out[0].data_ptr(), out[1].data_ptr(),
+                              out[2].data_ptr())
+    got = out.cpu().numpy()
+    for k in (0, 1, 2, 1021, 1022, 1023):
+        for a in range(3):
+            assert np.array_equal(got[a, k], ref[k % 3][a].reshape(-1)), (k, a)
+    est.close()

@@ -77,3 +77,35 @@ def test_admission_evicts_and_refuses():
     with pytest.raises(B.BarbieError):
         tiny.wait(t)
     tiny.close()
+
+
+def test_short_requests_beside_long_ones_with_tuning():
+    """tune=1 with reserved CUs (the default 1/8): the tuned layouts keep
+    every lane on its CU-masked server stream, so short requests (NN=128) that
+    arrive while long ones (NN=384) run finish well before them, and their
+    bins are the same bits as when they ran alone on the same server."""
+    torch = pytest.importorskip("torch")
+    gl = B.Geometry(NN=384, UL_k1=51, LL_k1=1, BW_k1=1)
+    gs = config1()
+    dl = B.make_field_device(gl.NN, NRi=2)
+    ds = torch.from_numpy(B.make_field(gs.NN, NRi=4)).to("cuda")
+    torch.cuda.synchronize()
+    srv = B.Server("bf16x3", max_inflight=4, max_batch=2, tune=True)
+    srv.prepare(gl, 2)
+    srv.prepare(gs, 2)
+    for g, d in ((gl, dl), (gs, ds)):   # service times learned before the mix
+        for _ in range(2):
+            ref = srv.wait(srv.submit(g, d.data_ptr()))
+    longs = [srv.submit(gl, dl.data_ptr()) for _ in range(6)]
+    shorts = [srv.submit(gs, ds.data_ptr()) for _ in range(4)]
+    lat_s = []
+    for t in shorts:
+        bs, nm, bi, lat = srv.wait(t)
+        assert np.array_equal(bs, ref[0]) and np.array_equal(nm, ref[1])
+        lat_s.append(lat[2])
+    lat_l = [srv.wait(t)[3][2] for t in longs]
+    print("short latencies (ms)", ["%.2f" % x for x in lat_s], "long",
+          ["%.2f" % x for x in lat_l])
+    assert max(lat_s) < 0.5 * max(lat_l)
+    assert srv.stats()["failed"] == 0
+    srv.close()

@@ -288,3 +288,34 @@ def test_oracle_config1_contraction_count_and_threads():
     assert n1 == n4 > 0
     assert np.allclose(bs1, bs4, rtol=1e-11, atol=1e-11 * np.max(np.abs(bs1)))
     assert np.allclose(nm1, nm4, rtol=1e-12, atol=0)
+    # threaded runs record every contraction in the serial order, same values
+    _, _, _, _, c1 = O.estimate(p, lo, hi, av, rd, field, p1_begin=2, p1_end=4, record=True)
+    _, _, _, _, c4 = O.estimate(p, lo, hi, av, rd, field, p1_begin=2, p1_end=4, nthreads=4,
+                                record=True)
+    assert c1.shape == c4.shape and np.array_equal(c1, c4)
+
+
+@pytest.mark.parametrize("gf", [small, medium])
+def test_oracle_on_disk_cache_gives_the_same_sums(gf, tmp_path):
+    """The reference's on-disk pattern (wrapid_fast files, barbie.c:274-324
+    reads) gives the in-memory oracle's sums bit for bit, and a missing cache
+    file is an error, not a silent zero."""
+    g = gf()
+    lo, hi, av = oracle_edges(g)
+    rd = O.calcr(g.NN, lo, hi, av)
+    field = make_field(g.NN, NRi=2)
+    p = oracle_params(g)
+    n = O.write_plane_cache(field, g.NN, g.Nmax, lo, hi, str(tmp_path), 2, nthreads=4)
+    assert n == g.Nmax * g.Nr
+    one = O.df(field, g.NN, 1, lo[1], hi[1])
+    disk = np.fromfile(tmp_path / f"dataPlaneRing_{g.NN}_1_1_R2").view(np.complex128)
+    assert np.array_equal(disk.reshape(g.NN, g.NN), one)
+    a = O.estimate(p, lo, hi, av, rd, field, record=True)
+    b = O.estimate_disk(p, lo, hi, av, rd, str(tmp_path), 2, record=True)
+    assert a[3] == b[3] > 0
+    for x, y in zip(a[:3], b[:3]):
+        assert np.array_equal(x, y)
+    assert np.array_equal(a[4], b[4])
+    (tmp_path / f"dataPlaneRing_{g.NN}_0_1_R2").unlink()
+    with pytest.raises(RuntimeError):
+        O.estimate_disk(p, lo, hi, av, rd, str(tmp_path), 2)
