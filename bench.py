@@ -8,14 +8,18 @@ A step is one full estimate of one field already resident in HBM: plane
 gather, plane-ring IFFT caches, contraction, binning (bb_estimate_device).
 
 Prints one JSON line.  `value` is the fields/hour of the split path
-(--precision, default bf16x3); the FP64 GPU path is timed in the same run and
-reported beside it, with the per-multipole error of the split path against
-it at fNL-style coefficients 0.1, 0.3 and 1.0.  `roofline` is the
-contraction kernel; `cpu_baseline` is the CPU oracle on a bounded sample of
-the same workload.  Further legs: `batched` (64 realizations per NN at
-NN = 128, 192, 256, batched launches) and `serving` (request server, Poisson
-arrivals over an NN = 128/256/384/512 mix weighted 4:3:2:1, p50/p99 latency
-at 50/80/95% of the measured saturation rate).
+(--precision, default bf16x3); the FP64 and FP32 GPU paths are timed in the
+same run and reported beside it, with the per-multipole error of the split
+and FP32 paths against FP64 at fNL-style coefficients 0.1, 0.3 and 1.0.
+`roofline` is the contraction kernel on the work it performs (the
+reference's NN-grid work beside it); `host_field_path` is the rate with the
+field in host memory (upload included); `cpu_baseline` is the CPU oracle on
+a bounded sample of the same workload (one thread, all threads, and the
+reference's on-disk pattern); `c1` times the NN=128 CPU configuration in
+full.  Further legs: `batched` (64 realizations per NN at NN = 128, 192,
+256, batched launches) and `serving` / `serving_fp64` (request server,
+Poisson arrivals over an NN = 128/256/384/512 mix weighted 4:3:2:1, p50/p99
+latency at 50/80/95% of the measured saturation rate).
 """
 import argparse
 import json
@@ -38,7 +42,8 @@ def parse():
     ap.add_argument("--full-grid", action="store_true",
                     help="run the transforms and sums on the NN grid (the reference's grid)")
     ap.add_argument("--NN", type=int, default=384)
-    ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--skip-fp64", action="store_true",
+                    help="time only --precision (no FP64 and FP32 legs)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="target CPU time of the single-thread oracle sample (0 disables it)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
@@ -292,13 +297,93 @@ def cpu_power_baseline(field, NN, got):
             "max_rel_diff_P0_gpu": float(np.max(np.abs(got[:, 1] - ref[:, 1]) / np.abs(ref[:, 1])))}
 
 
-def cpu_baseline(B, plan, field, target_s):
-    """CPU oracle on the p1 prefix whose contraction count makes about
-    target_s of single-thread work, extrapolated by contraction count; the
-    same sample again on all of the host share's cores (16 threads)."""
-    import numpy as np
+def host_info():
+    """The host CPU model and the threads this process may use."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "os_cpu_count": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0)), "threads_used": cpu_threads()}
+
+
+def cpu_threads():
+    """All cores of the host share this run may use (at most 16)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def oracle_setup(plan):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
+    g = plan.geom
+    lo, hi, av = plan.ring_edges()
+    t0 = time.perf_counter()
+    rd = O.calcr(g.NN, lo, hi, av)
+    t_calcr = time.perf_counter() - t0
+    params = {kk: getattr(g, kk) for kk in ("NN", "Nmax", "Nr", "n_mtbin", "UL_k1", "LL_k1",
+                                            "BW_k1", "LL", "ff", "beta")}
+    return O, params, (lo, hi, av), rd, t_calcr
+
+
+def fields_read(plan, p1_begin, p1_end):
+    """[Nmax, Nr] mask of the plane-ring fields the reference's loop reads for
+    p1 in [p1_begin, p1_end): buffer1 = D_{p1,r1} for every ring triple
+    (barbie.c:274-277, read whether or not the triple has a contraction at
+    this p1) and the second and third fields of each contraction."""
+    import numpy as np
+    g = plan.geom
+    pc = plan.contractions()
+    sel = pc[(pc[:, 0] >= p1_begin) & (pc[:, 0] < p1_end)]
+    want = np.zeros((g.Nmax, g.Nr), dtype=np.uint8)
+    r1 = np.unique(plan.triples()[:, 0])
+    for p1 in range(p1_begin, p1_end):
+        want[p1, r1] = 1
+    for pcol, rcol in ((2, 3), (4, 5)):
+        want[np.abs(sel[:, pcol]), sel[:, rcol]] = 1
+    return want
+
+
+def disk_leg(O, params, edges, rd, field, plan, p1_begin, p1_end, want, tmpdir):
+    """The reference's own access pattern: the fields written to disk by
+    wrapid_fast with 4 FFT threads (barbie.c:68,151-153, funcs.c:537-566),
+    then the loop reading buffer1 per ring triple and buffer2/buffer3 per
+    contraction from those files, one thread (barbie.c:274-324).  Returns
+    (seconds writing, files written, seconds in the loop, contractions)."""
+    import numpy as np
+    g = plan.geom
+    lo, hi, av = edges
+    t0 = time.perf_counter()
+    nw = O.write_plane_cache(field, g.NN, g.Nmax, lo, hi, tmpdir, 1, nthreads=4,
+                             want=None if want is None else want)
+    t_w = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, _, _, n, _ = O.estimate_disk(params, lo, hi, av, rd, tmpdir, 1, p1_begin=p1_begin,
+                                    p1_end=p1_end)
+    t_l = time.perf_counter() - t0
+    return t_w, nw, t_l, n
+
+
+def cpu_baseline(B, plan, field, target_s):
+    """CPU oracle (FP64, own FFT) on a bounded sample of the NN=384 workload,
+    extrapolated by contraction count:
+    * `value`: the in-memory oracle on one thread over the p1 prefix whose
+      contraction count makes about target_s of work;
+    * `all_cores`: the same sample with the ring triples of each p1 spread
+      over all threads of the host share;
+    * `reference_faithful`: the reference's pattern (on-disk plane-ring
+      cache built with 4 FFT threads, single-thread loop reading two cache
+      files per contraction) on the slice of the last p1 plane with
+      contractions, the cache build scaled by files (Nmax x Nr are written
+      per field) and the loop by contractions."""
+    import numpy as np
+    import tempfile
+    import shutil
+    O, params, edges, rd, t_calcr = oracle_setup(plan)
+    lo, hi, av = edges
     g = plan.geom
     pc = plan.contractions()
     counts = np.bincount(pc[:, 0], minlength=g.Nmax)
@@ -307,32 +392,126 @@ def cpu_baseline(B, plan, field, target_s):
     want = max(1, int(target_s / per))
     k = int(np.searchsorted(np.cumsum(counts), want)) + 1
     k = max(1, min(k, g.Nmax))
-    lo, hi, av = plan.ring_edges()
-    rd = O.calcr(g.NN, lo, hi, av)
-    params = {kk: getattr(g, kk) for kk in ("NN", "Nmax", "Nr", "n_mtbin", "UL_k1", "LL_k1",
-                                            "BW_k1", "LL", "ff", "beta")}
     t0 = time.perf_counter()
     _, _, _, n, _ = O.estimate(params, lo, hi, av, rd, field, p1_begin=0, p1_end=k, nthreads=1)
     dt = time.perf_counter() - t0
-    est_s = dt * ncontr / n
-    # All cores: the oracle's threads split the p1 range, so the sample is
-    # the last 16 p1 planes that hold contractions.
-    threads = 16
-    nzp = np.nonzero(counts)[0]
-    p1e = int(nzp.max()) + 1
-    p1b = max(0, p1e - threads)
+    est_s = dt * ncontr / n + t_calcr
+    threads = cpu_threads()
     t0 = time.perf_counter()
-    _, _, _, nm, _ = O.estimate(params, lo, hi, av, rd, field, p1_begin=p1b, p1_end=p1e,
+    _, _, _, nm, _ = O.estimate(params, lo, hi, av, rd, field, p1_begin=0, p1_end=k,
                                 nthreads=threads)
     dtm = time.perf_counter() - t0
-    return {"value": 3600.0 / est_s, "unit": "fields/hour", "cores": 1, "kind": "port",
-            "sample": f"CPU oracle (FP64, own FFT, 1 thread) over p1 in [0,{k}): {n} of {ncontr} "
-                      f"contractions in {dt:.1f} s, extrapolated by contraction count to "
-                      f"{est_s:.0f} s/field",
-            "all_cores": {"value": 3600.0 / (dtm * ncontr / nm), "cores": threads,
-                          "sample": f"p1 in [{p1b},{p1e}) with {threads} OpenMP threads (one "
-                                    f"p1 plane each): {nm} contractions in {dtm:.1f} s, "
-                                    "extrapolated by contraction count"}}
+    est_m = dtm * ncontr / nm + t_calcr
+    out = {"value": 3600.0 / est_s, "unit": "fields/hour", "cores": 1, "kind": "port",
+           "sample": f"CPU oracle (FP64, own FFT, in-memory fields, 1 thread) over p1 in [0,{k}): "
+                     f"{n} of {ncontr} contractions in {dt:.1f} s, extrapolated by contraction "
+                     f"count (+ calcr {t_calcr:.1f} s) to {est_s:.0f} s/field",
+           "host": host_info(),
+           "all_cores": {"value": 3600.0 / est_m, "cores": threads,
+                         "sample": f"same p1 prefix with {threads} OpenMP threads: {nm} "
+                                   f"contractions in {dtm:.1f} s, extrapolated to "
+                                   f"{est_m:.0f} s/field"}}
+    nzp = np.nonzero(counts)[0]
+    p1s = int(nzp.max())
+    mask = fields_read(plan, p1s, p1s + 1)
+    tmp = tempfile.mkdtemp(prefix="barbie_ref_cache_")
+    try:
+        t_w, nw, t_l, nd = disk_leg(O, params, edges, rd, field, plan, p1s, p1s + 1, mask, tmp)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    est_d = t_w * (g.Nmax * g.Nr) / nw + t_l * ncontr / nd + t_calcr
+    out["reference_faithful"] = {
+        "value": 3600.0 / est_d, "unit": "fields/hour", "cores": 4,
+        "sample": f"p1 = {p1s}: {nw} cache files written with 4 FFT threads in {t_w:.1f} s "
+                  f"(scaled to the {g.Nmax * g.Nr} files of a field), {nd} contractions reading "
+                  f"their cache files in {t_l:.1f} s on 1 thread (scaled to {ncontr}); "
+                  f"{est_d:.0f} s/field",
+        "seconds_per_field": est_d}
+    return out
+
+
+def c1_leg(B, torch):
+    """The CPU configuration, NN=128 (Nmax=32, Nr=12 rings of width 1, UL_k1=21),
+    timed in full: the oracle on one thread, on all threads of the host share
+    and in the reference's on-disk pattern; the FP64 GPU estimate of the same
+    field beside it, with its largest bin difference from the oracle."""
+    import numpy as np
+    import tempfile
+    import shutil
+    g = B.Geometry(NN=128, UL_k1=21, LL_k1=1, BW_k1=1, Nmax=32, Nr=12, n_mtbin=5,
+                   ring_rule=B.RINGS_UNIFORM, ring_width=1.0, last_ring_to_nmax=0)
+    plan = B.Plan(g)
+    field = B.make_field(g.NN, NRi=1, fnl=0.3)
+    O, params, edges, rd, t_calcr = oracle_setup(plan)
+    lo, hi, av = edges
+    t0 = time.perf_counter()
+    bs1, nm1, _, n1, _ = O.estimate(params, lo, hi, av, rd, field, nthreads=1)
+    t1 = time.perf_counter() - t0 + t_calcr
+    threads = cpu_threads()
+    t0 = time.perf_counter()
+    O.estimate(params, lo, hi, av, rd, field, nthreads=threads)
+    tm = time.perf_counter() - t0 + t_calcr
+    tmp = tempfile.mkdtemp(prefix="barbie_ref_cache_")
+    try:
+        t_w, nw, t_l, nd = disk_leg(O, params, edges, rd, field, plan, 0, g.Nmax, None, tmp)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    td = t_w + t_l + t_calcr
+    est = B.Estimator(plan, "fp64")
+    bs, nm, _ = est.estimate(field)
+    d_f = torch.from_numpy(field).to("cuda")
+    nb = g.nbins * 16
+    out = torch.zeros(3 * nb, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    ptr = out.data_ptr()
+    est.estimate_device(d_f.data_ptr(), ptr, ptr + 8 * nb, ptr + 16 * nb)
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        est.estimate_device(d_f.data_ptr(), ptr, ptr + 8 * nb, ptr + 16 * nb)
+    tg = (time.perf_counter() - t0) / reps
+    est.close()
+    err = float(np.max(np.abs(bs.ravel() - bs1)) / np.max(np.abs(bs1)))
+    return {"workload": "estimate NN=128 Nmax=32 Nr=12 (width 1) UL_k1=21 LL_k1=1 BW_k1=1",
+            "ncontr": int(n1),
+            "cpu_fields_per_hour": {"oracle_1_thread": 3600.0 / t1,
+                                    f"oracle_{threads}_threads": 3600.0 / tm,
+                                    "reference_faithful": 3600.0 / td},
+            "cpu_seconds": {"oracle_1_thread": t1, f"oracle_{threads}_threads": tm,
+                            "reference_faithful": td,
+                            "reference_faithful_parts": {"calcr": t_calcr, "cache_write": t_w,
+                                                         "loop_from_disk": t_l}},
+            "gpu_fp64_fields_per_hour": 3600.0 / tg, "gpu_fp64_ms": tg * 1e3,
+            "gpu_fp64_max_rel_diff_bins_vs_oracle": err,
+            "note": "CPU times are whole-field (no extrapolation); reference_faithful writes "
+                    f"{nw} dataPlaneRing files with 4 FFT threads, then reads 2 per "
+                    "contraction on 1 thread"}
+
+
+def host_path_rate(B, plan, precision, opts, field, reps=5):
+    """Fields/hour through bb_estimate with the field in pageable host memory
+    (the CLI's path): upload included."""
+    est = B.Estimator(plan, precision, **(opts or {}))
+    est.estimate(field)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        est.estimate(field)
+    dt = (time.perf_counter() - t0) / reps
+    est.close()
+    return {"fields_per_hour": 3600.0 / dt, "ms_per_field": dt * 1e3,
+            "note": "bb_estimate with the r2c field in pageable host memory; the library "
+                    "uploads what the estimate reads"}
+
+
+def traffic_from_profile(precision, NN):
+    """HBM bytes per contraction launch, from the rocprofv3 FETCH_SIZE and
+    WRITE_SIZE counters recorded under profiles/ (not measured by this run)."""
+    prof = os.path.join(ROOT, "profiles", f"contraction_{precision}_nn{NN}.txt")
+    if os.path.exists(prof):
+        for line in open(prof):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1]), os.path.relpath(prof, ROOT)
+    return None, None
 
 
 def main():
@@ -358,8 +537,9 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    others = [] if a.skip_fp64 else [p for p in ("fp64", "fp32") if p != a.precision]
     tuned = {}
-    for prec in ([a.precision] + ([] if a.skip_fp64 or a.precision == "fp64" else ["fp64"])):
+    for prec in [a.precision] + others:
         if a.layout and prec == a.precision:
             kv = dict(x.split("=") for x in a.layout.split(","))
             tuned[prec] = ({k: int(v) for k, v in kv.items()}, None, 0.0)
@@ -371,27 +551,21 @@ def main():
             tuned[prec] = (opts, tms, time.perf_counter() - t0)
     ms, st, bins, levels, wl = time_path(B, torch, plan, a.precision, d_field, a.steps,
                                          a.warmup, a.full_grid, tuned[a.precision][0])
-    res = {"fast": (ms, st, bins)}
-    if "fp64" in tuned and a.precision != "fp64":
-        r64 = time_path(B, torch, plan, "fp64", d_field, a.steps, a.warmup, a.full_grid,
-                        tuned["fp64"][0])
-        res["fp64"] = r64[:3]
-        wl64 = r64[4]
+    res = {}
+    for prec in others:
+        res[prec] = time_path(B, torch, plan, prec, d_field, a.steps, a.warmup, a.full_grid,
+                              tuned[prec][0])
 
-    # Algorithmic contraction work per field, as the reference computes it:
-    # Re(P C) is two real FMAs per point of the NN grid per contraction.
+    # Contraction work per field.  Performed: 4 flops (two real FMAs of
+    # Re(P C)) per point of each contraction's own grid, 4 * sum_c NG(c)^2;
+    # the reference's: the same over the NN grid, 4 * NN^2 * ncontr (the grid
+    # levels give the same sums exactly on fewer points).
     w_ref = 4.0 * g.NN * g.NN * s["ncontr"]
-    # Work this build performs: the same over each contraction's own grid.
     w_done = 4.0 * wl["points_summed"]
     t_contr = st[2] * 1e-3
     peak = PEAK[a.precision]
     value = 3600e3 / ms
-    traffic = None
-    prof = os.path.join(ROOT, "profiles", f"contraction_{a.precision}_nn{g.NN}.txt")
-    if os.path.exists(prof):
-        for line in open(prof):
-            if line.startswith("traffic_bytes_per_launch:"):
-                traffic = float(line.split(":")[1])
+    traffic, traffic_src = traffic_from_profile(a.precision, g.NN)
     out = {
         "metric": "NN=384 field estimates/hour, split-FP32 vs FP64 GPU; p50/p99 request latency",
         "value": value,
@@ -416,27 +590,48 @@ def main():
         "worklist": wl,
         "multipoles_without_denominator": unsupported_entries(bins, g.nbins),
         "roofline": {"bound": "mfma", "kernel": "contraction",
-                     "achieved": w_ref / t_contr / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                     "frac": w_ref / t_contr / peak, "traffic": traffic,
-                     "work": f"W_contr = 4 * NN^2 * ncontr = {w_ref:.4g} flop per launch "
-                             f"(reference grid); performed: 4 * sum_c NG(c)^2 = {w_done:.4g} "
-                             f"(grid levels {levels}), {w_done / t_contr / 1e12:.2f} TFLOP/s; "
-                             "bf16x3 peak taken as 2.5 PF/3"},
+                     "achieved": w_done / t_contr / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                     "frac": w_done / t_contr / peak, "traffic": traffic,
+                     "traffic_source": (f"{traffic_src}: rocprofv3 --pmc FETCH_SIZE x2 + "
+                                        "WRITE_SIZE of bench.py (not measured by this run)")
+                                       if traffic_src else None,
+                     "achieved_reference_work": w_ref / t_contr / 1e12,
+                     "work": f"performed W = 4 * sum_c NG(c)^2 = {w_done:.4g} flop per launch "
+                             f"(grid levels {levels}); the reference's NN-grid work "
+                             f"4 * NN^2 * ncontr = {w_ref:.4g} (the same sums) gives "
+                             "achieved_reference_work; bf16x3 peak taken as 2.5 PF/3 "
+                             "(three bf16 products per FP32 product)",
+                     "kernel_ms": st[2]},
     }
+    nb = g.nbins * 16
+    for prec, (msp, stp, binsp, levp, wlp) in res.items():
+        out[prec] = {"fields_per_hour": 3600e3 / msp, "ms_per_step": msp,
+                     "stages_ms": {"gather": stp[0], "fft_cache": stp[1],
+                                   "contraction": stp[2], "binning": stp[3]},
+                     "grid_levels": levp,
+                     "contraction_roofline": {
+                         "achieved": 4.0 * wlp["points_summed"] / (stp[2] * 1e-3) / 1e12,
+                         "peak": PEAK[prec] / 1e12, "unit": "TFLOP/s",
+                         "frac": 4.0 * wlp["points_summed"] / (stp[2] * 1e-3) / PEAK[prec],
+                         "achieved_reference_work": w_ref / (stp[2] * 1e-3) / 1e12},
+                     "worklist": wlp}
     if "fp64" in res:
-        ms64, st64, bins64 = res["fp64"]
-        nb = g.nbins * 16
-        out["fp64"] = {"fields_per_hour": 3600e3 / ms64, "ms_per_step": ms64,
-                       "stages_ms": {"gather": st64[0], "fft_cache": st64[1],
-                                     "contraction": st64[2], "binning": st64[3]},
-                       "contraction_TFLOPs_reference_work": w_ref / (st64[2] * 1e-3) / 1e12,
-                       "worklist": wl64}
-        out["speedup_vs_fp64"] = ms64 / ms
+        bins64 = res["fp64"][2]
+        out["speedup_vs_fp64"] = res["fp64"][0] / ms
         out["rel_err_vs_fp64_per_multipole"] = per_multipole_err(bins, bins64, nb)
+        if "fp32" in res:
+            out["fp32"]["speedup_vs_fp64"] = res["fp64"][0] / res["fp32"][0]
+            out["fp32"]["rel_err_vs_fp64_per_multipole"] = per_multipole_err(res["fp32"][2],
+                                                                             bins64, nb)
         if not a.no_accuracy:
             acc = accuracy_leg(B, torch, plan, g, a.precision)
             acc["0.3"] = max(out["rel_err_vs_fp64_per_multipole"])
             out["max_rel_err_vs_fp64_by_fnl"] = acc
+            if "fp32" in res:
+                acc32 = accuracy_leg(B, torch, plan, g, "fp32")
+                acc32["0.3"] = max(out["fp32"]["rel_err_vs_fp64_per_multipole"])
+                out["fp32"]["max_rel_err_vs_fp64_by_fnl"] = acc32
+    out["host_field_path"] = host_path_rate(B, plan, a.precision, tuned[a.precision][0], field)
     if not a.no_power:
         out["power_multipoles"], power_rows = power_leg(B, torch, d_field, g.NN)
     del d_field
@@ -448,10 +643,13 @@ def main():
         out["serving"] = sv
         out["p50_ms"] = sv["loads"]["0.8"]["p50_ms"]
         out["p99_ms"] = sv["loads"]["0.8"]["p99_ms"]
+        if not a.skip_fp64 and a.precision != "fp64":
+            out["serving_fp64"] = serving_leg(B, torch, "fp64")
     if a.cpu_seconds > 0:
         out["cpu_baseline"] = cpu_baseline(B, plan, field, a.cpu_seconds)
         if not a.no_power:
             out["cpu_baseline"]["power_multipoles"] = cpu_power_baseline(field, g.NN, power_rows)
+        out["c1"] = c1_leg(B, torch)
     print(json.dumps(out), flush=True)
 
 
