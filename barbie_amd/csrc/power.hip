@@ -281,19 +281,16 @@ float bin_threshold(float scale, int d)
     return v;
 }
 
-}  // namespace
-
-extern "C" int bb_power_multipoles(const double *field_r2c, int on_device, int NN, int Nbin,
-                                   double LL, double *out)
+// Validates the arguments and fills the kernel parameters (bins, the
+// thresholds of the log bins, the triple cut).
+int power_params(int NN, int Nbin, double LL, PsParams &P)
 {
     using bb::set_error;
-    if (!field_r2c || !out) return set_error(BB_E_INVALID, "null argument");
     if (NN < 4 || (NN & 1) || NN > 4096) return set_error(BB_E_INVALID, "NN must be even, 4..4096");
     if (Nbin < 1 || Nbin > kPsMaxBins)
         return set_error(BB_E_INVALID, "Nbin must be in [1, " + std::to_string(kPsMaxBins) + "]");
     if (!(LL > 0)) return set_error(BB_E_INVALID, "LL must be positive");
-    const long long N = NN, H = N / 2 + 1, nmodes = N * N * H, ntriples = H * H * H;
-    PsParams P;
+    const long long N = NN;
     std::memset(&P, 0, sizeof P);
     P.N = NN;
     P.Nbin = Nbin;
@@ -303,11 +300,74 @@ extern "C" int bb_power_multipoles(const double *field_r2c, int on_device, int N
     // The float m N of a mode is within 1e-6 of sqrt(a^2 + b^2 + k^2), so
     // folded triples with the integer sum above (thr[Nbin] (1 + 1e-4))^2
     // have a bin of at least Nbin and are not summed.
-    {
-        const double edge = (double)P.thr[Nbin] * (1.0 + 1e-4);
-        P.r2_cut = std::isfinite(edge) && edge * edge < (double)INT_MAX ? (int)(edge * edge)
-                                                                         : INT_MAX;
+    const double edge = (double)P.thr[Nbin] * (1.0 + 1e-4);
+    P.r2_cut = std::isfinite(edge) && edge * edge < (double)INT_MAX ? (int)(edge * edge) : INT_MAX;
+    return BB_OK;
+}
+
+// Launches one pass (geometry or field sums) and its sum over workgroups:
+// partials in part, the Nbin * (kPsGeom or kPsField) sums in sum.
+hipError_t power_pass(bool geom, const double2 *src, PsParams P, double *part, double *sum,
+                      hipStream_t s)
+{
+    const long long H = P.N / 2 + 1, ntriples = H * H * H;
+    const int n = P.Nbin * (geom ? kPsGeom : kPsField);
+    const int nb = geom ? kPsBlocksGeom : kPsBlocksField;
+    const long long stride = (long long)nb * kPsThreads;
+    P.sK = (int)(stride % H);
+    P.sB = (int)((stride / H) % H);
+    P.sA = (int)(stride / H / H);
+    if (geom)
+        hipLaunchKernelGGL(k_power<true>, dim3(nb), dim3(kPsThreads), 0, s, src, P, ntriples, part);
+    else
+        hipLaunchKernelGGL(k_power<false>, dim3(nb), dim3(kPsThreads), 0, s, src, P, ntriples,
+                           part);
+    hipLaunchKernelGGL(k_power_sum, dim3(n), dim3(kPsThreads), 0, s, part, nb, n, sum);
+    return hipGetLastError();
+}
+
+// The rows {kmean, P0, P2, P4, P6, num} from the field sums fsum
+// [bin][kPsField] and the geometry sums geom [bin][kPsGeom]
+// (normalisation of funcs.c:1055-1071, columns of calpow_mom.c:108-111).
+void power_rows(int NN, int Nbin, double LL, const double *fsum, const double *geom, double *out)
+{
+    const long long N = NN;
+    double acc[kPsMaxBins * kPsAcc];
+    for (int i = 0; i < Nbin; ++i)
+        for (int q = 0; q < kPsAcc; ++q)
+            acc[i * kPsAcc + q] = q < kPsField ? fsum[i * kPsField + q]
+                                               : geom[(size_t)i * kPsGeom + q - kPsField];
+    const double norml = 1. / (1. * N * N * N), pi = M_PI;
+    for (int i = 0; i < Nbin; ++i) {
+        const double *A = acc + (size_t)i * kPsAcc;
+        double *o = out + 6 * i;
+        o[0] = 0.0;
+        o[1] = 0.0;
+        o[2] = A[1];
+        o[3] = A[2];
+        o[4] = A[3];
+        o[5] = A[5];
+        if (A[5] > 0.0) {
+            o[1] = norml * A[0] / (1.0 * A[5] * pow(LL, 3.));
+            o[0] = 2. * pi * A[4] / (A[5] * LL);
+        }
+        if (A[6] > 0.0) o[2] = norml * A[1] / (1.0 * A[6] * pow(LL, 3.));
+        if (A[7] > 0.0) o[3] = norml * A[2] / (1.0 * A[7] * pow(LL, 3.));
+        if (A[8] > 0.0) o[4] = norml * A[3] / (1.0 * A[8] * pow(LL, 3.));
     }
+}
+
+}  // namespace
+
+extern "C" int bb_power_multipoles(const double *field_r2c, int on_device, int NN, int Nbin,
+                                   double LL, double *out)
+{
+    using bb::set_error;
+    if (!field_r2c || !out) return set_error(BB_E_INVALID, "null argument");
+    PsParams P;
+    int rc = power_params(NN, Nbin, LL, P);
+    if (rc) return rc;
+    const long long N = NN, H = N / 2 + 1, nmodes = N * N * H;
     int dev = 0, dev_count = 0;
     if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1 ||
         hipGetDevice(&dev) != hipSuccess) {
@@ -335,7 +395,6 @@ extern "C" int bb_power_multipoles(const double *field_r2c, int on_device, int N
     }
     hipStream_t s = sc.stream;
     double2 *d_field = nullptr;
-    int rc = BB_OK;
     auto fail = [&](int code, const std::string &msg) {
         (void)hipGetLastError();
         if (!rc) rc = set_error(code, msg);
@@ -343,19 +402,8 @@ extern "C" int bb_power_multipoles(const double *field_r2c, int on_device, int N
     // One pass and its sum over workgroups into host memory.
     auto run = [&](bool geom, const double2 *src, double *host) {
         const int n = Nbin * (geom ? kPsGeom : kPsField);
-        const int nb = geom ? kPsBlocksGeom : kPsBlocksField;
-        const long long stride = (long long)nb * kPsThreads;
-        P.sK = (int)(stride % H);
-        P.sB = (int)((stride / H) % H);
-        P.sA = (int)(stride / H / H);
-        if (geom)
-            hipLaunchKernelGGL(k_power<true>, dim3(nb), dim3(kPsThreads), 0, s, src, P, ntriples,
-                               sc.part);
-        else
-            hipLaunchKernelGGL(k_power<false>, dim3(nb), dim3(kPsThreads), 0, s, src, P, ntriples,
-                               sc.part);
-        hipLaunchKernelGGL(k_power_sum, dim3(n), dim3(kPsThreads), 0, s, sc.part, nb, n, sc.sum);
-        if (hipGetLastError() != hipSuccess) return fail(BB_E_DEVICE, "power kernels");
+        if (power_pass(geom, src, P, sc.part, sc.sum, s) != hipSuccess)
+            return fail(BB_E_DEVICE, "power kernels");
         if (hipMemcpyAsync(host, sc.sum, sizeof(double) * n, hipMemcpyDeviceToHost, s) !=
                 hipSuccess ||
             hipStreamSynchronize(s) != hipSuccess)
@@ -387,29 +435,70 @@ extern "C" int bb_power_multipoles(const double *field_r2c, int on_device, int N
         (void)hipFree(d_field);
     }
     if (rc) return rc;
-    double acc[kPsMaxBins * kPsAcc];
-    for (int i = 0; i < Nbin; ++i)
-        for (int q = 0; q < kPsAcc; ++q)
-            acc[i * kPsAcc + q] = q < kPsField ? fsum[i * kPsField + q]
-                                               : geom[(size_t)i * kPsGeom + q - kPsField];
-    // Normalisation, funcs.c:1055-1071; columns of calpow_mom.c:108-111.
-    const double norml = 1. / (1. * N * N * N), pi = M_PI;
-    for (int i = 0; i < Nbin; ++i) {
-        const double *A = acc + (size_t)i * kPsAcc;
-        double *o = out + 6 * i;
-        o[0] = 0.0;
-        o[1] = 0.0;
-        o[2] = A[1];
-        o[3] = A[2];
-        o[4] = A[3];
-        o[5] = A[5];
-        if (A[5] > 0.0) {
-            o[1] = norml * A[0] / (1.0 * A[5] * pow(LL, 3.));
-            o[0] = 2. * pi * A[4] / (A[5] * LL);
-        }
-        if (A[6] > 0.0) o[2] = norml * A[1] / (1.0 * A[6] * pow(LL, 3.));
-        if (A[7] > 0.0) o[3] = norml * A[2] / (1.0 * A[7] * pow(LL, 3.));
-        if (A[8] > 0.0) o[4] = norml * A[3] / (1.0 * A[8] * pow(LL, 3.));
+    power_rows(NN, Nbin, LL, fsum, geom.data(), out);
+    return BB_OK;
+}
+
+// ------------------------------------------------- the no-alloc variant --
+namespace {
+
+// Device scratch of bb_power_multipoles_noalloc: the workgroup partials of
+// one pass (kPsBlocks rows of kPsOut sums), then the geometry sums and the
+// field sums (kPsOut each).  Every pass overwrites the partials it reads, so
+// nothing needs clearing between calls.  Each device holds its own copy.
+constexpr int kNaPart = kPsOut * kPsBlocks;
+constexpr int kNaDoubles = kNaPart + 2 * kPsOut;
+__device__ __align__(256) double scratch[kNaDoubles];
+
+// Device address of `scratch` per device, looked up once per device with that
+// device current (the host-side name is not a device address).
+constexpr int kNaMaxDevices = 64;
+double *na_ptr[kNaMaxDevices] = {};
+std::mutex na_mu;
+
+}  // namespace
+
+extern "C" int bb_power_multipoles_noalloc(const double *d_field_r2c, int NN, int Nbin,
+                                           double LL, void *stream, double *out)
+{
+    using bb::set_error;
+    if (!d_field_r2c || !out) return set_error(BB_E_INVALID, "null argument");
+    PsParams P;
+    int rc = power_params(NN, Nbin, LL, P);
+    if (rc) return rc;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(BB_E_DEVICE, "no HIP device");
     }
+    if (dev < 0 || dev >= kNaMaxDevices)
+        return set_error(BB_E_DEVICE, "device index beyond the supported 64");
+    std::lock_guard<std::mutex> lock(na_mu);
+    if (!na_ptr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return set_error(BB_E_DEVICE, "scratch symbol lookup failed");
+        }
+        na_ptr[dev] = static_cast<double *>(p);
+    }
+    double *part = na_ptr[dev], *gsum = part + kNaPart, *fsum_d = gsum + kPsOut;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const double2 *src = reinterpret_cast<const double2 *>(d_field_r2c);
+    double geom[kPsMaxBins * kPsGeom], fsum[kPsMaxBins * kPsField];
+    if (power_pass(true, nullptr, P, part, gsum, s) != hipSuccess ||
+        power_pass(false, src, P, part, fsum_d, s) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(BB_E_DEVICE, "power kernels");
+    }
+    if (hipMemcpyAsync(geom, gsum, sizeof(double) * Nbin * kPsGeom, hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipMemcpyAsync(fsum, fsum_d, sizeof(double) * Nbin * kPsField, hipMemcpyDeviceToHost,
+                       s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(BB_E_DEVICE, "power kernels failed");
+    }
+    power_rows(NN, Nbin, LL, fsum, geom, out);
     return BB_OK;
 }

@@ -35,7 +35,7 @@ EXPORTED = [
     "bb_estimator_query", "bb_estimator_wait", "bb_estimator_stream", "bb_estimator_set_stream",
     "bb_estimator_contraction_sums", "bb_estimator_triangle_counts", "bb_estimator_field",
     "bb_estimator_timings", "bb_estimator_worklist", "bb_write_outputs", "bb_write_outputs_checked", "bb_export_cache",
-    "bb_power_multipoles",
+    "bb_power_multipoles", "bb_power_multipoles_noalloc",
     "bb_server_create", "bb_server_destroy", "bb_submit", "bb_poll", "bb_wait", "bb_server_pause",
     "bb_server_prepare",
     "bb_server_stats",
@@ -156,6 +156,8 @@ def lib() -> ctypes.CDLL:
         L.bb_export_cache.argtypes = [ctypes.c_void_p, _dp, ctypes.c_char_p, ctypes.c_int]
         L.bb_power_multipoles.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_double, _dp]
+        L.bb_power_multipoles_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                                  ctypes.c_double, ctypes.c_void_p, _dp]
         L.bb_write_outputs.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_char_p,
                                        ctypes.c_int]
         L.bb_write_outputs_checked.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_char_p,

@@ -351,6 +351,22 @@ int bb_export_cache(bb_estimator *e, const double *field_r2c, const char *dir, i
  * and that stream until the process ends; calls are serialised. */
 int bb_power_multipoles(const double *field_r2c, int on_device, int NN, int Nbin, double LL,
                         double *out);
+/* No counterpart in the reference: bb_power_multipoles for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  It calls no hipMalloc, hipFree,
+ * stream creation or allocating library routine: its partial sums live in a
+ * static device array of the library (about 0.4 MB per device, held until
+ * the process ends), and its kernels run on the caller's `stream`
+ * (hipStream_t; NULL: the null stream), which is synchronised before return.
+ * d_field_r2c: [NN][NN][NN+2] doubles in device memory, complete on `stream`
+ * (or before the call).  Limits: Nbin <= 10 (the array holds 10 bins), NN
+ * even in 4..4096; larger sizes fail with BB_E_INVALID.  out: as
+ * bb_power_multipoles, bit for bit.  Every call on a device shares that
+ * device's array (the geometry sums are recomputed each call, about 40 us at
+ * NN=384); the library serialises calls, so calls from several threads do
+ * not overlap on the array. */
+int bb_power_multipoles_noalloc(const double *d_field_r2c, int NN, int Nbin, double LL,
+                                void *stream, double *out);
 
 #ifdef __cplusplus
 }
