@@ -153,19 +153,51 @@ int main(int argc, char *argv[])
     char dir[512];
     snprintf(dir, sizeof dir, "ResFiles_N%ld", NN);
     mkdir(dir, 0777);
+    printf("plan (ring triples, contraction list) in  time %f\n", t_plan - t0);
+    printf("device setup and triangle counts in  time %f\n", t_est - t_plan);
+    printf("plane gather in  time %f\n", ms[0] * 1e-3);
+    printf("plane-ring IFFTs in  time %f\n", ms[1] * 1e-3);
+    printf("contraction in  time %f\n", ms[2] * 1e-3);
+    printf("binning in  time %f\n", ms[3] * 1e-3);
+    printf("whole run in  time %f\n", t_done - t0);
+
+    /* Timing log in the reference's format (barbie.c:216-221, 368-377): one
+     * "loop <p1> out of <Nmax>" line per plane p1, then "Total time" of the
+     * loop.  The estimate runs every p1 in the same launches, so each line
+     * carries p1's share of the contraction and binning stages, in
+     * proportion to the grid points its contractions sum. */
+    double *share = calloc((size_t)st.Nmax, sizeof(double));
+    {
+        int *rec = malloc(sizeof(int) * 8 * (size_t)st.ncontr);
+        int *clevel = malloc(sizeof(int) * (size_t)st.ncontr);
+        int nlev = 0, grids[BB_MAX_LEVELS];
+        bb_options o;
+        memset(&o, 0, sizeof o);
+        o.precision = prec;
+        if (share && rec && clevel && !bb_plan_contractions(plan, rec) &&
+            !bb_plan_grid_levels(plan, &o, &nlev, grids, clevel)) {
+            double all = 0.0;
+            for (long long c = 0; c < st.ncontr; ++c) {
+                const double w = (double)grids[clevel[c]] * grids[clevel[c]];
+                share[rec[8 * c]] += w;
+                all += w;
+            }
+            for (int q = 0; q < st.Nmax && all > 0; ++q) share[q] /= all;
+        }
+        free(rec);
+        free(clevel);
+    }
+    const double t_loop = (ms[2] + ms[3]) * 1e-3;
     snprintf(fname, sizeof fname, "%s/time_%ld_R%d.txt", dir, NN, NRi);
     fp = fopen(fname, "w");
     if (fp) {
-        fprintf(fp, "plan (ring triples, contraction list) in  time %f\n", t_plan - t0);
-        fprintf(fp, "device setup and triangle counts in  time %f\n", t_est - t_plan);
-        fprintf(fp, "plane gather in  time %f\n", ms[0] * 1e-3);
-        fprintf(fp, "plane-ring IFFTs in  time %f\n", ms[1] * 1e-3);
-        fprintf(fp, "contraction in  time %f\n", ms[2] * 1e-3);
-        fprintf(fp, "binning in  time %f\n", ms[3] * 1e-3);
-        fprintf(fp, "Total time %f\n", t_done - t0);
+        for (int q = 0; q < st.Nmax; ++q)
+            fprintf(fp, "loop %d out of %d in  time %f\n", q, st.Nmax, share ? t_loop * share[q] : 0.0);
+        fprintf(fp, "Total time %f\n", t_loop);
         fclose(fp);
     }
-    printf("Total time %f\n", t_done - t0);
+    free(share);
+    printf("Total time %f\n", t_loop);
 
     const char *chk = getenv("BARBIE_CHECK_DENOMINATORS");
     if (chk && atoi(chk)) {

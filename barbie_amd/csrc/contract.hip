@@ -62,13 +62,9 @@ template <int NT>
 __device__ __forceinline__ void row_tables(const WorkItem &it, int *row_slot, int *row_split)
 {
     for (int r = threadIdx.x; r < it.nrows; r += NT) {
-        int b = 0, start = 0;
-        while (b < it.nblk - 1 && r >= start + it.blk_rows[b]) {
-            start += it.blk_rows[b];
-            ++b;
-        }
-        row_slot[r] = it.blk_slot[b] + (r - start);
-        row_split[r] = it.blk_split[b];
+        const int v = it.rslot[r];
+        row_slot[r] = v & (kRowSplit - 1);
+        row_split[r] = (v & kRowSplit) ? 1 : 0;
     }
 }
 
@@ -472,7 +468,9 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_contract_bf16x3(
     for (int j = 0; j < TWW; ++j) {
         // Empty rows and the columns past the window's rings read the zero
         // ring (slot nrows): their products are zeros, which cost the same
-        // instructions but less power than arbitrary data.
+        // instructions but less power than arbitrary data.  (Reading distinct
+        // staged rings there instead avoids the LDS pass that many lanes on
+        // one address cost, but the kernel is power-bound: it ran slower.)
         const bool row_ok = TW[j].a1[m] >= 0, col_ok = m < TW[j].nb;
         oa1[j] = dma_off(row_ok ? (int)TW[j].a1[m] : nrows, 4 * kq);
         oa2[j] = dma_off(row_ok ? (int)TW[j].a2[m] : nrows, 4 * kq);

@@ -114,7 +114,13 @@ struct WorkItem {
     int16_t nblk, nrows;
     int32_t tw0;                 // first of kWaves * kTWW reserved TWins
     int8_t cnt[kWaves];          // TWins per wave
+    // Staged row r: field slot (in the level) of its ring, with kRowSplit set
+    // when the row is staged as bf16 hi/lo B operands.  Rows need not follow
+    // the blocks' order: the BF16x3 layout places pair rings so that the 16
+    // rings of a tile's A rows fall into distinct LDS bank groups.
+    int32_t rslot[kRowCap];
 };
+constexpr int32_t kRowSplit = 1 << 30;
 
 // One workgroup of the contraction launch: chunks [cb, ce) of an item's
 // points, written to partial tile `split` of each of its windows.

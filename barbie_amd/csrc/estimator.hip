@@ -223,6 +223,93 @@ struct Layout {
 
 namespace {
 
+// LDS rows of a work item's staged rings.  Rows are read by the
+// contraction as 16 lanes at a time, one ring per lane, and the BF16x3
+// layout puts bytes 16 (r mod 16) .. +15 of every 256 of row r into bank
+// group r mod 16 (dma_off in contract.hip): lanes that read distinct rings of
+// the same group wait for each other.  The B operand reads 16 consecutive
+// rows (one per group) and is conflict-free; the A rows (the D1 and the D2
+// rings of a pair block) are not consecutive.  With split_b (BF16x3), the B
+// rings take the first rows as contiguous ranges, and the pair rings the
+// remaining rows, assigned greedily so that the distinct D1 rings of each
+// block, and its distinct D2 rings, fall into distinct groups where
+// possible (empty A rows read the zero ring, row nrows, which counts as a
+// member of their block).  Otherwise every range is contiguous, in key order.
+template <class KeyMap, class Key>
+void assign_rows(const std::vector<PairBlock> &blocks, const std::vector<size_t> &item_blocks,
+                 const KeyMap &keys, bool split_b, int nrows, std::map<Key, int> &row0,
+                 std::map<std::pair<int, int>, int> &prow)
+{
+    int next = 0;
+    if (!split_b) {
+        for (auto &kv : keys) {
+            row0[kv.first] = next - kv.second.lo;
+            for (int r = kv.second.lo; r <= kv.second.hi; ++r) prow[{kv.first.plane, r}] = next + r - kv.second.lo;
+            next += kv.second.hi - kv.second.lo + 1;
+        }
+        return;
+    }
+    std::vector<std::pair<int, int>> rings;   // pair rings in key order
+    for (auto &kv : keys) {
+        if (kv.first.split) {
+            row0[kv.first] = next - kv.second.lo;
+            next += kv.second.hi - kv.second.lo + 1;
+        } else {
+            for (int r = kv.second.lo; r <= kv.second.hi; ++r) rings.push_back({kv.first.plane, r});
+        }
+    }
+    const int first = next, zero_group = nrows & 15;
+    std::map<std::pair<int, int>, int> id;
+    for (size_t k = 0; k < rings.size(); ++k) id[rings[k]] = (int)k;
+    // Cliques: per block, the distinct D1 rings and the distinct D2 rings.
+    std::vector<std::vector<int>> member(rings.size());   // ring -> cliques
+    std::vector<std::array<int, 16>> count;               // clique -> rings per group
+    for (size_t bi : item_blocks) {
+        const PairBlock &B = blocks[bi];
+        for (int side = 0; side < 2; ++side) {
+            std::vector<int> ids;
+            bool empty = false;
+            for (int m = 0; m < 16; ++m) {
+                if (B.r1[m] < 0) {
+                    empty = true;
+                    continue;
+                }
+                ids.push_back(side ? id.at({B.pb[m], B.r2[m]}) : id.at({B.pa[m], B.r1[m]}));
+            }
+            std::sort(ids.begin(), ids.end());
+            ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+            const int q = (int)count.size();
+            count.push_back({});
+            count.back().fill(0);
+            if (empty) count.back()[zero_group] = 1;
+            for (int x : ids) member[x].push_back(q);
+        }
+    }
+    std::array<std::vector<int>, 16> free_rows;
+    for (int r = first + (int)rings.size() - 1; r >= first; --r) free_rows[r & 15].push_back(r);
+    std::vector<int> order(rings.size());
+    for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return member[a].size() > member[b].size(); });
+    for (int x : order) {
+        int best = -1;
+        long long best_cost = 0;
+        for (int g = 0; g < 16; ++g) {
+            if (free_rows[g].empty()) continue;
+            long long c = 0;
+            for (int q : member[x]) c += count[q][g];
+            if (best < 0 || c < best_cost ||
+                (c == best_cost && free_rows[g].size() > free_rows[best].size())) {
+                best = g;
+                best_cost = c;
+            }
+        }
+        prow[rings[x]] = free_rows[best].back();
+        free_rows[best].pop_back();
+        for (int q : member[x]) count[q][best]++;
+    }
+}
+
 // Packs pair blocks of one level (already grouped so that neighbours share
 // planes) into work items.  split_b: stage the r3 rings as bf16 hi/lo
 // operands (BF16x3) instead of raw values.  pos[cid] receives tile * 256 +
@@ -266,14 +353,14 @@ int pack_items(std::vector<PairBlock> &blocks, const Level &L, bool split_b, int
             load[w] += B.nwin;
             wave_blocks[w].push_back(b);
         }
-        // Lay out the item: blocks in key order.
+        // Lay out the item: the staged field ranges in key order (blk_*), and
+        // the LDS row of every staged ring.
         WorkItem it;
         std::memset(&it, 0, sizeof it);
         it.off = L.off;
         it.npts = L.npts;
         it.fstride = L.stride;
         it.scale = L.scale;
-        std::map<Key, int> row0;
         int nrows = 0;
         for (auto &kv : keys) {
             const int bi = it.nblk++;
@@ -283,10 +370,22 @@ int pack_items(std::vector<PairBlock> &blocks, const Level &L, bool split_b, int
             it.blk_slot[bi] = f0;
             it.blk_rows[bi] = (int16_t)(kv.second.hi - kv.second.lo + 1);
             it.blk_split[bi] = (int16_t)kv.first.split;
-            row0[kv.first] = nrows - kv.second.lo;   // row of ring r = row0 + r
             nrows += it.blk_rows[bi];
         }
         it.nrows = (int16_t)nrows;
+        std::vector<size_t> item_blocks;
+        for (int w = 0; w < waves; ++w)
+            item_blocks.insert(item_blocks.end(), wave_blocks[w].begin(), wave_blocks[w].end());
+        std::map<Key, int> row0;             // B operand rings: row of ring r = row0 + r
+        std::map<std::pair<int, int>, int> prow;   // pair rings (plane, ring) -> row
+        assign_rows(blocks, item_blocks, keys, split_b, nrows, row0, prow);
+        for (auto &kv : keys) {
+            for (int r = kv.second.lo; r <= kv.second.hi; ++r) {
+                const bool pair_ring = !(split_b && kv.first.split);
+                const int row = pair_ring ? prow.at({kv.first.plane, r}) : row0.at(kv.first) + r;
+                it.rslot[row] = L.fid(kv.first.plane, r) | (kv.first.split ? kRowSplit : 0);
+            }
+        }
         it.tw0 = (int32_t)W.tws.size();
         W.max_rows = std::max(W.max_rows, nrows);
         {
@@ -308,8 +407,8 @@ int pack_items(std::vector<PairBlock> &blocks, const Level &L, bool split_b, int
                     TWin &T = W.tws[it.tw0 + w * kTWW + j];
                     std::memset(&T, 0, sizeof T);
                     for (int m = 0; m < 16; ++m) {
-                        T.a1[m] = (int16_t)(B.r1[m] >= 0 ? row0.at({B.pa[m], 0}) + B.r1[m] : -1);
-                        T.a2[m] = (int16_t)(B.r2[m] >= 0 ? row0.at({B.pb[m], 0}) + B.r2[m] : 0);
+                        T.a1[m] = (int16_t)(B.r1[m] >= 0 ? prow.at({B.pa[m], B.r1[m]}) : -1);
+                        T.a2[m] = (int16_t)(B.r2[m] >= 0 ? prow.at({B.pb[m], B.r2[m]}) : 0);
                     }
                     const int ws = B.lo + 16 * v;
                     T.b0 = (int16_t)(rc + ws);
@@ -467,6 +566,20 @@ int check_lists(const Layout &L, size_t ncontr)
         }
         if (rows != it.nrows || it.nrows > kRowCap || it.nblk > kMaxBlk || item_sp[i] < 1)
             return set_error(BB_E_INVALID, "internal: work item row blocks are inconsistent");
+        {
+            // every staged field appears on exactly one row
+            std::vector<long long> want, got;
+            for (int b = 0; b < it.nblk; ++b)
+                for (int r = 0; r < it.blk_rows[b]; ++r)
+                    want.push_back(((long long)it.blk_slot[b] + r) * 2 + (it.blk_split[b] ? 1 : 0));
+            for (int r = 0; r < it.nrows; ++r)
+                got.push_back((long long)(it.rslot[r] & (kRowSplit - 1)) * 2 +
+                              ((it.rslot[r] & kRowSplit) ? 1 : 0));
+            std::sort(want.begin(), want.end());
+            std::sort(got.begin(), got.end());
+            if (want != got || std::adjacent_find(got.begin(), got.end()) != got.end())
+                return set_error(BB_E_INVALID, "internal: staged rows do not match the fields");
+        }
         for (int w = 0; w < kWaves; ++w) {
             const int limit = w < L.waves ? std::min(item_tww[i], (int)kTWW) : 0;
             if (it.cnt[w] < 0 || it.cnt[w] > limit)

@@ -22,6 +22,9 @@
  *    (the reference loop at barbie.c:111-114 writes past the array).
  *
  * Parity status of each function (how it is pinned) is stated next to it.
+ * orc_input_ps, orc_sort and orc_cal_coeff are also compared bit for bit
+ * with the reference's own functions, compiled from its funcs.c into
+ * oracle/_ref by `make ref` (the rest of the reference needs FFTW and GSL).
  */
 #include <math.h>
 #include <stdio.h>
@@ -212,8 +215,27 @@ void orc_cal_coeff(double px, double pz, double *bs, double *norm)
     norm[15] = pow(-1. + pow(pz, 2), 6);
 }
 
-/* input_ps, funcs.c:73-81 */
-static double input_ps(double kk) { return 1.0 * pow(kk, -2.0); }
+/* input_ps, funcs.c:73-81.  Pinned bit for bit against the reference's own
+ * function (tests/test_golden.py). */
+double orc_input_ps(double kk) { return 1.0 * pow(kk, -2.0); }
+#define input_ps orc_input_ps
+
+/* Triangle parameters of a sorted K[3][2] = {(|k|, k_z)}, barbie.c:293-297:
+ * out = {K1, T, MU, pz, px}.  Pinned by closed 3-D triangles: MU is minus
+ * the cosine between the two largest sides, pz the z cosine of the largest
+ * and px the z component of the unit vector in the triangle's plane
+ * perpendicular to the largest side (tests/test_golden.py). */
+void orc_triangle_params(const double K[3][2], double *out)
+{
+    double T = K[1][0] / K[0][0];
+    double MU = 0.5 * (T + 1. / T - (K[2][0] / K[0][0]) * (K[2][0] / K[1][0]));
+    double pz = K[0][1] / K[0][0];
+    out[0] = K[0][0];
+    out[1] = T;
+    out[2] = MU;
+    out[3] = pz;
+    out[4] = ((K[1][1] / K[1][0]) + MU * pz) / (sqrt(1. - (MU * MU)));
+}
 
 /* ---------------------------------------------------------------------- */
 /* The estimator loop, barbie.c:203-373.
@@ -469,11 +491,9 @@ static long estimate_impl(const orc_params *P, const double *low, const double *
                                       {sqrt(avg[r3] * avg[r3] + p3 * p3), p3}};
                     orc_sort(K);
                     if (K[0][0] > (LL_k1) && K[0][0] < UL_k1) {
-                        double K1 = K[0][0];
-                        double T = K[1][0] / K[0][0];
-                        double MU = 0.5 * (T + 1. / T - (K[2][0] / K[0][0]) * (K[2][0] / K[1][0]));
-                        double pz = K[0][1] / K[0][0];
-                        double px = ((K[1][1] / K[1][0]) + MU * pz) / (sqrt(1. - (MU * MU)));
+                        double tp[5];
+                        orc_triangle_params((const double(*)[2])K, tp);
+                        double K1 = tp[0], T = tp[1], MU = tp[2], pz = tp[3], px = tp[4];
                         double Num = rdata[rp * 4 + 3];
                         /* barbie.c:301-305 */
                         double mu_1 = K[0][1] / K[0][0];

@@ -45,6 +45,9 @@ def lib():
         L.orc_write_outputs.argtypes = [ctypes.POINTER(OrcParams), _dp, _dp, _dp, ctypes.c_char_p,
                                         ctypes.c_int]
         L.orc_calpow_mom.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double, _dp]
+        L.orc_input_ps.argtypes = [ctypes.c_double]
+        L.orc_input_ps.restype = ctypes.c_double
+        L.orc_triangle_params.argtypes = [_dp, _dp]
         L.orc_write_plane_cache.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp,
                                             _dp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_void_p]
@@ -191,3 +194,15 @@ def estimate_disk(params, lo, hi, av, rdata, directory, NRi, p1_begin=0, p1_end=
     if record:
         contr = contr[:n].copy()
     return bs, nm, bi, n, contr
+
+
+def input_ps(kk):
+    return lib().orc_input_ps(float(kk))
+
+
+def triangle_params(K):
+    """barbie.c:293-297 on a sorted {(|k|, k_z)} x 3: (K1, T, MU, pz, px)."""
+    a = np.ascontiguousarray(K, dtype=np.float64).reshape(3, 2)
+    out = np.zeros(5)
+    lib().orc_triangle_params(_p(a), _p(out))
+    return out

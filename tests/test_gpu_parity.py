@@ -227,8 +227,14 @@ def test_cli_end_to_end(tmp_path):
     dump = np.fromfile(outdir / f"BS_bins_N{g.NN}_R7.f64").reshape(3, -1)
     assert rel_err(dump[0], o["bs"]) <= 1e-10
     assert rel_err(dump[1], o["nm"]) <= 1e-12
-    assert (outdir / f"time_{g.NN}_R7.txt").read_text().strip().splitlines()[-1].startswith(
-        "Total time")
+    # timing log in the reference's format (barbie.c:368-377): one line per
+    # plane p1, then the loop's total
+    lines = (outdir / f"time_{g.NN}_R7.txt").read_text().strip().splitlines()
+    assert len(lines) == g.Nmax + 1 and lines[-1].startswith("Total time ")
+    t = [float(ln.split("in  time ")[1]) for ln in lines[:-1]]
+    for q, ln in enumerate(lines[:-1]):
+        assert ln.startswith(f"loop {q} out of {g.Nmax} in  time "), ln
+    assert abs(sum(t) - float(lines[-1].split()[-1])) <= 1e-5 + 1e-3 * sum(t)
 
 
 def test_cli_split_precision_with_checked_denominators(tmp_path):
