@@ -244,4 +244,9 @@ __device__ __forceinline__ double wave_sum(double v)
 int tune_layout(const bb_plan *p, const bb_options *base, const double *d_field, int repeats,
                 bool allow_mix, bb_options *best, double *best_ms);
 
+// Radices and division constants of a length-N transform (false: N is not
+// 2,3,5-smooth or too long), and its twiddles exp(+2 pi i t / N).
+bool fft_desc(int N, FftDesc &fd);
+std::vector<double2> fft_twiddles(int N);
+
 }  // namespace bb

@@ -8,11 +8,15 @@ workload's own parameters: P(k) = A k^-1.5 in grid units (A = 1e3), modes
 with k >= NN/3 and k = 0 set to zero, delta = dG + f (dG^2 - <dG^2>) with
 f = 0.3, then Kaiser (1 + beta mu^2), beta = 1, mu = kz/k, cell size 1.
 The reference draws its Gaussian modes with GSL's mt19937 + ziggurat (absent
-from this image); here numpy's PCG64 with seed 100020 + 146*NRi
-(generate_field.c:47) draws a real white-noise cube whose FFT gives
-Hermitian Gaussian modes with <|Delta|^2> = V P(k).
+from this image).  Two generators here, both seeded with 100020 + 146*NRi
+(generate_field.c:47):
+  * make_field (host, numpy): PCG64 draws a real white-noise cube whose FFT
+    gives Hermitian Gaussian modes with <|Delta|^2> = V P(k);
+  * make_field_device (GPU): the library's bb_generate_field, which draws
+    every Fourier mode from a Philox4x64-10 counter-based stream and runs
+    the same pipeline in HIP.
 
-This is an input tool (the step before the estimator), not part of the
+These are input tools (the step before the estimator), not part of the
 estimator's hot path.
 """
 from __future__ import annotations
@@ -65,31 +69,16 @@ def read_field(path: str, NN: int) -> np.ndarray:
 def make_field_device(NN: int, NRi: int = 1, fnl: float = 0.3, amp: float = 1e3,
                       index: float = -1.5, beta: float = 1.0, kcut_frac: float = 1.0 / 3.0,
                       device: str = "cuda"):
-    """The construction of make_field on the GPU with PyTorch (white noise
-    from torch's generator seeded with seed_for(NRi), so the realization
-    differs from make_field's while the statistics are the same).  Returns a
-    float64 tensor (NN, NN, NN+2) on `device`.  A workload tool for sizes
-    where the host generator is slow; parity tests use make_field."""
+    """The same construction on the GPU by the library's own generator
+    (bb_generate_field: a Philox4x64-10 stream with seed seed_for(NRi), one
+    block per Fourier mode, then the c2r / non-Gaussian / r2c / Kaiser steps
+    in HIP), so the realization differs from make_field's while the
+    statistics are the same.  Returns a float64 tensor (NN, NN, NN+2) on
+    `device`, complete when returned."""
     import torch
-    gen = torch.Generator(device=device)
-    gen.manual_seed(seed_for(NRi))
-    white = torch.randn((NN, NN, NN), generator=gen, dtype=torch.float64, device=device)
-    W = torch.fft.rfftn(white)
-    del white
-    k = torch.fft.fftfreq(NN, d=1.0 / NN, dtype=torch.float64, device=device)
-    kz = torch.arange(NN // 2 + 1, dtype=torch.float64, device=device)
-    k2 = k[:, None, None] ** 2 + k[None, :, None] ** 2 + kz[None, None, :] ** 2
-    kk = torch.sqrt(k2)
-    pk = torch.where((kk > 0) & (kk < NN * kcut_frac), amp * kk.clamp_min(1e-300) ** index,
-                     torch.zeros_like(kk))
-    W *= torch.sqrt(pk)
-    del pk
-    dG = torch.fft.irfftn(W, s=(NN, NN, NN))
-    del W
-    var = torch.mean(dG * dG)
-    dG += fnl * (dG * dG - var)
-    D = torch.fft.rfftn(dG)
-    del dG
-    mu2 = torch.where(kk > 0, kz[None, None, :] ** 2 / k2.clamp_min(1e-300), torch.zeros_like(kk))
-    D *= (1.0 + beta * mu2)
-    return torch.view_as_real(D).reshape(NN, NN, NN + 2).contiguous()
+    from .native import generate_field
+    out = torch.empty((NN, NN, NN + 2), dtype=torch.float64, device=device)
+    torch.cuda.synchronize()
+    generate_field(NN, seed_for(NRi), out=out.data_ptr(), amp=amp, index=index, kcut=NN * kcut_frac,
+                   fnl=fnl, beta=beta)
+    return out

@@ -35,7 +35,7 @@ EXPORTED = [
     "bb_estimator_query", "bb_estimator_wait", "bb_estimator_stream", "bb_estimator_set_stream",
     "bb_estimator_contraction_sums", "bb_estimator_triangle_counts", "bb_estimator_field",
     "bb_estimator_timings", "bb_estimator_worklist", "bb_write_outputs", "bb_write_outputs_checked", "bb_export_cache",
-    "bb_power_multipoles", "bb_power_multipoles_noalloc",
+    "bb_power_multipoles", "bb_power_multipoles_noalloc", "bb_generate_field",
     "bb_server_create", "bb_server_destroy", "bb_submit", "bb_poll", "bb_wait", "bb_server_pause",
     "bb_server_prepare",
     "bb_server_stats",
@@ -80,6 +80,12 @@ class BBServerOptions(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_inflight", ctypes.c_int),
                 ("max_batch", ctypes.c_int), ("precision", ctypes.c_int), ("tune", ctypes.c_int),
                 ("reserved_cus", ctypes.c_int)]
+
+
+class BBFieldParams(ctypes.Structure):
+    _fields_ = [("amp", ctypes.c_double), ("index", ctypes.c_double), ("kunit", ctypes.c_double),
+                ("kcut", ctypes.c_double), ("fnl", ctypes.c_double), ("beta", ctypes.c_double),
+                ("LL", ctypes.c_double), ("stage", ctypes.c_int)]
 
 
 class BarbieError(RuntimeError):
@@ -158,6 +164,9 @@ def lib() -> ctypes.CDLL:
                                           ctypes.c_double, _dp]
         L.bb_power_multipoles_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                                   ctypes.c_double, ctypes.c_void_p, _dp]
+        L.bb_generate_field.argtypes = [ctypes.c_int, ctypes.c_ulonglong,
+                                        ctypes.POINTER(BBFieldParams), ctypes.c_void_p,
+                                        ctypes.c_int]
         L.bb_write_outputs.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_char_p,
                                        ctypes.c_int]
         L.bb_write_outputs_checked.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_char_p,
@@ -515,3 +524,21 @@ class Server:
         o = (ctypes.c_longlong * len(SERVER_STATS_KEYS))()
         _check(lib().bb_server_stats(self.handle, o))
         return dict(zip(SERVER_STATS_KEYS, list(o)))
+
+
+def generate_field(NN: int, seed: int, out=None, amp: float = 1e3, index: float = -1.5,
+                   kunit: float = 1.0, kcut=None, fnl: float = 0.3, beta: float = 1.0,
+                   LL: float = 1.0, stage: int = 0):
+    """bb_generate_field: the seeded non-Gaussian redshift-space field on the
+    GPU, in the r2c layout [NN][NN][NN+2] doubles.  out: None (returns a
+    numpy array) or a device address (int) to write into.  kcut None: NN/3."""
+    p = BBFieldParams(float(amp), float(index), float(kunit),
+                      float(NN / 3.0 if kcut is None else kcut), float(fnl), float(beta),
+                      float(LL), int(stage))
+    if out is None:
+        host = np.empty((NN, NN, NN + 2))
+        _check(lib().bb_generate_field(int(NN), int(seed), ctypes.byref(p),
+                                       host.ctypes.data_as(ctypes.c_void_p), 0))
+        return host
+    _check(lib().bb_generate_field(int(NN), int(seed), ctypes.byref(p), ctypes.c_void_p(out), 1))
+    return None

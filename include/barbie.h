@@ -368,6 +368,34 @@ int bb_power_multipoles(const double *field_r2c, int on_device, int NN, int Nbin
 int bb_power_multipoles_noalloc(const double *d_field_r2c, int NN, int Nbin, double LL,
                                 void *stream, double *out);
 
+/* Seeded input field on the GPU: the step before the estimator, the
+ * reference's generator (generate_field.c:22-107: Fill_Modes, Make_Image,
+ * Non_Gauss, Get_Modes, RSD of ngrf_2d_funcs.c:341-577, 734-765) with a
+ * counter-based random stream (Philox4x64-10, one block per Fourier mode,
+ * Box-Muller) in place of GSL's mt19937 + ziggurat, which this build cannot
+ * reproduce.  Output: the FFTW in-place r2c buffer [NN][NN][NN+2] doubles
+ * that barbie reads (host memory, or device memory when on_device != 0).
+ * Fourier modes get Re, Im ~ N(0, V P(k) / 2), V = (NN LL)^3, P(k) = amp
+ * k^index with k = kunit |(ia, ja, kz)| (kunit 1: grid units; the
+ * reference's 2 pi / (NN LL)), zero at k = 0 and for k >= kcut (kcut <= 0:
+ * no cut); the kz = 0 and NN/2 planes are Hermitian.  Then delta = c2r / V,
+ * delta += fnl (delta^2 - <delta^2>), times LL^3, r2c, times 1 + beta mu^2.
+ * stage 1 stops after the Gaussian modes (before the division by V).
+ * NN: even, 4..1024, prime factors 2, 3, 5.  The work runs on a stream of
+ * the library's own, synchronised before return; calls are serialised. */
+typedef struct bb_field_params {
+    double amp;     /* P(k) amplitude (the workload: 1e3) */
+    double index;   /* P(k) exponent (-1.5) */
+    double kunit;   /* k per grid index (1) */
+    double kcut;    /* modes with k >= kcut are zero (NN/3); <= 0: none */
+    double fnl;     /* non-Gaussian coefficient (0.3) */
+    double beta;    /* Kaiser factor (1) */
+    double LL;      /* cell size (1; the reference's generator uses 0.56) */
+    int stage;      /* 0: the field; 1: the Gaussian Fourier modes only */
+} bb_field_params;
+int bb_generate_field(int NN, unsigned long long seed, const bb_field_params *p,
+                      double *field_r2c, int on_device);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,6 +20,12 @@ class OrcParams(ctypes.Structure):
                 ("beta", ctypes.c_double)]
 
 
+class OrcFieldParams(ctypes.Structure):
+    _fields_ = [("amp", ctypes.c_double), ("index", ctypes.c_double), ("kunit", ctypes.c_double),
+                ("kcut", ctypes.c_double), ("fnl", ctypes.c_double), ("beta", ctypes.c_double),
+                ("LL", ctypes.c_double), ("stage", ctypes.c_int)]
+
+
 _lib = None
 
 
@@ -45,6 +51,15 @@ def lib():
         L.orc_write_outputs.argtypes = [ctypes.POINTER(OrcParams), _dp, _dp, _dp, ctypes.c_char_p,
                                         ctypes.c_int]
         L.orc_calpow_mom.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double, _dp]
+        L.orc_philox4x64.argtypes = [ctypes.POINTER(ctypes.c_ulonglong),
+                                     ctypes.POINTER(ctypes.c_ulonglong),
+                                     ctypes.POINTER(ctypes.c_ulonglong)]
+        L.orc_gen_log.argtypes = [ctypes.c_double]
+        L.orc_gen_log.restype = ctypes.c_double
+        L.orc_gen_sincos_2pi.argtypes = [ctypes.c_double, _dp, _dp]
+        L.orc_gen_normal_pair.argtypes = [ctypes.c_ulonglong, ctypes.c_ulonglong, _dp]
+        L.orc_generate_field.argtypes = [ctypes.c_int, ctypes.c_ulonglong,
+                                         ctypes.POINTER(OrcFieldParams), _dp]
         L.orc_input_ps.argtypes = [ctypes.c_double]
         L.orc_input_ps.restype = ctypes.c_double
         L.orc_triangle_params.argtypes = [_dp, _dp]
@@ -205,4 +220,38 @@ def triangle_params(K):
     a = np.ascontiguousarray(K, dtype=np.float64).reshape(3, 2)
     out = np.zeros(5)
     lib().orc_triangle_params(_p(a), _p(out))
+    return out
+
+
+def philox4x64(ctr, key):
+    c = (ctypes.c_ulonglong * 4)(*[int(x) for x in ctr])
+    k = (ctypes.c_ulonglong * 2)(*[int(x) for x in key])
+    o = (ctypes.c_ulonglong * 4)()
+    lib().orc_philox4x64(c, k, o)
+    return [int(x) for x in o]
+
+
+def gen_log(x):
+    return lib().orc_gen_log(float(x))
+
+
+def gen_sincos_2pi(u):
+    s, c = ctypes.c_double(), ctypes.c_double()
+    lib().orc_gen_sincos_2pi(float(u), ctypes.byref(s), ctypes.byref(c))
+    return s.value, c.value
+
+
+def gen_normal_pair(c, seed):
+    g = np.zeros(2)
+    lib().orc_gen_normal_pair(int(c), int(seed), _p(g))
+    return g
+
+
+def generate_field(NN, seed, amp=1e3, index=-1.5, kunit=1.0, kcut=None, fnl=0.3, beta=1.0,
+                   LL=1.0, stage=0):
+    """The generator's CPU restatement: [NN][NN][NN+2] doubles (r2c layout)."""
+    P = OrcFieldParams(amp, index, kunit, NN / 3.0 if kcut is None else kcut, fnl, beta, LL,
+                       stage)
+    out = np.zeros((NN, NN, NN + 2))
+    assert lib().orc_generate_field(NN, int(seed), ctypes.byref(P), _p(out)) == 0
     return out
