@@ -22,6 +22,14 @@
  * funcs.c:328-415, 537-566) into ResFiles_N<NN>/data_files, which the
  * reference computes on its first run; this build keeps them in HBM and
  * writes them only on request.
+ * BARBIE_READ_CACHE=1 reads existing caches back before the estimate, as
+ * the reference does when its sentinel files exist (barbie.c:161-196): the
+ * ring triples and triangle counts of closed_rings_<NN> / rcountFor_<NN>
+ * are checked against this geometry (and its key, when cache_config_* is
+ * present) and their Num values used; a mismatch exits with status 2
+ * (docs/REVIEW.md:9).  The plane-ring fields are always built from the
+ * field; existing dataPlaneRing_* files are compared with them on a sample
+ * and a warning names a stale cache.
  * BARBIE_CHECK_DENOMINATORS=1 writes the multipole files with each
  * multipole's own denominator checked (docs/REVIEW.md:11): entries without a
  * positive finite denominator are written as "nan" and counted on stderr.
@@ -140,6 +148,34 @@ int main(int argc, char *argv[])
     }
     size_t nb = (size_t)st.nbins * BB_NPOLES;
     double *bins = calloc(3 * nb, sizeof(double));
+    const char *rcache = getenv("BARBIE_READ_CACHE");
+    if (rcache && atoi(rcache) != 0) {
+        char cdir[600], sentinel[800];
+        snprintf(cdir, sizeof cdir, "ResFiles_N%ld/data_files", NN);
+        snprintf(sentinel, sizeof sentinel, "%s/closed_rings_%ld", cdir, NN);
+        FILE *sf = fopen(sentinel, "rb");
+        if (sf) {
+            fclose(sf);
+            long long bad = 0;
+            if (bb_import_rings(est, cdir, NRi, 1e-9, 1, &bad)) {
+                fprintf(stderr, "barbie: %s\n", bb_last_error());
+                return 2;
+            }
+            printf("Ring triples and triangle counts read from %s\n", cdir);
+        }
+        snprintf(sentinel, sizeof sentinel, "%s/dataPlaneRing_%ld_0_0_R%d", cdir, NN, NRi);
+        sf = fopen(sentinel, "rb");
+        if (sf) {
+            fclose(sf);
+            double worst = 0.0;
+            if (bb_check_plane_cache(est, field, cdir, NRi, 37, &worst))
+                fprintf(stderr, "barbie: plane-ring cache check: %s\n", bb_last_error());
+            else if (worst > 1e-9)
+                fprintf(stderr, "barbie: warning: the plane-ring cache in %s does not belong to "
+                                "this field (largest relative difference %.3g); it is not used\n",
+                        cdir, worst);
+        }
+    }
     double t_est = now();
     rc = bb_estimate(est, field, bins, bins + nb, bins + 2 * nb);
     if (rc) {

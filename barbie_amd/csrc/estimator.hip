@@ -1216,6 +1216,129 @@ int compute_num(bb_estimator *E)
 // time, and written as they arrive.  A text file cache_config_<NN>_R<NRi>.txt
 // beside them records the geometry and its 64-bit FNV-1a hash, the key the
 // reference's file names lack.
+// 64-bit FNV-1a key of a geometry: NN, Nmax, Nr, n_mtbin and the ring edges.
+// The reference's cache file names hold only NN, p, r and NRi, so caches of
+// two geometries with the same NN cannot be told apart by name.
+unsigned long long geometry_hash(const PlanData &d)
+{
+    unsigned long long h = 1469598103934665603ULL;
+    auto mix = [&](const void *p, size_t n) {
+        const unsigned char *b = (const unsigned char *)p;
+        for (size_t k = 0; k < n; ++k) h = (h ^ b[k]) * 1099511628211ULL;
+    };
+    const int ints[4] = {d.g.NN, d.g.Nmax, d.g.Nr, d.g.n_mtbin};
+    mix(ints, sizeof ints);
+    mix(d.low.data(), d.low.size() * sizeof(double));
+    mix(d.high.data(), d.high.size() * sizeof(double));
+    return h;
+}
+
+std::string cache_name(const PlanData &d, const std::string &dir, int p, int r, int NRi)
+{
+    return dir + "/dataPlaneRing_" + std::to_string(d.g.NN) + "_" + std::to_string(p) + "_" +
+           std::to_string(r) + "_R" + std::to_string(NRi);
+}
+
+// The plane-ring fields D_{p,r} on the NN grid (FP64, the estimator's gather
+// and transforms, in batches of about 256 MiB) of the fields (p, r) =
+// (f / Nr, f % Nr) with f a multiple of `stride`; fn(p, r, data) receives
+// each in wrapid_fast's layout (funcs.c:554-563: 2 NN^2 doubles, row-major
+// interleaved complex) and returns 0 or an error.
+template <class Fn>
+int visit_nn_fields(bb_estimator *E, const double *field_r2c, int stride, Fn fn)
+{
+    const PlanData &d = E->plan->d;
+    const int NN = d.g.NN, Nmax = d.g.Nmax, Nr = d.g.Nr;
+    const std::vector<int> K = ring_kx(d);
+    Level V;
+    V.NG = NN;
+    V.npts = (long long)NN * NN;
+    V.stride = (V.npts + 31) / 32 * 32;
+    LevelDesc D;
+    if (int rc = level_desc(V, 0, D)) return rc;
+    std::vector<int> sel;
+    for (int f = 0; f < Nmax * Nr; f += std::max(1, stride)) sel.push_back(f);
+    const int nsel = (int)sel.size();
+    const int nb = std::max(1, std::min(nsel, (int)((256LL << 20) / (V.stride * 16))));
+    const std::vector<double2> tw = twiddles(NN);
+    const size_t fsz = (size_t)NN * NN * (NN / 2 + 1);
+    std::vector<void *> tmp;
+    double2 *d_field, *d_planes, *d_cache, *d_tw;
+    FieldTask *d_tasks;
+    LevelDesc *d_ld;
+    int2 *d_blk;
+    const size_t maxblk = (size_t)nb * (NN / std::max(1, std::min(D.rpb, D.cpb)) + 2) * 2;
+    int rc = dev_alloc(&d_field, fsz, tmp);
+    if (!rc) rc = dev_alloc(&d_planes, (size_t)Nmax * NN * NN, tmp);
+    if (!rc) rc = dev_alloc(&d_cache, (size_t)nb * V.stride, tmp);
+    if (!rc) rc = dev_alloc(&d_tw, (size_t)NN, tmp);
+    if (!rc) rc = dev_alloc(&d_tasks, (size_t)nb, tmp);
+    if (!rc) rc = dev_alloc(&d_ld, 1, tmp);
+    if (!rc) rc = dev_alloc(&d_blk, maxblk, tmp);
+    hipStream_t s = E->stream;
+    if (!rc) rc = upload(d_field, field_r2c, fsz * sizeof(double2), s);
+    if (!rc) rc = upload(d_tw, tw.data(), sizeof(double2) * NN, s);
+    if (!rc) rc = upload(d_ld, &D, sizeof D, s);
+    if (!rc) {
+        hipError_t e = launch_gather(d_field, d_planes, NN, NN, NN / 2, Nmax, 1, s);
+        if (e != hipSuccess) rc = set_error(BB_E_DEVICE, hipGetErrorString(e));
+    }
+    std::vector<double2> host((size_t)nb * V.stride);
+    std::vector<double> row((size_t)2 * NN * NN);
+    for (int s0 = 0; s0 < nsel && !rc; s0 += nb) {
+        const int n = std::min(nb, nsel - s0);
+        std::vector<FieldTask> tasks;
+        for (int k = 0; k < n; ++k) {
+            const int p = sel[s0 + k] / Nr, r = sel[s0 + k] % Nr;
+            const int band = d.nhi[r] > 0 ? std::min(NN / 2, K[r]) : -1;
+            tasks.push_back(FieldTask{p, r, k, band, d.nlo[r], d.nhi[r], 0, 0, 3});
+        }
+        std::vector<int2> brow, bcol;
+        fft_blocks(tasks.data(), n, &D, brow, bcol);
+        if (brow.size() + bcol.size() > maxblk) {
+            rc = set_error(BB_E_INVALID, "internal: field block list");
+            break;
+        }
+        const size_t nrow = brow.size();
+        brow.insert(brow.end(), bcol.begin(), bcol.end());
+        rc = upload(d_tasks, tasks.data(), sizeof(FieldTask) * n, s);
+        if (!rc && !brow.empty()) rc = upload(d_blk, brow.data(), sizeof(int2) * brow.size(), s);
+        if (!rc) {
+            FftBlocks fb;
+            fb.row = d_blk;
+            fb.nrow = (int)nrow;
+            fb.col = d_blk + nrow;
+            fb.ncol = (int)bcol.size();
+            const BatchStrides one;
+            hipError_t e = launch_fft_f64(d_planes, NN, d_cache, d_tasks, fb, d_ld, &D, 1, d_tw,
+                                          nullptr, one, 1, s);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(host.data(), d_cache, sizeof(double2) * n * V.stride,
+                                   hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) rc = set_error(BB_E_DEVICE, hipGetErrorString(e));
+        }
+        for (int k = 0; k < n && !rc; ++k) {
+            const int p = sel[s0 + k] / Nr, r = sel[s0 + k] % Nr;
+            const double2 *src = host.data() + (size_t)k * V.stride;   // column-major
+            for (int i = 0; i < NN; ++i)
+                for (int j = 0; j < NN; ++j) {
+                    row[2 * ((size_t)i * NN + j)] = src[(size_t)j * NN + i].x;
+                    row[2 * ((size_t)i * NN + j) + 1] = src[(size_t)j * NN + i].y;
+                }
+            rc = fn(p, r, row.data());
+        }
+    }
+    (void)hipStreamSynchronize(s);
+    for (void *q : tmp) (void)hipFree(q);
+    return rc;
+}
+
+// The reference's on-disk caches (funcs.c:328-452, 537-566) from a field:
+// closed_rings_<NN> (rcount x {r1, r2, r3, Num} doubles, calCR order),
+// rcountFor_<NN> (the count as text), and for every plane p < Nmax and ring
+// r < Nr dataPlaneRing_<NN>_<p>_<r>_R<NRi>.  A text file
+// cache_config_<NN>_R<NRi>.txt beside them records the geometry and its key.
 int export_cache(bb_estimator *E, const double *field_r2c, const std::string &dir, int NRi)
 {
     const PlanData &d = E->plan->d;
@@ -1242,114 +1365,99 @@ int export_cache(bb_estimator *E, const double *field_r2c, const std::string &di
     }
     // Geometry key.
     {
-        unsigned long long h = 1469598103934665603ULL;
-        auto mix = [&](const void *p, size_t n) {
-            const unsigned char *b = (const unsigned char *)p;
-            for (size_t k = 0; k < n; ++k) h = (h ^ b[k]) * 1099511628211ULL;
-        };
-        const int ints[4] = {NN, Nmax, Nr, d.g.n_mtbin};
-        mix(ints, sizeof ints);
-        mix(d.low.data(), d.low.size() * sizeof(double));
-        mix(d.high.data(), d.high.size() * sizeof(double));
         const std::string cf = path("cache_config_" + nn + "_R" + std::to_string(NRi) + ".txt");
         FILE *f = std::fopen(cf.c_str(), "w");
         if (!f) return set_error(BB_E_IO, "cannot create " + cf);
         std::fprintf(f, "NN %d\nNmax %d\nNr %d\nn_mtbin %d\nNRi %d\nkey %016llx\n", NN, Nmax, Nr,
-                     d.g.n_mtbin, NRi, h);
+                     d.g.n_mtbin, NRi, geometry_hash(d));
         for (int r = 0; r < Nr; ++r)
             std::fprintf(f, "ring %d %.17g %.17g\n", r, d.low[r], d.high[r]);
         if (std::fclose(f) != 0) return set_error(BB_E_IO, "write failed: " + cf);
     }
     // Plane-ring fields on the NN grid, FP64.
-    const std::vector<int> K = ring_kx(d);
-    Level V;
-    V.NG = NN;
-    V.npts = (long long)NN * NN;
-    V.stride = (V.npts + 31) / 32 * 32;
-    LevelDesc D;
-    if (int rc = level_desc(V, 0, D)) return rc;
-    const int nfield = Nmax * Nr;
-    const int nb = std::max(1, std::min(nfield, (int)((256LL << 20) / (V.stride * 16))));
-    const std::vector<double2> tw = twiddles(NN);
-    const size_t fsz = (size_t)NN * NN * (NN / 2 + 1);
-    std::vector<void *> tmp;
-    double2 *d_field, *d_planes, *d_cache, *d_tw;
-    FieldTask *d_tasks;
-    LevelDesc *d_ld;
-    int2 *d_blk;
-    const size_t maxblk = (size_t)nb * (NN / std::max(1, std::min(D.rpb, D.cpb)) + 2) * 2;
-    int rc = dev_alloc(&d_field, fsz, tmp);
-    if (!rc) rc = dev_alloc(&d_planes, (size_t)Nmax * NN * NN, tmp);
-    if (!rc) rc = dev_alloc(&d_cache, (size_t)nb * V.stride, tmp);
-    if (!rc) rc = dev_alloc(&d_tw, (size_t)NN, tmp);
-    if (!rc) rc = dev_alloc(&d_tasks, (size_t)nb, tmp);
-    if (!rc) rc = dev_alloc(&d_ld, 1, tmp);
-    if (!rc) rc = dev_alloc(&d_blk, maxblk, tmp);
-    hipStream_t s = E->stream;
-    if (!rc) rc = upload(d_field, field_r2c, fsz * sizeof(double2), s);
-    if (!rc) rc = upload(d_tw, tw.data(), sizeof(double2) * NN, s);
-    if (!rc) rc = upload(d_ld, &D, sizeof D, s);
-    if (!rc) {
-        hipError_t e = launch_gather(d_field, d_planes, NN, NN, NN / 2, Nmax, 1, s);
-        if (e != hipSuccess) rc = set_error(BB_E_DEVICE, hipGetErrorString(e));
+    return visit_nn_fields(E, field_r2c, 1, [&](int p, int r, const double *data) {
+        const std::string fn = cache_name(d, dir, p, r, NRi);
+        FILE *f = std::fopen(fn.c_str(), "wb");
+        if (!f) return set_error(BB_E_IO, "cannot create " + fn);
+        const size_t n = (size_t)2 * NN * NN;
+        const bool ok = std::fwrite(data, sizeof(double), n, f) == n;
+        if ((std::fclose(f) != 0) || !ok) return set_error(BB_E_IO, "write failed: " + fn);
+        return (int)BB_OK;
+    });
+}
+
+// The key recorded in cache_config_<NN>_R<NRi>.txt, if the file exists:
+// 1 and *key set, 0 when there is no such file, an error when it is
+// unreadable.
+int read_cache_key(const std::string &file, unsigned long long *key)
+{
+    FILE *f = std::fopen(file.c_str(), "r");
+    if (!f) return 0;
+    char line[256];
+    int found = 0;
+    while (std::fgets(line, sizeof line, f))
+        if (std::sscanf(line, "key %llx", key) == 1) found = 1;
+    std::fclose(f);
+    if (!found) return set_error(BB_E_IO, file + " holds no key");
+    return 1;
+}
+
+// read_rings (funcs.c:429-452) with the checks the reference lacks.
+int import_rings(bb_estimator *E, const std::string &dir, int NRi, double rel_tol, int use_num,
+                 long long *nmismatch)
+{
+    const PlanData &d = E->plan->d;
+    const std::string nn = std::to_string(d.g.NN);
+    unsigned long long key = 0;
+    const std::string cf = dir + "/cache_config_" + nn + "_R" + std::to_string(NRi) + ".txt";
+    const int have = read_cache_key(cf, &key);
+    if (have < 0) return have;
+    if (have == 1 && key != geometry_hash(d))
+        return set_error(BB_E_INVALID, cf + ": the cache was written for another geometry");
+    int rcount = -1;
+    {
+        const std::string fn = dir + "/rcountFor_" + nn;
+        FILE *f = std::fopen(fn.c_str(), "r");
+        if (!f) return set_error(BB_E_IO, "cannot open " + fn);
+        const int ok = std::fscanf(f, "%d", &rcount);
+        std::fclose(f);
+        if (ok != 1 || rcount < 0) return set_error(BB_E_IO, fn + ": no ring-triple count");
     }
-    std::vector<double2> host((size_t)nb * V.stride);
-    std::vector<double> row((size_t)2 * NN * NN);
-    for (int f0 = 0; f0 < nfield && !rc; f0 += nb) {
-        const int n = std::min(nb, nfield - f0);
-        std::vector<FieldTask> tasks;
-        for (int k = 0; k < n; ++k) {
-            const int p = (f0 + k) / Nr, r = (f0 + k) % Nr;
-            const int band = d.nhi[r] > 0 ? std::min(NN / 2, K[r]) : -1;
-            tasks.push_back(FieldTask{p, r, k, band, d.nlo[r], d.nhi[r], 0, 0, 3});
-        }
-        std::vector<int2> brow, bcol;
-        fft_blocks(tasks.data(), n, &D, brow, bcol);
-        if (brow.size() + bcol.size() > maxblk) {
-            rc = set_error(BB_E_INVALID, "internal: export block list");
-            break;
-        }
-        const size_t nrow = brow.size();
-        brow.insert(brow.end(), bcol.begin(), bcol.end());
-        rc = upload(d_tasks, tasks.data(), sizeof(FieldTask) * n, s);
-        if (!rc && !brow.empty()) rc = upload(d_blk, brow.data(), sizeof(int2) * brow.size(), s);
-        if (!rc) {
-            FftBlocks fb;
-            fb.row = d_blk;
-            fb.nrow = (int)nrow;
-            fb.col = d_blk + nrow;
-            fb.ncol = (int)bcol.size();
-            const BatchStrides one;
-            hipError_t e = launch_fft_f64(d_planes, NN, d_cache, d_tasks, fb, d_ld, &D, 1, d_tw,
-                                          nullptr, one, 1, s);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(host.data(), d_cache, sizeof(double2) * n * V.stride,
-                                   hipMemcpyDeviceToHost, s);
-            if (e == hipSuccess) e = hipStreamSynchronize(s);
-            if (e != hipSuccess) rc = set_error(BB_E_DEVICE, hipGetErrorString(e));
-        }
-        for (int k = 0; k < n && !rc; ++k) {
-            const int p = (f0 + k) / Nr, r = (f0 + k) % Nr;
-            const double2 *src = host.data() + (size_t)k * V.stride;   // column-major
-            for (int i = 0; i < NN; ++i)
-                for (int j = 0; j < NN; ++j) {
-                    row[2 * ((size_t)i * NN + j)] = src[(size_t)j * NN + i].x;
-                    row[2 * ((size_t)i * NN + j) + 1] = src[(size_t)j * NN + i].y;
-                }
-            const std::string fn = path("dataPlaneRing_" + nn + "_" + std::to_string(p) + "_" +
-                                        std::to_string(r) + "_R" + std::to_string(NRi));
-            FILE *f = std::fopen(fn.c_str(), "wb");
-            if (!f) {
-                rc = set_error(BB_E_IO, "cannot create " + fn);
-                break;
-            }
-            const bool ok = std::fwrite(row.data(), sizeof(double), row.size(), f) == row.size();
-            if ((std::fclose(f) != 0) || !ok) rc = set_error(BB_E_IO, "write failed: " + fn);
-        }
+    const int want = (int)(d.tri.size() / 3);
+    if (rcount != want)
+        return set_error(BB_E_INVALID, "rcountFor_" + nn + " holds " + std::to_string(rcount) +
+                                           " ring triples; this geometry has " + std::to_string(want));
+    std::vector<double> data((size_t)4 * rcount);
+    {
+        const std::string fn = dir + "/closed_rings_" + nn;
+        FILE *f = std::fopen(fn.c_str(), "rb");
+        if (!f) return set_error(BB_E_IO, "cannot open " + fn);
+        const size_t got = std::fread(data.data(), sizeof(double), data.size(), f);
+        std::fclose(f);
+        if (got != data.size()) return set_error(BB_E_IO, fn + " is short");
     }
-    (void)hipStreamSynchronize(s);
-    for (void *q : tmp) (void)hipFree(q);
-    return rc;
+    double scale = 0.0;
+    for (int rp = 0; rp < rcount; ++rp) scale = std::max(scale, std::fabs(E->num_host[rp]));
+    long long bad = 0;
+    for (int rp = 0; rp < rcount; ++rp) {
+        for (int q = 0; q < 3; ++q)
+            if (data[4 * rp + q] != (double)d.tri[3 * rp + q])
+                return set_error(BB_E_INVALID, "closed_rings_" + nn + ": ring triple " +
+                                                   std::to_string(rp) + " differs from this geometry's");
+        if (!(std::fabs(data[4 * rp + 3] - E->num_host[rp]) <= rel_tol * scale)) ++bad;
+    }
+    if (nmismatch) *nmismatch = bad;
+    if (bad)
+        return set_error(BB_E_INVALID, "closed_rings_" + nn + ": " + std::to_string(bad) +
+                                           " triangle counts differ from this geometry's");
+    if (use_num) {
+        for (int rp = 0; rp < rcount; ++rp) E->num_host[rp] = data[4 * rp + 3];
+        if (int rc = upload(E->d_num, E->num_host.data(), sizeof(double) * rcount, E->stream))
+            return rc;
+        if (hipStreamSynchronize(E->stream) != hipSuccess)
+            return set_error(BB_E_DEVICE, "triangle-count upload failed");
+    }
+    return BB_OK;
 }
 
 // Device bytes held by an estimator of layout L (steady state).
@@ -1682,6 +1790,11 @@ int check_options(const bb_options &o)
 }
 
 }  // namespace
+
+namespace bb {
+bool fft_desc(int N, FftDesc &fd) { return make_fft_desc(N, fd); }
+std::vector<double2> fft_twiddles(int N) { return twiddles(N); }
+}  // namespace bb
 
 extern "C" {
 
@@ -2021,6 +2134,46 @@ int bb_export_cache(bb_estimator *e, const double *field_r2c, const char *dir, i
     if (!e || !field_r2c || !dir) return set_error(BB_E_INVALID, "null argument");
     if (int rc = estimate_finish(e)) return rc;
     return export_cache(e, field_r2c, dir, NRi);
+}
+
+int bb_import_rings(bb_estimator *e, const char *dir, int NRi, double rel_tol, int use_num,
+                    long long *nmismatch)
+{
+    if (!e || !dir) return set_error(BB_E_INVALID, "null argument");
+    if (!(rel_tol >= 0)) return set_error(BB_E_INVALID, "rel_tol must be >= 0");
+    if (nmismatch) *nmismatch = 0;
+    if (int rc = estimate_finish(e)) return rc;
+    return import_rings(e, dir, NRi, rel_tol, use_num, nmismatch);
+}
+
+int bb_check_plane_cache(bb_estimator *e, const double *field_r2c, const char *dir, int NRi,
+                         int stride, double *max_rel)
+{
+    if (!e || !field_r2c || !dir || !max_rel) return set_error(BB_E_INVALID, "null argument");
+    if (stride < 1) return set_error(BB_E_INVALID, "stride must be >= 1");
+    if (int rc = estimate_finish(e)) return rc;
+    const PlanData &d = e->plan->d;
+    const size_t n = (size_t)2 * d.g.NN * d.g.NN;
+    std::vector<double> disk(n);
+    double worst = 0.0;
+    const int rc = visit_nn_fields(e, field_r2c, stride, [&](int p, int r, const double *data) {
+        const std::string fn = cache_name(d, dir, p, r, NRi);
+        FILE *f = std::fopen(fn.c_str(), "rb");
+        if (!f) return set_error(BB_E_IO, "cannot open " + fn);
+        const size_t got = std::fread(disk.data(), sizeof(double), n, f);
+        std::fclose(f);
+        if (got != n) return set_error(BB_E_IO, fn + " is short");
+        double scale = 0.0, diff = 0.0;
+        for (size_t k = 0; k < n; ++k) {
+            scale = std::max(scale, std::fabs(data[k]));
+            diff = std::max(diff, std::fabs(disk[k] - data[k]));
+        }
+        const double rel = scale > 0 ? diff / scale : (diff > 0 ? INFINITY : 0.0);
+        worst = std::max(worst, rel);
+        return (int)BB_OK;
+    });
+    *max_rel = worst;
+    return rc;
 }
 
 static void worklist_of(const Layout &L, long long *o)

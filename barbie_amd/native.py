@@ -34,7 +34,7 @@ EXPORTED = [
     "bb_estimate_batch", "bb_estimate_batch_device", "bb_estimate_batch_async",
     "bb_estimator_query", "bb_estimator_wait", "bb_estimator_stream", "bb_estimator_set_stream",
     "bb_estimator_contraction_sums", "bb_estimator_triangle_counts", "bb_estimator_field",
-    "bb_estimator_timings", "bb_estimator_worklist", "bb_write_outputs", "bb_write_outputs_checked", "bb_export_cache",
+    "bb_estimator_timings", "bb_estimator_worklist", "bb_write_outputs", "bb_write_outputs_checked", "bb_export_cache", "bb_import_rings", "bb_check_plane_cache",
     "bb_power_multipoles", "bb_power_multipoles_noalloc", "bb_generate_field",
     "bb_server_create", "bb_server_destroy", "bb_submit", "bb_poll", "bb_wait", "bb_server_pause",
     "bb_server_prepare",
@@ -160,6 +160,10 @@ def lib() -> ctypes.CDLL:
         L.bb_estimator_timings.argtypes = [ctypes.c_void_p, _dp]
         L.bb_estimator_worklist.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)]
         L.bb_export_cache.argtypes = [ctypes.c_void_p, _dp, ctypes.c_char_p, ctypes.c_int]
+        L.bb_import_rings.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_double,
+                                      ctypes.c_int, ctypes.POINTER(ctypes.c_longlong)]
+        L.bb_check_plane_cache.argtypes = [ctypes.c_void_p, _dp, ctypes.c_char_p, ctypes.c_int,
+                                           ctypes.c_int, _dp]
         L.bb_power_multipoles.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_double, _dp]
         L.bb_power_multipoles_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
@@ -402,6 +406,25 @@ class Estimator:
         [NN][NN][NN+2] doubles) in `directory`."""
         f = np.ascontiguousarray(field, dtype=np.float64)
         _check(lib().bb_export_cache(self.handle, _p(f), directory.encode(), int(NRi)))
+
+    def import_rings(self, directory: str, NRi: int, rel_tol: float = 1e-12,
+                     use_num: bool = True) -> int:
+        """bb_import_rings: read closed_rings / rcountFor back (checked);
+        returns the number of mismatching triangle counts (0 on success)."""
+        n = ctypes.c_longlong()
+        _check(lib().bb_import_rings(self.handle, directory.encode(), int(NRi), float(rel_tol),
+                                     int(use_num), ctypes.byref(n)))
+        return n.value
+
+    def check_plane_cache(self, field: np.ndarray, directory: str, NRi: int,
+                          stride: int = 1) -> float:
+        """bb_check_plane_cache: largest relative difference between the
+        dataPlaneRing files and the field's own plane-ring fields."""
+        f = np.ascontiguousarray(field, dtype=np.float64)
+        r = ctypes.c_double()
+        _check(lib().bb_check_plane_cache(self.handle, _p(f), directory.encode(), int(NRi),
+                                          int(stride), ctypes.byref(r)))
+        return r.value
 
     def contraction_sums(self) -> np.ndarray:
         S = np.zeros(self.plan.stats["ncontr"])

@@ -337,6 +337,28 @@ int bb_write_outputs_checked(const bb_plan *p, const double *bin_bs, const doubl
  * built on the device (FP64, the estimator's transforms on the NN grid) in
  * batches of about 256 MiB. */
 int bb_export_cache(bb_estimator *e, const double *field_r2c, const char *dir, int NRi);
+/* Reads an existing ring-triple cache back, as read_rings does
+ * (funcs.c:429-452: rcountFor_<NN> and closed_rings_<NN> in dir, written by
+ * calCR, funcs.c:402-415, or by bb_export_cache), with the checks the
+ * reference lacks (docs/REVIEW.md:9): when cache_config_<NN>_R<NRi>.txt is
+ * present its key must be this geometry's; the count and the ring triples
+ * must be this plan's, in calCR order; every Num must equal the estimator's
+ * within rel_tol of the largest (*nmismatch, may be NULL: how many do not).
+ * Any failure is BB_E_INVALID (BB_E_IO for missing or short files) and
+ * changes nothing.  use_num != 0: the estimator then bins with the file's
+ * Num values, as the reference does after read_rings. */
+int bb_import_rings(bb_estimator *e, const char *dir, int NRi, double rel_tol, int use_num,
+                    long long *nmismatch);
+/* Checks an existing plane-ring cache (dataPlaneRing_<NN>_<p>_<r>_R<NRi> in
+ * dir, wrapid_fast's files, funcs.c:537-566) against the fields of
+ * field_r2c (host [NN][NN][NN+2] doubles): every stride-th (p, r) in (p, r)
+ * order is built on the device (FP64, NN grid) and compared; *max_rel gets
+ * the largest max |file - field| / max |field|.  The reference reuses such
+ * files whenever its sentinel file exists (barbie.c:193-196), also when they
+ * belong to another field or geometry; a large *max_rel says so.
+ * BB_E_IO when a file is missing or short. */
+int bb_check_plane_cache(bb_estimator *e, const double *field_r2c, const char *dir, int NRi,
+                         int stride, double *max_rel);
 
 /* Power-spectrum multipoles of a field: the program calpow_mom
  * (calpow_mom.c:22-117) and its calpow_mom function (funcs.c:860-1097) on the
