@@ -13,6 +13,7 @@
 #include <map>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "device.hpp"
@@ -984,6 +985,9 @@ struct bb_estimator {
     bool pending = false;             // an enqueued batch not yet waited for
     double2 *d_field = nullptr;       // staging for bb_estimate (host input)
     int field_cap = 0;                // fields the staging holds
+    double2 *h_stage = nullptr;       // pinned host staging of the modes the estimate reads
+    double2 *d_stage = nullptr;       // its device copy
+    size_t stage_cap = 0;             // complex values both hold
     double2 *d_planes = nullptr;      // NGtop x NGtop per plane
     void *d_cache = nullptr;
     void *d_split = nullptr;          // BF16x3: bf16 hi/lo operands of every field
@@ -1016,6 +1020,7 @@ struct bb_estimator {
     ~bb_estimator()
     {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (h_stage) (void)hipHostFree(h_stage);
         if (gexec) (void)hipGraphExecDestroy(gexec);
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -1469,6 +1474,11 @@ size_t layout_bytes(const PlanData &d, const Layout &L, size_t F)
     const size_t NN = d.g.NN, Nr = d.g.Nr, Nmax = d.g.Nmax;
     const size_t top2 = (size_t)L.NGtop * L.NGtop;
     size_t b = NN * NN * (NN / 2 + 1) * 16;                  // host-field staging
+    {
+        const size_t km = std::min(NN / 2, (size_t)std::max(0, L.kmax));
+        const size_t nr = std::min(NN, 2 * km + 1);
+        b += nr * nr * Nmax * 16;                            // its compact upload
+    }
     size_t per = Nmax * top2 * 16;                           // gathered planes
     per += (size_t)L.cache_pts * (L.prec == CPrec::F64 ? 16 : 8);
     if (L.prec == CPrec::BF16X3) per += (size_t)L.cache_pts * 8;
@@ -1729,25 +1739,103 @@ int estimate_device(bb_estimator *E, int nf, const double *d_fields, double *d_b
     return estimate_finish(E);
 }
 
-// Host fields: staged through device memory, outputs copied back.
+// Scatters the compact modes (stage: per field [nr][nr][Nmax], rows and
+// columns in the order of `idx`) into their places of the r2c buffers.
+__global__ __launch_bounds__(256) void k_expand_modes(const double2 *__restrict__ stage,
+                                                      double2 *__restrict__ fields,
+                                                      const int *__restrict__ idx, int nr,
+                                                      int Nmax, int NN, long long nper)
+{
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= nper) return;
+    const int f = blockIdx.y;
+    const int p = (int)(t % Nmax);
+    const long long ij = t / Nmax;
+    const int cj = (int)(ij % nr), ci = (int)(ij / nr);
+    const long long H = NN / 2 + 1;
+    fields[(long long)f * NN * NN * H + ((long long)idx[ci] * NN + idx[cj]) * H + p] =
+        stage[(long long)f * nper + t];
+}
+
+// Host fields: only the modes the estimate reads cross PCIe -- kz < Nmax and
+// |kx|, |ky| <= kmax (the largest |k_x| of a ring mode the estimate uses;
+// the plane gather zeroes or masks every other mode).  Host threads gather
+// them into pinned memory, one copy moves them, a kernel puts them in their
+// places of a device r2c buffer whose other entries stay zero.  Outputs are
+// copied back.
 int estimate_host(bb_estimator *E, int nf, const double *fields, double *bs, double *norm,
                   double *biana)
 {
     const PlanData &d = E->plan->d;
-    const size_t NN = d.g.NN;
-    const size_t fsz = NN * NN * (NN / 2 + 1);   // complex values per field
+    const size_t NN = d.g.NN, H = NN / 2 + 1, Nmax = d.g.Nmax;
+    const size_t fsz = NN * H;                   // complex values per field ...
+    const size_t fszf = fsz * NN;                // ... all of them
     if (nf < 1 || nf > E->batch)
         return set_error(BB_E_RANGE, "batch of " + std::to_string(nf) + " fields; the estimator "
                                          "holds " + std::to_string(E->batch));
     if (E->field_cap < nf) {
         double2 *p = nullptr;
-        int rc = dev_alloc(&p, fsz * (size_t)nf, E->owned);
+        int rc = dev_alloc(&p, fszf * (size_t)nf, E->owned);
         if (rc) return rc;
+        HIPCHK(hipMemsetAsync(p, 0, fszf * 16 * (size_t)nf, E->stream));
         E->d_field = p;   // an older, smaller staging stays owned until destroy
         E->field_cap = nf;
     }
-    HIPCHK(hipMemcpyAsync(E->d_field, fields, fsz * 16 * (size_t)nf, hipMemcpyHostToDevice,
-                          E->stream));
+    // rows (and columns) ia with |ia| <= kmax, in buffer order
+    const int km = std::min((int)NN / 2, std::max(0, E->L->kmax));
+    std::vector<int> idx;
+    for (int i = 0; i < (int)NN; ++i) {
+        const int ia = i <= (int)NN / 2 ? i : i - (int)NN;
+        if (ia <= km && -ia <= km) idx.push_back(i);
+    }
+    const int nr = (int)idx.size();
+    const size_t nper = (size_t)nr * nr * Nmax;
+    if (E->stage_cap < nper * (size_t)nf) {
+        double2 *h = nullptr, *dv = nullptr;
+        if (hipHostMalloc(&h, nper * nf * sizeof(double2), hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            return set_error(BB_E_NOMEM, "pinned staging for host fields");
+        }
+        if (int rc = dev_alloc(&dv, nper * (size_t)nf + (size_t)nr / 2 + 1, E->owned)) {
+            (void)hipHostFree(h);
+            return rc;
+        }
+        HIPCHK(hipStreamSynchronize(E->stream));   // the old staging may still be read
+        if (E->h_stage) (void)hipHostFree(E->h_stage);
+        E->h_stage = h;
+        E->d_stage = dv;
+        E->stage_cap = nper * (size_t)nf;
+    }
+    HIPCHK(hipStreamSynchronize(E->stream));   // earlier uploads from the staging are done
+    {
+        const double2 *src = reinterpret_cast<const double2 *>(fields);
+        double2 *dst = E->h_stage;
+        const int rows = nf * nr;
+        const int nth = std::max(1, std::min(8, rows / 16));
+        auto work = [&](int t) {
+            for (int q = t; q < rows; q += nth) {
+                const int f = q / nr, ci = q % nr;
+                for (int cj = 0; cj < nr; ++cj)
+                    std::memcpy(dst + (size_t)f * nper + ((size_t)ci * nr + cj) * Nmax,
+                                src + (size_t)f * fszf + ((size_t)idx[ci] * NN + idx[cj]) * H,
+                                Nmax * sizeof(double2));
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nth; ++t) pool.emplace_back(work, t);
+        work(0);
+        for (auto &th : pool) th.join();
+    }
+    int *d_idx = reinterpret_cast<int *>(E->d_stage + nper * (size_t)nf);
+    HIPCHK(hipMemcpyAsync(E->d_stage, E->h_stage, nper * nf * sizeof(double2),
+                          hipMemcpyHostToDevice, E->stream));
+    HIPCHK(hipMemcpyAsync(d_idx, idx.data(), sizeof(int) * nr, hipMemcpyHostToDevice, E->stream));
+    {
+        dim3 grid((unsigned)((nper + 255) / 256), (unsigned)nf);
+        hipLaunchKernelGGL(k_expand_modes, grid, dim3(256), 0, E->stream, E->d_stage, E->d_field,
+                           d_idx, nr, (int)Nmax, (int)NN, (long long)nper);
+        HIPCHK(hipGetLastError());
+    }
     const size_t nb = (size_t)d.nbins * BB_NPOLES;
     double *B = E->d_bins;   // [bs | norm | biana], each nf * nb
     const size_t span = nb * (size_t)E->batch;

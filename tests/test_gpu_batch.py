@@ -186,3 +186,35 @@ def test_batch_of_1024_fields_on_the_nn_grid():
         for a in range(3):
             assert np.array_equal(got[a, k], ref[k % 3][a].reshape(-1)), (k, a)
     est.close()
+
+
+@pytest.mark.parametrize("full_grid", [False, True])
+def test_host_field_upload_reads_only_the_modes_it_needs(full_grid):
+    """bb_estimate with a host field uploads only kz < Nmax, |kx|, |ky| <=
+    kmax: the modes outside that box may hold anything (here NaN) and the
+    bins are the device path's bits for the clean field; batches too."""
+    torch = pytest.importorskip("torch")
+    g = config1()
+    plan = B.Plan(g)
+    est = B.Estimator(plan, "fp64", full_grid=full_grid, max_batch=2)
+    f = B.make_field(g.NN, NRi=6)
+    d = torch.from_numpy(f).to("cuda")
+    nb = g.nbins * 16
+    out = torch.zeros(3, nb, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    est.estimate_device(d.data_ptr(), out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr())
+    ref = out.cpu().numpy()
+    lo, hi, av = plan.ring_edges()
+    km = int(np.floor(hi.max()))
+    dirty = f.copy().view(np.complex128)
+    ia = np.fft.fftfreq(g.NN, 1.0 / g.NN).astype(int)
+    outside = (np.abs(ia)[:, None, None] > km) | (np.abs(ia)[None, :, None] > km) | \
+              (np.arange(g.NN // 2 + 1)[None, None, :] >= g.Nmax)
+    dirty[np.broadcast_to(outside, dirty.shape)] = np.nan
+    dirty = dirty.view(np.float64)
+    got = est.estimate(dirty)
+    for k in range(3):
+        assert np.array_equal(got[k].ravel(), ref[k]), k
+    two = est.estimate_batch(np.stack([dirty, f]))
+    for k in range(3):
+        assert np.array_equal(two[k][0].ravel(), ref[k]) and np.array_equal(two[k][1].ravel(), ref[k])
