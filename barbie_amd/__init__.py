@@ -6,10 +6,10 @@ that boundary and the seeded input generator.
 """
 from .native import (Geometry, Plan, Estimator, Server, BarbieError, normalise, lib, LIB_PATH, BIN_PATH,
                      NPOLES, PRECISIONS, RINGS_REFERENCE, RINGS_UNIFORM, RINGS_EXPLICIT,
-                     power_multipoles, generate_field)
+                     power_multipoles, generate_field, estimate_mixed)
 from .field import make_field, make_field_device, write_field, read_field, seed_for
 
 __all__ = ["Geometry", "Plan", "Estimator", "Server", "BarbieError", "normalise", "lib", "LIB_PATH",
            "BIN_PATH", "NPOLES", "PRECISIONS", "RINGS_REFERENCE", "RINGS_UNIFORM",
            "RINGS_EXPLICIT", "make_field", "make_field_device", "write_field", "read_field", "seed_for",
-           "power_multipoles", "generate_field"]
+           "power_multipoles", "generate_field", "estimate_mixed"]

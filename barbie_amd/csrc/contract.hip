@@ -39,6 +39,8 @@
 // while the waves work on the first (DmaStager, dma_off).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device.hpp"
 
 namespace bb {
@@ -438,23 +440,18 @@ __device__ __forceinline__ void pair_operands(const unsigned char *rows, int oa1
 
 // A staged ring of 32 points: raw rows hold float2 per point; split rows
 // hold, per group of 8 points, [Re hi x8][Re lo x8][Im hi x8][Im lo x8] bf16.
-// NW waves, TWW tile windows per wave, MINB workgroups per CU.
-template <int NW, int TWW, int MINB>
-__global__ __launch_bounds__(64 * NW, MINB) void k_contract_bf16x3(
-    const float2 *__restrict__ cache, const __bf16 *__restrict__ splitc,
-    const WorkItem *__restrict__ items, const WorkUnit *__restrict__ units,
-    const TWin *__restrict__ tws, double *__restrict__ out, BatchStrides bs)
+// One work unit (the workgroup's share of an item) of the BF16x3 kernels:
+// NW waves, TWW tile windows per wave.
+template <int NW, int TWW>
+__device__ __forceinline__ void bf16x3_unit(const float2 *__restrict__ cache,
+                                            const __bf16 *__restrict__ splitc,
+                                            const WorkItem &it, const WorkUnit U,
+                                            const TWin *__restrict__ tws,
+                                            double *__restrict__ out, unsigned char *lds)
 {
-    cache += blockIdx.y * bs.cache;   // field of the batch
-    splitc += blockIdx.y * bs.cache * 4;
-    out += blockIdx.y * bs.part;
     constexpr int BUF = dma_buf_stride(NW, TWW);
-    extern __shared__ __align__(16) unsigned char lds[];
     int *row_slot = reinterpret_cast<int *>(lds + 2 * BUF);
     int *row_split = row_slot + kRowCap;
-
-    const WorkUnit U = units[blockIdx.x];
-    const WorkItem &it = items[U.item];
     const int nrows = uni(it.nrows);
     row_tables<64 * NW>(it, row_slot, row_split);
     const long long cb = U.cb, ce = U.ce;
@@ -564,6 +561,43 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_contract_bf16x3(
     }
 }
 
+// NW waves, TWW tile windows per wave, MINB workgroups per CU; one
+// workgroup per unit, blockIdx.y the field of the batch.
+template <int NW, int TWW, int MINB>
+__global__ __launch_bounds__(64 * NW, MINB) void k_contract_bf16x3(
+    const float2 *__restrict__ cache, const __bf16 *__restrict__ splitc,
+    const WorkItem *__restrict__ items, const WorkUnit *__restrict__ units,
+    const TWin *__restrict__ tws, double *__restrict__ out, BatchStrides bs)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    const WorkUnit U = units[blockIdx.x];
+    bf16x3_unit<NW, TWW>(cache + blockIdx.y * bs.cache, splitc + blockIdx.y * bs.cache * 4,
+                         items[U.item], U, tws, out + blockIdx.y * bs.part, lds);
+}
+
+// Persistent form over the units of several estimates (fields of different
+// geometries and batches): a round of resident workgroups takes units, in
+// the order given (largest first), from a queue counter until none is left;
+// each unit carries its own field, work lists and output.
+template <int NW, int TWW, int MINB>
+__global__ __launch_bounds__(64 * NW, MINB) void k_contract_bf16x3_mixed(
+    const MixUnit *__restrict__ mu, int nunits, int *__restrict__ next)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int take;
+    for (;;) {
+        __syncthreads();   // the previous unit is done with the LDS
+        if (threadIdx.x == 0) take = atomicAdd(next, 1);
+        __syncthreads();
+        const int u = uni(take);   // uniform: the unit's pointers are scalar loads
+        if (u >= nunits) break;
+        const MixUnit M = mu[u];
+        bf16x3_unit<NW, TWW>(M.cache, M.split, *M.item, M.unit, M.tws, M.out, lds);
+    }
+}
+
+__global__ void k_zero_int(int *p) { *p = 0; }
+
 // ----------------------------------------------------------------- host --
 int contract_chunk(CPrec p) { return p == CPrec::F64 ? kRowBytes / 16 : kRowBytes / 8; }
 
@@ -602,6 +636,33 @@ static hipError_t launch_nw(CPrec p, int tww, const void *cache, const void *spl
                                   (int)lds_d);
         hipLaunchKernelGGL(k, grid, dim3(64 * NW), lds_d, s, (const float2 *)cache,
                            (const __bf16 *)split, items, units, tws, out, bs);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_contract_mixed(int tww, const MixUnit *units, int nunits, int *counter,
+                                 hipStream_t s)
+{
+    if (nunits <= 0) return hipSuccess;
+    int ncu = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, counter);
+    if (tww == kTWWOcc3) {
+        auto k = k_contract_bf16x3_mixed<4, kTWWOcc3, 3>;
+        const size_t lds_d = dma_lds_bytes(4, kTWWOcc3);
+        (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds_d);
+        const int grid = std::min(nunits, 3 * ncu);
+        hipLaunchKernelGGL(k, dim3(grid), dim3(256), lds_d, s, units, nunits, counter);
+    } else {
+        auto k = k_contract_bf16x3_mixed<4, kTWW32, 2>;
+        const size_t lds_d = dma_lds_bytes(4, kTWW32);
+        (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds_d);
+        const int grid = std::min(nunits, 2 * ncu);
+        hipLaunchKernelGGL(k, dim3(grid), dim3(256), lds_d, s, units, nunits, counter);
     }
     return hipGetLastError();
 }

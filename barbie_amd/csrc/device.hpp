@@ -199,6 +199,20 @@ inline int contract_tww(CPrec p)
 // flagged blk_split are read from it).  One workgroup per unit.
 // waves: 4 or 8 waves per workgroup; tww: tile windows per wave (the work
 // lists must be packed for both).
+// One unit of a mixed BF16x3 launch: an estimate's field and work lists.
+struct MixUnit {
+    const float2 *cache;
+    const __bf16 *split;
+    const WorkItem *item;
+    const TWin *tws;
+    double *out;
+    WorkUnit unit;
+};
+// All units of several BF16x3 estimates with 4-wave layouts of one shape
+// (tww kTWW32 or kTWWOcc3) in one persistent launch; counter: one int of
+// device memory (the unit queue).
+hipError_t launch_contract_mixed(int tww, const MixUnit *units, int nunits, int *counter,
+                                 hipStream_t s);
 hipError_t launch_contract(CPrec p, int waves, int tww, const void *cache, const void *split,
                            const WorkItem *items, const WorkUnit *units, int nunits,
                            const TWin *tws, double *out, const BatchStrides &bs, int nf,

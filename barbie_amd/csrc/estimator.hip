@@ -985,6 +985,11 @@ struct bb_estimator {
     bool pending = false;             // an enqueued batch not yet waited for
     double2 *d_field = nullptr;       // staging for bb_estimate (host input)
     int field_cap = 0;                // fields the staging holds
+    MixUnit *d_mix = nullptr;         // bb_estimate_mixed (this estimator first): unit list
+    size_t mix_cap = 0;
+    int *d_mix_counter = nullptr;
+    std::vector<const void *> mix_key; // what the uploaded unit list was built for
+    int mix_n = 0;
     double2 *h_stage = nullptr;       // pinned host staging of the modes the estimate reads
     double2 *d_stage = nullptr;       // its device copy
     size_t stage_cap = 0;             // complex values both hold
@@ -1657,6 +1662,129 @@ int enqueue_kernels(bb_estimator *E, int nf, const double *d_fields, double *d_b
     return BB_OK;
 }
 
+// The stages before and after the contraction, on the estimator's stream
+// (bb_estimate_mixed runs the contractions of several estimators in one
+// launch between them).
+int enqueue_pre(bb_estimator *E, int nf, const double *d_fields)
+{
+    const PlanData &d = E->plan->d;
+    const Layout &L = *E->L;
+    const int nz = d.g.Nmax * d.g.Nr * nf;
+    HIPCHK(launch_gather((const double2 *)d_fields, E->d_planes, L.NN, L.NGtop, L.kmax,
+                         d.g.Nmax, nf, E->stream, E->d_nz, nz));
+    return run_fields(E, E->d_planes, L.NGtop, E->d_nz, nf);
+}
+int enqueue_post(bb_estimator *E, int nf, double *d_bs, double *d_norm, double *d_biana)
+{
+    const Layout &L = *E->L;
+    HIPCHK(launch_binning(E->d_sg, (int)L.sg.size() - 1, E->d_bin_sg, E->d_recs, E->d_part,
+                          E->d_num, E->d_nz, E->d_avg, L.bp, E->d_binpart, d_bs, d_norm, d_biana,
+                          E->bst, nf, E->stream));
+    return BB_OK;
+}
+
+int estimate_finish(bb_estimator *E);
+
+// Several estimates of different geometries with their contractions in one
+// persistent launch (bb_estimate_mixed).
+int estimate_mixed(int n, bb_estimator *const *ests, const int *nfs, const double *const *fields,
+                   double *const *bs, double *const *nm, double *const *bi)
+{
+    bb_estimator *E0 = ests[0];
+    const Layout &L0 = *E0->L;
+    for (int k = 0; k < n; ++k) {
+        bb_estimator *E = ests[k];
+        const Layout &L = *E->L;
+        if (E->cprec != CPrec::BF16X3 || L.waves != 4 || L.tww != L0.tww || !L.units_f.empty())
+            return set_error(BB_E_INVALID, "mixed estimates need BF16x3 estimators of one "
+                                           "4-wave layout shape (same occupancy, no mix_fp32)");
+        if (nfs[k] < 1 || nfs[k] > E->batch)
+            return set_error(BB_E_RANGE, "estimator " + std::to_string(k) + ": batch of " +
+                                             std::to_string(nfs[k]) + " fields");
+        if (!fields[k]) return set_error(BB_E_INVALID, "null field pointer");
+        for (int q = 0; q < k; ++q)
+            if (ests[q] == E) return set_error(BB_E_INVALID, "an estimator appears twice");
+        if (int rc = estimate_finish(E)) return rc;
+    }
+    // The unit list: every unit of every field of every estimate, largest
+    // first (chunks x tile windows of the item's busiest wave).
+    std::vector<const void *> key;
+    for (int k = 0; k < n; ++k) {
+        key.push_back(ests[k]);
+        key.push_back((const void *)(intptr_t)nfs[k]);
+    }
+    if (key != E0->mix_key) {
+        std::vector<std::pair<long long, MixUnit>> all;
+        for (int k = 0; k < n; ++k) {
+            bb_estimator *E = ests[k];
+            const Layout &L = *E->L;
+            for (int f = 0; f < nfs[k]; ++f)
+                for (const WorkUnit &u : L.units) {
+                    const WorkItem &it = L.items[u.item];
+                    int w = 0;
+                    for (int q = 0; q < kWaves; ++q) w = std::max(w, (int)it.cnt[q]);
+                    MixUnit M;
+                    M.cache = reinterpret_cast<const float2 *>(E->d_cache) + (size_t)f * E->bst.cache;
+                    M.split = reinterpret_cast<const __bf16 *>(E->d_split) + (size_t)f * E->bst.cache * 4;
+                    M.item = E->d_items + u.item;
+                    M.tws = E->d_tws;
+                    M.out = E->d_part + (size_t)f * E->bst.part;
+                    M.unit = u;
+                    all.push_back({(long long)(u.ce - u.cb) * w, M});
+                }
+        }
+        std::stable_sort(all.begin(), all.end(),
+                         [](const auto &a, const auto &b) { return a.first > b.first; });
+        std::vector<MixUnit> host;
+        host.reserve(all.size());
+        for (auto &x : all) host.push_back(x.second);
+        if (E0->mix_cap < host.size()) {
+            MixUnit *p = nullptr;
+            if (int rc = dev_alloc(&p, host.size(), E0->owned)) return rc;
+            E0->d_mix = p;
+            E0->mix_cap = host.size();
+        }
+        if (!E0->d_mix_counter)
+            if (int rc = dev_alloc(&E0->d_mix_counter, 1, E0->owned)) return rc;
+        if (int rc = upload(E0->d_mix, host.data(), sizeof(MixUnit) * host.size(), E0->stream))
+            return rc;
+        HIPCHK(hipStreamSynchronize(E0->stream));
+        E0->mix_key = key;
+        E0->mix_n = (int)host.size();
+    }
+    // Stages before the contraction on each estimator's stream; the fused
+    // contraction on the first's once all are done; binning after it.
+    std::vector<hipEvent_t> ev(n + 1, nullptr);
+    for (auto &e : ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    int rc = BB_OK;
+    for (int k = 0; k < n && !rc; ++k) {
+        rc = enqueue_pre(ests[k], nfs[k], fields[k]);
+        if (!rc && hipEventRecord(ev[k], ests[k]->stream) != hipSuccess)
+            rc = set_error(BB_E_DEVICE, "event record failed");
+    }
+    for (int k = 0; k < n && !rc; ++k)
+        if (hipStreamWaitEvent(E0->stream, ev[k], 0) != hipSuccess)
+            rc = set_error(BB_E_DEVICE, "stream wait failed");
+    if (!rc) {
+        const hipError_t e = launch_contract_mixed(L0.tww, E0->d_mix, E0->mix_n, E0->d_mix_counter,
+                                                   E0->stream);
+        if (e != hipSuccess) rc = set_error(BB_E_DEVICE, hipGetErrorString(e));
+    }
+    if (!rc && hipEventRecord(ev[n], E0->stream) != hipSuccess)
+        rc = set_error(BB_E_DEVICE, "event record failed");
+    for (int k = 0; k < n && !rc; ++k) {
+        if (hipStreamWaitEvent(ests[k]->stream, ev[n], 0) != hipSuccess)
+            rc = set_error(BB_E_DEVICE, "stream wait failed");
+        if (!rc) rc = enqueue_post(ests[k], nfs[k], bs[k], nm[k], bi[k]);
+    }
+    for (int k = 0; k < n; ++k)
+        if (hipStreamSynchronize(ests[k]->stream) != hipSuccess && !rc)
+            rc = set_error(BB_E_DEVICE, "mixed estimate failed");
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    (void)hipGetLastError();
+    return rc;
+}
+
 // Enqueues the estimates of nf fields (back to back from d_fields) on the
 // estimator's stream; outputs back to back per field.  Nothing waits.  With
 // use_graph, the first call for a set of arguments runs eagerly and captures
@@ -2222,6 +2350,18 @@ int bb_export_cache(bb_estimator *e, const double *field_r2c, const char *dir, i
     if (!e || !field_r2c || !dir) return set_error(BB_E_INVALID, "null argument");
     if (int rc = estimate_finish(e)) return rc;
     return export_cache(e, field_r2c, dir, NRi);
+}
+
+int bb_estimate_mixed(int n, bb_estimator *const *ests, const int *nfields,
+                      const double *const *d_fields, double *const *d_bs, double *const *d_norm,
+                      double *const *d_biana)
+{
+    if (n < 1 || !ests || !nfields || !d_fields || !d_bs || !d_norm || !d_biana)
+        return set_error(BB_E_INVALID, "null argument");
+    for (int k = 0; k < n; ++k)
+        if (!ests[k] || !d_bs[k] || !d_norm[k] || !d_biana[k])
+            return set_error(BB_E_INVALID, "null argument");
+    return estimate_mixed(n, ests, nfields, d_fields, d_bs, d_norm, d_biana);
 }
 
 int bb_import_rings(bb_estimator *e, const char *dir, int NRi, double rel_tol, int use_num,

@@ -31,7 +31,7 @@ EXPORTED = [
     "bb_estimator_create", "bb_estimator_create_opts",
     "bb_estimator_destroy", "bb_estimator_grid", "bb_estimator_levels",
     "bb_estimator_footprint", "bb_estimator_footprint_opts", "bb_estimate", "bb_estimate_device",
-    "bb_estimate_batch", "bb_estimate_batch_device", "bb_estimate_batch_async",
+    "bb_estimate_batch", "bb_estimate_batch_device", "bb_estimate_batch_async", "bb_estimate_mixed",
     "bb_estimator_query", "bb_estimator_wait", "bb_estimator_stream", "bb_estimator_set_stream",
     "bb_estimator_contraction_sums", "bb_estimator_triangle_counts", "bb_estimator_field",
     "bb_estimator_timings", "bb_estimator_worklist", "bb_write_outputs", "bb_write_outputs_checked", "bb_export_cache", "bb_import_rings", "bb_check_plane_cache",
@@ -160,6 +160,9 @@ def lib() -> ctypes.CDLL:
         L.bb_estimator_timings.argtypes = [ctypes.c_void_p, _dp]
         L.bb_estimator_worklist.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)]
         L.bb_export_cache.argtypes = [ctypes.c_void_p, _dp, ctypes.c_char_p, ctypes.c_int]
+        _vpp = ctypes.POINTER(ctypes.c_void_p)
+        L.bb_estimate_mixed.argtypes = [ctypes.c_int, _vpp, ctypes.POINTER(ctypes.c_int), _vpp,
+                                        _vpp, _vpp, _vpp]
         L.bb_import_rings.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_double,
                                       ctypes.c_int, ctypes.POINTER(ctypes.c_longlong)]
         L.bb_check_plane_cache.argtypes = [ctypes.c_void_p, _dp, ctypes.c_char_p, ctypes.c_int,
@@ -565,3 +568,14 @@ def generate_field(NN: int, seed: int, out=None, amp: float = 1e3, index: float 
         return host
     _check(lib().bb_generate_field(int(NN), int(seed), ctypes.byref(p), ctypes.c_void_p(out), 1))
     return None
+
+
+def estimate_mixed(estimators, nfields, d_fields, d_bs, d_norm, d_biana):
+    """bb_estimate_mixed: estimators (Estimator objects, BF16x3, one layout
+    shape), per estimator the field count and the device addresses of its
+    fields and of its three outputs."""
+    n = len(estimators)
+    arr = lambda xs: (ctypes.c_void_p * n)(*[ctypes.c_void_p(int(x)) for x in xs])
+    _check(lib().bb_estimate_mixed(n, arr([e.handle.value for e in estimators]),
+                                   (ctypes.c_int * n)(*[int(x) for x in nfields]), arr(d_fields),
+                                   arr(d_bs), arr(d_norm), arr(d_biana)))

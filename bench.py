@@ -194,6 +194,55 @@ def batched_leg(B, torch, precision, nfields=64, distinct=8, reps=3):
     return res
 
 
+def mixed_leg(B, torch, counts=(22, 21, 21), reps=3):
+    """64 seeded realizations of mixed NN in {128, 192, 256} (22/21/21 fields)
+    in one call: the contractions of all of them in one persistent launch
+    (bb_estimate_mixed, BF16x3), against the same fields as one batched
+    call per NN, one after the other.  Fields/hour of both, and whether every
+    field's bins are the same bits both ways."""
+    import numpy as np
+    sizes = (128, 192, 256)
+    ests, fields, outs = [], [], []
+    for NN, nf in zip(sizes, counts):
+        g = geometry(B, NN)
+        plan = B.Plan(g)
+        ests.append(B.Estimator(plan, "bf16x3", max_batch=nf, waves=4, occupancy=3))
+        f = torch.empty((nf, NN, NN, NN + 2), dtype=torch.float64, device="cuda")
+        for k in range(nf):
+            B.generate_field(NN, B.seed_for(k + 1), out=f[k].data_ptr())
+        fields.append(f)
+        outs.append(torch.zeros(2, 3, nf, g.nbins * 16, dtype=torch.float64, device="cuda"))
+    torch.cuda.synchronize()
+
+    def separate(o):
+        for e, f, out, nf in zip(ests, fields, outs, counts):
+            e.estimate_batch_device(nf, f.data_ptr(), out[o, 0].data_ptr(), out[o, 1].data_ptr(),
+                                    out[o, 2].data_ptr())
+
+    def mixed(o):
+        B.estimate_mixed(ests, counts, [f.data_ptr() for f in fields],
+                         [x[o, 0].data_ptr() for x in outs], [x[o, 1].data_ptr() for x in outs],
+                         [x[o, 2].data_ptr() for x in outs])
+
+    res = {}
+    for name, fn, o in (("separate", separate, 0), ("mixed", mixed, 1)):
+        fn(o)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn(o)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        res[name] = {"fields_per_hour": sum(counts) * 3600.0 / dt, "ms_per_call": dt * 1e3}
+    same = all(bool(torch.equal(x[0], x[1])) for x in outs)
+    for e in ests:
+        e.close()
+    return {"fields": dict(zip([str(n) for n in sizes], counts)), "layout": "waves=4,occupancy=3",
+            "separate_per_NN_batches": res["separate"], "mixed_one_launch": res["mixed"],
+            "speedup": res["separate"]["ms_per_call"] / res["mixed"]["ms_per_call"],
+            "bins_identical": same}
+
+
 def serving_leg(B, torch, precision, n_sat=200, n_load=300, seed=7):
     """Request server over the NN mix 128/256/384/512 (weights 4:3:2:1), one
     device-resident field per NN.  Saturation: a burst of n_sat requests.
@@ -638,6 +687,8 @@ def main():
     torch.cuda.empty_cache()
     if not a.no_batched:
         out["batched"] = batched_leg(B, torch, a.precision)
+        if a.precision == "bf16x3":
+            out["batched"]["mixed_NN"] = mixed_leg(B, torch)
     if not a.no_serving:
         sv = serving_leg(B, torch, a.precision)
         out["serving"] = sv

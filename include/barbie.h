@@ -212,6 +212,20 @@ int bb_estimate_batch_device(bb_estimator *e, int nfields, const double *d_field
  * the estimator waits for all pending batches first. */
 int bb_estimate_batch_async(bb_estimator *e, int nfields, const double *d_fields_r2c,
                             double *d_bin_bs, double *d_bin_norm, double *d_bin_biana);
+/* Mixed geometries (no counterpart in the reference, which runs one field
+ * of one geometry per process): the estimates of n BF16x3 estimators of
+ * different plans -- nfields[k] fields back to back at d_fields[k] for
+ * estimator k, outputs as bb_estimate_batch_device's -- with the contraction
+ * work of all of them in one persistent launch: a round of resident
+ * workgroups takes the units of every field of every geometry from one
+ * queue, largest first.  The plane gathers and transforms run on each
+ * estimator's stream before it, the binning after it; the call returns when
+ * all are done.  The estimators must share one 4-wave layout shape (the
+ * same occupancy, mix_fp32 0); each field's bins are the same bits as its
+ * own estimator gives it alone. */
+int bb_estimate_mixed(int n, bb_estimator *const *ests, const int *nfields,
+                      const double *const *d_fields, double *const *d_bs, double *const *d_norm,
+                      double *const *d_biana);
 int bb_estimator_query(bb_estimator *e, int *done);
 int bb_estimator_wait(bb_estimator *e);
 /* The estimator's HIP stream (hipStream_t), for callers that order their own

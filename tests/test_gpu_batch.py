@@ -218,3 +218,48 @@ def test_host_field_upload_reads_only_the_modes_it_needs(full_grid):
     two = est.estimate_batch(np.stack([dirty, f]))
     for k in range(3):
         assert np.array_equal(two[k][0].ravel(), ref[k]) and np.array_equal(two[k][1].ravel(), ref[k])
+
+
+def test_mixed_geometries_in_one_launch():
+    """bb_estimate_mixed: fields of three geometries (NN 48, 64, 128) with
+    their contractions in one persistent launch give each field the bits its
+    own estimator gives it; estimators of another precision or layout shape,
+    or one estimator twice, are refused."""
+    torch = pytest.importorskip("torch")
+    geoms = [medium(), B.Geometry(NN=64, UL_k1=10, LL_k1=1, BW_k1=1, Nmax=16, Nr=8,
+                                  ring_rule=B.RINGS_UNIFORM, ring_width=1.0, last_ring_to_nmax=0),
+             config1()]
+    nfs = [3, 2, 2]
+    opts = dict(waves=4, occupancy=3)
+    ests, fields, outs, refs = [], [], [], []
+    for g, nf in zip(geoms, nfs):
+        e = B.Estimator(B.Plan(g), "bf16x3", max_batch=nf, **opts)
+        f = torch.from_numpy(np.stack([B.make_field(g.NN, NRi=k + 1) for k in range(nf)])).to("cuda")
+        nb = g.nbins * 16
+        o = torch.zeros(3, nf, nb, dtype=torch.float64, device="cuda")
+        r = torch.zeros(3, nf, nb, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        e.estimate_batch_device(nf, f.data_ptr(), r[0].data_ptr(), r[1].data_ptr(), r[2].data_ptr())
+        ests.append(e)
+        fields.append(f)
+        outs.append(o)
+        refs.append(r.cpu().numpy())
+    for rep in range(2):   # the second call reuses the uploaded unit list
+        for o in outs:
+            o.zero_()
+        torch.cuda.synchronize()
+        B.estimate_mixed(ests, nfs, [f.data_ptr() for f in fields], [o[0].data_ptr() for o in outs],
+                         [o[1].data_ptr() for o in outs], [o[2].data_ptr() for o in outs])
+        for o, r in zip(outs, refs):
+            assert np.array_equal(o.cpu().numpy(), r), rep
+    e64 = B.Estimator(B.Plan(geoms[0]), "fp64")
+    args = lambda es: (es, [1] * len(es), [fields[0].data_ptr()] * len(es),
+                       [outs[0][0].data_ptr()] * len(es), [outs[0][1].data_ptr()] * len(es),
+                       [outs[0][2].data_ptr()] * len(es))
+    with pytest.raises(B.BarbieError):
+        B.estimate_mixed(*args([ests[0], e64]))
+    with pytest.raises(B.BarbieError):
+        B.estimate_mixed(*args([ests[0], ests[0]]))
+    e8 = B.Estimator(B.Plan(geoms[1]), "bf16x3", waves=8)
+    with pytest.raises(B.BarbieError):
+        B.estimate_mixed(*args([ests[0], e8]))
