@@ -2426,6 +2426,22 @@ static void worklist_of(const Layout &L, long long *o)
     for (const Level &V : L.lv)
         for (const FieldTask &T : V.tasks) band_rows += std::min(V.NG, 2 * T.rowband + 1);
     o[16] = band_rows;
+    // Pair products formed (16 rows per window that starts a block), the
+    // rows of them that hold a pair, and tile windows, each times the points
+    // of the item's grid.
+    long long formed = 0, used = 0, wpts = 0;
+    for (const WorkItem &it : L.items)
+        for (int w = 0; w < L.waves; ++w)
+            for (int j = 0; j < it.cnt[w]; ++j) {
+                const TWin &T = L.tws[(size_t)it.tw0 + w * kTWW + j];
+                wpts += it.npts;
+                if (!(T.flags & TW_NEWA)) continue;
+                formed += 16 * it.npts;
+                for (int m = 0; m < 16; ++m) used += T.a1[m] >= 0 ? it.npts : 0;
+            }
+    o[17] = formed;
+    o[18] = used;
+    o[19] = wpts;
 }
 
 int bb_estimator_worklist(const bb_estimator *e, long long *o)

@@ -199,6 +199,22 @@ __global__ __launch_bounds__(kPassThreads) void k_rowpass(const double2 *__restr
     }
 }
 
+// Two FP32 values as a bf16 pair, round to nearest even (one conversion
+// instruction); `a` in the low half.
+__device__ __forceinline__ unsigned pk_bf16(float a, float b)
+{
+    typedef float f32x2 __attribute__((ext_vector_type(2)));
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2){a, b}, bf16x2));
+}
+// hi = bf16(v) and lo = bf16(v - hi) of two values, as bf16 pairs.  v - hi
+// is exact in FP32 (hi holds the leading 8 bits of v).
+__device__ __forceinline__ void split2(float a, float b, unsigned &hi, unsigned &lo)
+{
+    hi = pk_bf16(a, b);
+    lo = pk_bf16(a - __uint_as_float(hi << 16), b - __uint_as_float(hi & 0xffff0000u));
+}
+
 // 8 consecutive points of the result to the cache (dst, unless null) and
 // the bf16 operands (sp, unless null), as 16-byte stores.
 __device__ __forceinline__ void store8(double2 *dst, __bf16 *, const double2 *v)
@@ -223,19 +239,17 @@ __device__ __forceinline__ void store8(float2 *dst, __bf16 *sp, const float2 *v)
     }
     if (sp) {
         // [Re hi x8][Re lo x8][Im hi x8][Im lo x8], hi = bf16(v), lo = bf16(v - hi)
-        __bf16 g[32];
+        unsigned h[2][4], l[2][4];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const __bf16 hr = (__bf16)re[t], hm = (__bf16)im[t];
-            g[t] = hr;
-            g[8 + t] = (__bf16)(re[t] - (float)hr);
-            g[16 + t] = hm;
-            g[24 + t] = (__bf16)(im[t] - (float)hm);
+        for (int q = 0; q < 4; ++q) {
+            split2(re[2 * q], re[2 * q + 1], h[0][q], l[0][q]);
+            split2(im[2 * q], im[2 * q + 1], h[1][q], l[1][q]);
         }
-        const uint4 *src = reinterpret_cast<const uint4 *>(g);
         uint4 *s4 = reinterpret_cast<uint4 *>(sp);
-#pragma unroll
-        for (int t = 0; t < 4; ++t) s4[t] = src[t];
+        s4[0] = make_uint4(h[0][0], h[0][1], h[0][2], h[0][3]);
+        s4[1] = make_uint4(l[0][0], l[0][1], l[0][2], l[0][3]);
+        s4[2] = make_uint4(h[1][0], h[1][1], h[1][2], h[1][3]);
+        s4[3] = make_uint4(l[1][0], l[1][1], l[1][2], l[1][3]);
     }
 }
 
@@ -332,10 +346,15 @@ __global__ __launch_bounds__(kPassThreads) void k_colpass(C2 *__restrict__ cache
             const bool own = T.forms & 1, ops = split && (T.forms & 2);
             for (int g = threadIdx.x; g < ngroups; g += blockDim.x) {
                 C2 v[8];
+                if (8 * g + 8 <= npt) {   // a whole group: no per-point test
 #pragma unroll
-                for (int t = 0; t < 8; ++t) {
-                    const int x = 8 * g + t;
-                    v[t] = x < npt ? res[x] : mk<C2>(0, 0);
+                    for (int t = 0; t < 8; ++t) v[t] = res[8 * g + t];
+                } else {
+#pragma unroll
+                    for (int t = 0; t < 8; ++t) {
+                        const int x = 8 * g + t;
+                        v[t] = x < npt ? res[x] : mk<C2>(0, 0);
+                    }
                 }
                 store8(own ? fld + 8 * g : nullptr, ops ? split + (base + 8 * g) * 4 : nullptr, v);
             }

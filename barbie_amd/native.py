@@ -45,7 +45,8 @@ REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 # bb_estimator_worklist / bb_plan_worklist entries, in order.
 WORKLIST_KEYS = ("items", "units", "tiles", "max_rings", "staged_rings", "staged_bytes", "grid",
                  "chunk", "waves", "wave_balance_x1000", "levels", "partial_tiles",
-                 "points_summed", "slots", "total_cost", "cache_points", "band_rows")
+                 "points_summed", "slots", "total_cost", "cache_points", "band_rows",
+                 "pair_points", "pair_points_used", "window_points")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)

@@ -255,11 +255,13 @@ int bb_estimator_field(bb_estimator *e, int level, int plane, int ring, double *
  * their level (points summed), [13] tile-window slots (items x waves x
  * windows per wave), [14] cost of the busiest waves (windows x chunks, summed
  * over items), [15] points of all plane-ring fields of all levels (the
- * cache), [16] band rows the row pass transforms (summed over fields).  The
- * environment variables
+ * cache), [16] band rows the row pass transforms (summed over fields),
+ * [17] pair-product points formed (16 rows per pair block, times the points
+ * of its grid), [18] the part of [17] in rows that hold a pair, [19] tile
+ * windows times the points of their grid.  The environment variables
  * BARBIE_CONTRACT_WAVES (4 or 8) and BARBIE_GRID_LEVELS (overrides
  * grid_levels) are read when the estimator is created. */
-#define BB_WORKLIST_N 17
+#define BB_WORKLIST_N 20
 int bb_estimator_worklist(const bb_estimator *e, long long *out);
 /* The same work-list figures for an estimator with these options, computed
  * on the host without a device ([13]: tile window slots executed, i.e.
