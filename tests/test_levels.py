@@ -131,3 +131,27 @@ def test_occupancy_option_is_validated():
     w2 = plan.worklist("bf16x3", waves=4)
     assert w3["max_rings"] <= 95
     assert w3["tiles"] == w2["tiles"]
+
+
+def test_pair_point_figures_are_consistent():
+    """The work-list figures of the pair products: every contraction's pair
+    row is formed once, so the pair points used lie between the points of
+    the distinct pairs and those formed (16 rows per block); tile windows
+    times points cover every contraction's points once per 16 columns."""
+    g = medium()
+    plan = B.Plan(g)
+    C = plan.contractions()
+    for prec, opts in (("bf16x3", dict(waves=4, occupancy=3)), ("fp64", {})):
+        w = plan.worklist(prec, **opts)
+        grids, cl = plan.grid_levels(prec)
+        ng2 = np.array([grids[l] ** 2 for l in cl], dtype=np.int64)
+        # distinct pair rows (level, p1, r1, signed p2, r2), each with its points
+        pairs = {}
+        for c in range(len(C)):
+            pairs[(cl[c], C[c, 0], C[c, 1], C[c, 2], C[c, 3])] = ng2[c]
+        distinct = sum(pairs.values())
+        assert w["pair_points_used"] >= distinct
+        assert w["pair_points"] >= w["pair_points_used"]
+        assert w["pair_points"] % 16 == 0
+        assert 256 * w["window_points"] >= w["points_summed"]
+        assert w["points_summed"] == int(ng2.sum())
