@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -988,8 +989,9 @@ struct bb_estimator {
     MixUnit *d_mix = nullptr;         // bb_estimate_mixed (this estimator first): unit list
     size_t mix_cap = 0;
     int *d_mix_counter = nullptr;
-    std::vector<const void *> mix_key; // what the uploaded unit list was built for
+    std::vector<unsigned long long> mix_key;   // what the uploaded unit list was built for
     int mix_n = 0;
+    unsigned long long serial = 0;    // creation number: unique per estimator in the process
     double2 *h_stage = nullptr;       // pinned host staging of the modes the estimate reads
     double2 *d_stage = nullptr;       // its device copy
     size_t stage_cap = 0;             // complex values both hold
@@ -1707,11 +1709,13 @@ int estimate_mixed(int n, bb_estimator *const *ests, const int *nfs, const doubl
         if (int rc = estimate_finish(E)) return rc;
     }
     // The unit list: every unit of every field of every estimate, largest
-    // first (chunks x tile windows of the item's busiest wave).
-    std::vector<const void *> key;
+    // first (chunks x tile windows of the item's busiest wave).  It holds the
+    // estimators' device addresses, so it is keyed by their creation numbers
+    // (an estimator created at a destroyed one's address gets a new number).
+    std::vector<unsigned long long> key;
     for (int k = 0; k < n; ++k) {
-        key.push_back(ests[k]);
-        key.push_back((const void *)(intptr_t)nfs[k]);
+        key.push_back(ests[k]->serial);
+        key.push_back((unsigned long long)nfs[k]);
     }
     if (key != E0->mix_key) {
         std::vector<std::pair<long long, MixUnit>> all;
@@ -2157,6 +2161,8 @@ int bb_estimator_create_opts(const bb_plan *p, const bb_options *opt, bb_estimat
     if (rc) return rc;
     bb_estimator *E = new (std::nothrow) bb_estimator;
     if (!E) return set_error(BB_E_NOMEM, "out of host memory");
+    static std::atomic<unsigned long long> serials{0};
+    E->serial = ++serials;
     E->plan = p;
     E->opt = *opt;
     E->cprec = cp;

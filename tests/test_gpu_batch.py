@@ -252,6 +252,18 @@ def test_mixed_geometries_in_one_launch():
                          [o[1].data_ptr() for o in outs], [o[2].data_ptr() for o in outs])
         for o, r in zip(outs, refs):
             assert np.array_equal(o.cpu().numpy(), r), rep
+    # A replaced estimator (possibly at the old one's address) gets a fresh
+    # unit list: the same call shape must not reuse the old device addresses.
+    ests[1].close()
+    ests[1] = B.Estimator(B.Plan(geoms[1]), "bf16x3", max_batch=nfs[1], **opts)
+    torch.cuda.synchronize()
+    for o in outs:
+        o.zero_()
+    torch.cuda.synchronize()
+    B.estimate_mixed(ests, nfs, [f.data_ptr() for f in fields], [o[0].data_ptr() for o in outs],
+                     [o[1].data_ptr() for o in outs], [o[2].data_ptr() for o in outs])
+    for o, r in zip(outs, refs):
+        assert np.array_equal(o.cpu().numpy(), r), "after replacing an estimator"
     e64 = B.Estimator(B.Plan(geoms[0]), "fp64")
     args = lambda es: (es, [1] * len(es), [fields[0].data_ptr()] * len(es),
                        [outs[0][0].data_ptr()] * len(es), [outs[0][1].data_ptr()] * len(es),
