@@ -577,7 +577,7 @@ __global__ __launch_bounds__(64 * NW, MINB) void k_contract_bf16x3(
 
 // Persistent form over the units of several estimates (fields of different
 // geometries and batches): a round of resident workgroups takes units, in
-// the order given (largest first), from a queue counter until none is left;
+// the order given, from a queue counter until none is left;
 // each unit carries its own field, work lists and output.
 template <int NW, int TWW, int MINB>
 __global__ __launch_bounds__(64 * NW, MINB) void k_contract_bf16x3_mixed(

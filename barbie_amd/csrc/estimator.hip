@@ -1718,15 +1718,31 @@ int estimate_mixed(int n, bb_estimator *const *ests, const int *nfs, const doubl
         key.push_back((unsigned long long)nfs[k]);
     }
     if (key != E0->mix_key) {
-        std::vector<std::pair<long long, MixUnit>> all;
+        // Estimators in order of decreasing mean unit cost, each one's fields
+        // in turn, each field's units in the layout's largest-first order:
+        // the large units of every size come first (a short tail), and the
+        // units of one field run together (its cache stays in the caches).
+        auto unit_cost = [](const Layout &L, const WorkUnit &u) {
+            const WorkItem &it = L.items[u.item];
+            int w = 0;
+            for (int q = 0; q < kWaves; ++q) w = std::max(w, (int)it.cnt[q]);
+            return (double)(u.ce - u.cb) * w;
+        };
+        std::vector<std::pair<double, int>> order;
         for (int k = 0; k < n; ++k) {
-            bb_estimator *E = ests[k];
+            const Layout &L = *ests[k]->L;
+            double c = 0;
+            for (const WorkUnit &u : L.units) c += unit_cost(L, u);
+            order.push_back({L.units.empty() ? 0.0 : c / (double)L.units.size(), k});
+        }
+        std::stable_sort(order.begin(), order.end(),
+                         [](const auto &a, const auto &b) { return a.first > b.first; });
+        std::vector<MixUnit> host;
+        for (const auto &o : order) {
+            bb_estimator *E = ests[o.second];
             const Layout &L = *E->L;
-            for (int f = 0; f < nfs[k]; ++f)
+            for (int f = 0; f < nfs[o.second]; ++f)
                 for (const WorkUnit &u : L.units) {
-                    const WorkItem &it = L.items[u.item];
-                    int w = 0;
-                    for (int q = 0; q < kWaves; ++q) w = std::max(w, (int)it.cnt[q]);
                     MixUnit M;
                     M.cache = reinterpret_cast<const float2 *>(E->d_cache) + (size_t)f * E->bst.cache;
                     M.split = reinterpret_cast<const __bf16 *>(E->d_split) + (size_t)f * E->bst.cache * 4;
@@ -1734,14 +1750,9 @@ int estimate_mixed(int n, bb_estimator *const *ests, const int *nfs, const doubl
                     M.tws = E->d_tws;
                     M.out = E->d_part + (size_t)f * E->bst.part;
                     M.unit = u;
-                    all.push_back({(long long)(u.ce - u.cb) * w, M});
+                    host.push_back(M);
                 }
         }
-        std::stable_sort(all.begin(), all.end(),
-                         [](const auto &a, const auto &b) { return a.first > b.first; });
-        std::vector<MixUnit> host;
-        host.reserve(all.size());
-        for (auto &x : all) host.push_back(x.second);
         if (E0->mix_cap < host.size()) {
             MixUnit *p = nullptr;
             if (int rc = dev_alloc(&p, host.size(), E0->owned)) return rc;

@@ -218,7 +218,8 @@ int bb_estimate_batch_async(bb_estimator *e, int nfields, const double *d_fields
  * estimator k, outputs as bb_estimate_batch_device's -- with the contraction
  * work of all of them in one persistent launch: a round of resident
  * workgroups takes the units of every field of every geometry from one
- * queue, largest first.  The plane gathers and transforms run on each
+ * queue (the estimators in order of decreasing unit size, each one's fields
+ * in turn, each field's units largest first).  The plane gathers and transforms run on each
  * estimator's stream before it, the binning after it; the call returns when
  * all are done.  The estimators must share one 4-wave layout shape (the
  * same occupancy, mix_fp32 0); each field's bins are the same bits as its
