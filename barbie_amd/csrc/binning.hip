@@ -3,7 +3,7 @@
 // comparison), :343-363 (bins, zero-signal rule, accumulation) and cal_coeff
 // (funcs.c:708-745).
 //
-// Contractions arrive sorted by bin, cut into segments of at most kBinSeg
+// Contractions arrive sorted by bin (20-byte records), cut into segments of at most kBinSeg
 // records within one bin (so that the largest bins do not set the time: at
 // NN=384 one bin holds 26 110 of 2.76 M records).  One workgroup of four
 // waves owns one segment: each thread accumulates a strided subset of the
@@ -134,7 +134,10 @@ __global__ __launch_bounds__(kBinThreads) void k_binning(const int32_t *__restri
         Biana = Biana * (1 + bp.beta * m1 * m1) * (1 + bp.beta * m2 * m2) * (1 + bp.beta * m3 * m3);
         // barbie.c:355-357.  The sum is exactly zero iff one of its three
         // fields is identically zero; the fields' zero flags decide it.
-        if (!nzflag[R.f1] || !nzflag[R.f2] || !nzflag[R.f3]) {
+        const int f1 = R.p1 * bp.Nr + R.r1;   // p1 >= 0
+        const int f2 = (R.p2 < 0 ? -R.p2 : R.p2) * bp.Nr + R.r2;
+        const int f3 = (R.p3 < 0 ? -R.p3 : R.p3) * bp.Nr + R.r3;
+        if (!nzflag[f1] || !nzflag[f2] || !nzflag[f3]) {
             Biana = 0.;
             Num = 0.;
         }

@@ -129,19 +129,22 @@ struct WorkUnit {
 };
 
 // Contraction record used by the binning kernel (bin-sorted order).
+// One contraction for the binning (20 bytes).  The nonzero flag of a field
+// (the zero-field rule) is flag |p| * Nr + r.
 struct BinRec {
-    int16_t p1, p2, p3, r1, r2, r3;
-    int16_t nsp;     // partial sums (x-splits), 256 doubles apart
-    int16_t pad;
-    int32_t rp;      // ring triple (index into Num)
-    int32_t pos;     // position of the contraction's first partial sum
-    int32_t f1, f2, f3;  // nonzero flags of the three fields (zero-field rule)
+    int16_t p1, p2, p3;
+    int16_t nsp;             // partial sums (x-splits), 256 doubles apart
+    uint8_t r1, r2, r3, pad; // rings (Nr <= 255)
+    int32_t rp;              // ring triple (index into Num)
+    int32_t pos;             // position of the contraction's first partial sum
 };
+static_assert(sizeof(BinRec) == 20, "binning record layout");
 
 struct BinParams {
     int nbins;
     int n_mtbin;
     int NN;
+    int Nr;        // rings per plane (nonzero flag of (p, r): |p| * Nr + r)
     double Cunits;
     double ff, beta;
 };

@@ -895,6 +895,8 @@ int build_layout(const PlanData &d, CPrec prec, const bb_options &opt, int waves
     if ((rc = check_lists(L, d.contr.size()))) return rc;
 
     // Binning records, sorted by bin.
+    if (Nr > 255 || Nmax > 32767)
+        return set_error(BB_E_INVALID, "the binning records hold up to 255 rings and 32767 planes");
     L.recs.resize(d.contr.size());
     L.seg.assign(d.nbins + 1, 0);
     for (const Contraction &k : d.contr) L.seg[k.bin + 1]++;
@@ -905,14 +907,11 @@ int build_layout(const PlanData &d, CPrec prec, const bb_options &opt, int waves
             const Contraction &k = d.contr[c];
             BinRec R;
             std::memset(&R, 0, sizeof R);
-            R.p1 = k.p1; R.p2 = k.p2; R.p3 = k.p3;
-            R.r1 = k.r1; R.r2 = k.r2; R.r3 = k.r3;
+            R.p1 = (int16_t)k.p1; R.p2 = (int16_t)k.p2; R.p3 = (int16_t)k.p3;
+            R.r1 = (uint8_t)k.r1; R.r2 = (uint8_t)k.r2; R.r3 = (uint8_t)k.r3;
             R.nsp = L.nsp[c];
             R.rp = k.rp;
             R.pos = L.pos[c];
-            R.f1 = k.p1 * Nr + k.r1;
-            R.f2 = std::abs(k.p2) * Nr + k.r2;
-            R.f3 = std::abs(k.p3) * Nr + k.r3;
             L.recs[at[k.bin]++] = R;
         }
     }
@@ -934,6 +933,7 @@ int build_layout(const PlanData &d, CPrec prec, const bb_options &opt, int waves
     L.bp.nbins = d.nbins;
     L.bp.n_mtbin = d.g.n_mtbin;
     L.bp.NN = NN;
+    L.bp.Nr = Nr;
     L.bp.Cunits = 2. * M_PI / (NN * d.g.LL);
     L.bp.ff = d.g.ff;
     L.bp.beta = d.g.beta;
