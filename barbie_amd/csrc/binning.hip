@@ -17,8 +17,11 @@
 namespace bb {
 
 // 16 orientation weights bs[] and norms norm[] of (px, pz), funcs.c:708-745.
+// Contraction to FMA is off here and in rec_coeffs, so that every kernel that
+// inlines them computes the same bits.
 __device__ __forceinline__ void cal_coeff(double px, double pz, double *bs, double *norm)
 {
+#pragma clang fp contract(off)
     const double x2 = px * px, x4 = x2 * x2, x6 = x4 * x2;
     const double z2 = pz * pz, z4 = z2 * z2, z6 = z4 * z2;
     const double om = -1. + z2;  // (-1 + pz^2)
@@ -70,6 +73,93 @@ __device__ __forceinline__ void cal_coeff(double px, double pz, double *bs, doub
 
 constexpr int kBinThreads = 256;   // four waves per bin
 
+// The field-independent part of one contraction's binning: the triangle
+// parameters (barbie.c:287-298; funcs.c:113-126 sort), the weights and norms
+// (cal_coeff) and the analytic bispectrum (barbie.c:301-305) before the zero
+// rule and the triangle count.
+__device__ __forceinline__ void rec_coeffs(const BinRec &R, const double *__restrict__ avg,
+                                           const BinParams &bp, double *bs, double *nm,
+                                           double &biana)
+{
+#pragma clang fp contract(off)
+    const double a1 = avg[R.r1], a2 = avg[R.r2], a3 = avg[R.r3];
+    const double p1 = R.p1, p2 = R.p2, p3 = R.p3;
+    double K[3][2] = {{sqrt(a1 * a1 + p1 * p1), p1},
+                      {sqrt(a2 * a2 + p2 * p2), p2},
+                      {sqrt(a3 * a3 + p3 * p3), p3}};
+    // funcs.c:113-126: descending, strict '<'
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2 - i; j++)
+            if (K[j][0] < K[j + 1][0]) {
+                double s1 = K[j][0], s2 = K[j][1];
+                K[j][0] = K[j + 1][0]; K[j][1] = K[j + 1][1];
+                K[j + 1][0] = s1; K[j + 1][1] = s2;
+            }
+    const double T = K[1][0] / K[0][0];
+    const double MU = 0.5 * (T + 1. / T - (K[2][0] / K[0][0]) * (K[2][0] / K[1][0]));
+    const double pz = K[0][1] / K[0][0];
+    const double px = ((K[1][1] / K[1][0]) + MU * pz) / (sqrt(1. - (MU * MU)));
+    const double m1 = K[0][1] / K[0][0], m2 = K[1][1] / K[1][0], m3 = K[2][1] / K[2][0];
+    const double k0 = bp.Cunits * K[0][0], k1 = bp.Cunits * K[1][0], k2 = bp.Cunits * K[2][0];
+    const double P0 = 1.0 / (k0 * k0), P1 = 1.0 / (k1 * k1), P2 = 1.0 / (k2 * k2);
+    double B = 2. * bp.ff * (P0 * P1 + P1 * P2 + P2 * P0);
+    biana = B * (1 + bp.beta * m1 * m1) * (1 + bp.beta * m2 * m2) * (1 + bp.beta * m3 * m3);
+    cal_coeff(px, pz, bs, nm);
+}
+
+// One contraction's contributions to its bin's 48 sums (barbie.c:343-363):
+// Sv the contraction's sum, Num its triangle count, biana its analytic
+// value; the zero rule (barbie.c:355-357) from the fields' nonzero flags.
+__device__ __forceinline__ void rec_accumulate(const BinRec &R, double Sv, double Num,
+                                               double biana, const double *bs, const double *nm,
+                                               const int *__restrict__ nzflag, int Nr,
+                                               double *abs_, double *anm, double *abi)
+{
+#pragma clang fp contract(off)
+    const int f1 = R.p1 * Nr + R.r1;   // p1 >= 0
+    const int f2 = (R.p2 < 0 ? -R.p2 : R.p2) * Nr + R.r2;
+    const int f3 = (R.p3 < 0 ? -R.p3 : R.p3) * Nr + R.r3;
+    // The sum is exactly zero iff one of its three fields is identically
+    // zero; the fields' zero flags decide it.
+    if (!nzflag[f1] || !nzflag[f2] || !nzflag[f3]) {
+        biana = 0.;
+        Num = 0.;
+    }
+    const double bn = biana * Num;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        abs_[k] = fma(Sv, bs[k], abs_[k]);
+        abi[k] = fma(bn, bs[k], abi[k]);
+        anm[k] = fma(Num, nm[k], anm[k]);
+    }
+}
+
+// Wave sums of the 48 accumulators, then the four waves' sums in wave order,
+// to binpart[0..47].
+__device__ __forceinline__ void bin_reduce_block(const double *abs_, const double *anm,
+                                                 const double *abi, double *__restrict__ binpart)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __shared__ double part[kBinThreads / 64][48];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const double v0 = wave_sum(abs_[k]), v1 = wave_sum(anm[k]), v2 = wave_sum(abi[k]);
+        if (lane == 0) {
+            part[wave][k] = v0;
+            part[wave][16 + k] = v1;
+            part[wave][32 + k] = v2;
+        }
+    }
+    __syncthreads();
+    if (tid < 48) {
+        double v = part[0][tid];
+        for (int w = 1; w < kBinThreads / 64; ++w) v += part[w][tid];
+        binpart[tid] = v;
+    }
+}
+
 
 __global__ __launch_bounds__(kBinThreads) void k_binning(const int32_t *__restrict__ sg,
                                                 const BinRec *__restrict__ recs,
@@ -84,7 +174,7 @@ __global__ __launch_bounds__(kBinThreads) void k_binning(const int32_t *__restri
     S += fb * st.part;
     nzflag += fb * st.nz;
     binpart += fb * st.binpart;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x;
     double abs_[16], anm[16], abi[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) abs_[k] = anm[k] = abi[k] = 0.0;
@@ -107,66 +197,67 @@ __global__ __launch_bounds__(kBinThreads) void k_binning(const int32_t *__restri
             Sn = 0.0;
             for (int sp = 0; sp < Rn.nsp; ++sp) Sn += S[(long long)Rn.pos + 256 * sp];
         }
-        const double a1 = avg[R.r1], a2 = avg[R.r2], a3 = avg[R.r3];
-        const double p1 = R.p1, p2 = R.p2, p3 = R.p3;
-        double K[3][2] = {{sqrt(a1 * a1 + p1 * p1), p1},
-                          {sqrt(a2 * a2 + p2 * p2), p2},
-                          {sqrt(a3 * a3 + p3 * p3), p3}};
-        // funcs.c:113-126: descending, strict '<'
-#pragma unroll
-        for (int i = 0; i < 2; i++)
-#pragma unroll
-            for (int j = 0; j < 2 - i; j++)
-                if (K[j][0] < K[j + 1][0]) {
-                    double s1 = K[j][0], s2 = K[j][1];
-                    K[j][0] = K[j + 1][0]; K[j][1] = K[j + 1][1];
-                    K[j + 1][0] = s1; K[j + 1][1] = s2;
-                }
-        const double T = K[1][0] / K[0][0];
-        const double MU = 0.5 * (T + 1. / T - (K[2][0] / K[0][0]) * (K[2][0] / K[1][0]));
-        const double pz = K[0][1] / K[0][0];
-        const double px = ((K[1][1] / K[1][0]) + MU * pz) / (sqrt(1. - (MU * MU)));
-        double Num = num[R.rp];
-        const double m1 = K[0][1] / K[0][0], m2 = K[1][1] / K[1][0], m3 = K[2][1] / K[2][0];
-        const double k0 = bp.Cunits * K[0][0], k1 = bp.Cunits * K[1][0], k2 = bp.Cunits * K[2][0];
-        const double P0 = 1.0 / (k0 * k0), P1 = 1.0 / (k1 * k1), P2 = 1.0 / (k2 * k2);
-        double Biana = 2. * bp.ff * (P0 * P1 + P1 * P2 + P2 * P0);
-        Biana = Biana * (1 + bp.beta * m1 * m1) * (1 + bp.beta * m2 * m2) * (1 + bp.beta * m3 * m3);
-        // barbie.c:355-357.  The sum is exactly zero iff one of its three
-        // fields is identically zero; the fields' zero flags decide it.
-        const int f1 = R.p1 * bp.Nr + R.r1;   // p1 >= 0
-        const int f2 = (R.p2 < 0 ? -R.p2 : R.p2) * bp.Nr + R.r2;
-        const int f3 = (R.p3 < 0 ? -R.p3 : R.p3) * bp.Nr + R.r3;
-        if (!nzflag[f1] || !nzflag[f2] || !nzflag[f3]) {
-            Biana = 0.;
-            Num = 0.;
-        }
-        double bs[16], nm[16];
-        cal_coeff(px, pz, bs, nm);
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            abs_[k] += Sv * bs[k];
-            abi[k] += Biana * Num * bs[k];
-            anm[k] += Num * nm[k];
-        }
+        double bs[16], nm[16], biana;
+        rec_coeffs(R, avg, bp, bs, nm, biana);
+        rec_accumulate(R, Sv, num[R.rp], biana, bs, nm, nzflag, bp.Nr, abs_, anm, abi);
     }
-    // Wave sums, then the four waves' sums in wave order.
-    __shared__ double part[kBinThreads / 64][48];
+    bin_reduce_block(abs_, anm, abi, binpart + (long long)sgi * 48);
+}
+
+// The field-independent coefficients of every record, computed once (the
+// same code as k_binning): table [33][nrec], rows bs[0..15], nm[0..15], biana.
+__global__ __launch_bounds__(256) void k_bin_coeffs(const BinRec *__restrict__ recs, int nrec,
+                                                    const double *__restrict__ avg, BinParams bp,
+                                                    double *__restrict__ W)
+{
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= nrec) return;
+    double bs[16], nm[16], biana;
+    rec_coeffs(recs[c], avg, bp, bs, nm, biana);
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-        const double v0 = wave_sum(abs_[k]), v1 = wave_sum(anm[k]), v2 = wave_sum(abi[k]);
-        if (lane == 0) {
-            part[wave][k] = v0;
-            part[wave][16 + k] = v1;
-            part[wave][32 + k] = v2;
+        W[(long long)k * nrec + c] = bs[k];
+        W[(long long)(16 + k) * nrec + c] = nm[k];
+    }
+    W[32LL * nrec + c] = biana;
+}
+
+// k_binning with the coefficients read from the table instead of computed:
+// the same sums in the same order, so the same bits.  Workgroups of one
+// segment for consecutive fields are adjacent (blockIdx.x the field), so the
+// segment's coefficients are read from the caches by all but the first.
+__global__ __launch_bounds__(kBinThreads) void k_binning_w(const int32_t *__restrict__ sg,
+                                                  const BinRec *__restrict__ recs, int nrec,
+                                                  const double *__restrict__ W,
+                                                  const double *__restrict__ S,
+                                                  const double *__restrict__ num,
+                                                  const int *__restrict__ nzflag, BinParams bp,
+                                                  double *__restrict__ binpart, BatchStrides st)
+{
+    const int fb = blockIdx.x;    // field of the batch
+    const int sgi = blockIdx.y;   // segment
+    S += fb * st.part;
+    nzflag += fb * st.nz;
+    binpart += fb * st.binpart;
+    const int tid = threadIdx.x;
+    double abs_[16], anm[16], abi[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) abs_[k] = anm[k] = abi[k] = 0.0;
+    const int end = sg[sgi + 1];
+    for (int c = sg[sgi] + tid; c < end; c += kBinThreads) {
+        const BinRec R = recs[c];
+        double Sv = 0.0;
+        for (int sp = 0; sp < R.nsp; ++sp) Sv += S[(long long)R.pos + 256 * sp];
+        double bs[16], nm[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            bs[k] = W[(long long)k * nrec + c];
+            nm[k] = W[(long long)(16 + k) * nrec + c];
         }
+        rec_accumulate(R, Sv, num[R.rp], W[32LL * nrec + c], bs, nm, nzflag, bp.Nr, abs_, anm,
+                       abi);
     }
-    __syncthreads();
-    if (tid < 48) {
-        double v = part[0][tid];
-        for (int w = 1; w < kBinThreads / 64; ++w) v += part[w][tid];
-        binpart[(long long)sgi * 48 + tid] = v;
-    }
+    bin_reduce_block(abs_, anm, abi, binpart + (long long)sgi * 48);
 }
 
 // Bin sums: thread (bin, t < 48) adds the bin's segment sums in order;
@@ -189,15 +280,30 @@ __global__ __launch_bounds__(256) void k_bin_reduce(const int32_t *__restrict__ 
     if (o) o[fb * st.bins + (long long)b * 16 + (t & 15)] = v;
 }
 
+hipError_t launch_bin_coeffs(const BinRec *recs, int nrec, const double *avg, const BinParams &bp,
+                             double *W, hipStream_t s)
+{
+    if (nrec <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bin_coeffs, dim3((nrec + 255) / 256), dim3(256), 0, s, recs, nrec, avg,
+                       bp, W);
+    return hipGetLastError();
+}
+
 hipError_t launch_binning(const int32_t *sg, int nseg, const int32_t *bin_sg, const BinRec *recs,
                           const double *S, const double *num, const int *nzflag, const double *avg,
                           const BinParams &bp, double *binpart, double *bins_bs,
                           double *bins_norm, double *bins_biana, const BatchStrides &st, int nf,
-                          hipStream_t s)
+                          hipStream_t s, const double *W, int nrec)
 {
     if (bp.nbins <= 0) return hipSuccess;
     if (nf < 1 || nf > 65535) return hipErrorInvalidValue;
-    if (nseg > 0) {
+    if (nseg > 0 && W) {
+        if (nseg > 65535) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_binning_w, dim3(nf, nseg), dim3(kBinThreads), 0, s, sg, recs, nrec, W,
+                           S, num, nzflag, bp, binpart, st);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    } else if (nseg > 0) {
         hipLaunchKernelGGL(k_binning, dim3(nseg, nf), dim3(kBinThreads), 0, s, sg, recs, S, num,
                            nzflag, avg, bp, binpart, st);
         hipError_t e = hipGetLastError();

@@ -229,7 +229,14 @@ hipError_t launch_binning(const int32_t *sg, int nseg, const int32_t *bin_sg, co
                           const double *S, const double *num, const int *nzflag, const double *avg,
                           const BinParams &bp, double *binpart, double *bins_bs,
                           double *bins_norm, double *bins_biana, const BatchStrides &bs, int nf,
-                          hipStream_t s);
+                          hipStream_t s, const double *W = nullptr, int nrec = 0);
+// The records' field-independent binning coefficients, [33][nrec] doubles
+// (k_binning's own computation): with W, launch_binning reads them instead
+// of computing them for every field, with the same bits.
+hipError_t launch_bin_coeffs(const BinRec *recs, int nrec, const double *avg, const BinParams &bp,
+                             double *W, hipStream_t s);
+// Largest coefficient table an estimator keeps (batches of more than one field).
+constexpr size_t kBinTableBytes = 256ull << 20;
 
 // v of the lane that the DPP control CTRL selects (within a row of 16
 // lanes), for a double: both halves moved.

@@ -1009,6 +1009,7 @@ struct bb_estimator {
     BinRec *d_recs = nullptr;
     int32_t *d_sg = nullptr, *d_bin_sg = nullptr;
     double *d_binpart = nullptr;      // [segment][48] per field
+    double *d_bincoef = nullptr;      // [33][records] binning coefficients (batches), or null
     double *d_num = nullptr;
     std::vector<double> num_host;
     int *d_nz = nullptr;              // [Nmax * Nr]
@@ -1472,7 +1473,15 @@ int import_rings(bb_estimator *E, const std::string &dir, int NRi, double rel_to
     return BB_OK;
 }
 
-// Device bytes held by an estimator of layout L (steady state).
+// Estimators of more than one field per batch keep the records' binning
+// coefficients in a table when it is at most kBinTableBytes: every field of
+// a batch then reads them instead of computing them again.
+size_t bin_table_bytes(const Layout &L, size_t F)
+{
+    const size_t b = L.recs.size() * 33 * sizeof(double);
+    return (F > 1 && b <= kBinTableBytes && L.recs.size() <= 0x7fffffff) ? b : 0;
+}
+
 // Device bytes held by an estimator of layout L with F fields per batch:
 // per-field buffers times F, shared work lists once, and the staging of one
 // host field (bb_estimate; bb_estimate_batch grows it to the batch).
@@ -1506,6 +1515,7 @@ size_t layout_bytes(const PlanData &d, const Layout &L, size_t F)
          L.units.size() * sizeof(WorkUnit);
     b += L.recs.size() * sizeof(BinRec) + (L.sg.size() + L.bin_sg.size()) * 4;
     b += (L.sg.size() - 1) * 48 * 8 * (size_t)F;
+    b += bin_table_bytes(L, F);
     b += d.tri.size() / 3 * 8 + Nr * 8;
     return b;
 }
@@ -1619,6 +1629,11 @@ int estimator_init(bb_estimator *E)
     if ((rc = upload(E->d_bin_sg, L.bin_sg.data(), sizeof(int32_t) * L.bin_sg.size(), E->stream)))
         return rc;
     if ((rc = upload(E->d_avg, d.avg.data(), sizeof(double) * Nr, E->stream))) return rc;
+    if (bin_table_bytes(L, F) > 0) {
+        if ((rc = dev_alloc(&E->d_bincoef, L.recs.size() * 33, E->owned))) return rc;
+        HIPCHK(launch_bin_coeffs(E->d_recs, (int)L.recs.size(), E->d_avg, L.bp, E->d_bincoef,
+                                 E->stream));
+    }
     HIPCHK(hipMemsetAsync(E->d_part, 0, sizeof(double) * nslots, E->stream));
     // Padding points between fields are read by the contraction and never
     // written by the transforms: zero them (and everything else) once.
@@ -1660,7 +1675,7 @@ int enqueue_kernels(bb_estimator *E, int nf, const double *d_fields, double *d_b
     if (timed) HIPCHK(hipEventRecord(E->ev[3], s));
     HIPCHK(launch_binning(E->d_sg, (int)L.sg.size() - 1, E->d_bin_sg, E->d_recs, E->d_part,
                           E->d_num, E->d_nz, E->d_avg, L.bp, E->d_binpart, d_bs, d_norm, d_biana,
-                          E->bst, nf, s));
+                          E->bst, nf, s, E->d_bincoef, (int)L.recs.size()));
     return BB_OK;
 }
 
@@ -1681,7 +1696,7 @@ int enqueue_post(bb_estimator *E, int nf, double *d_bs, double *d_norm, double *
     const Layout &L = *E->L;
     HIPCHK(launch_binning(E->d_sg, (int)L.sg.size() - 1, E->d_bin_sg, E->d_recs, E->d_part,
                           E->d_num, E->d_nz, E->d_avg, L.bp, E->d_binpart, d_bs, d_norm, d_biana,
-                          E->bst, nf, E->stream));
+                          E->bst, nf, E->stream, E->d_bincoef, (int)L.recs.size()));
     return BB_OK;
 }
 

@@ -154,8 +154,9 @@ def accuracy_leg(B, torch, plan, g, precision):
 
 
 def batched_leg(B, torch, precision, nfields=64, distinct=8, reps=3):
-    """64 realizations per NN as batched launches; fields/hour batched and
-    one at a time on the same estimator."""
+    """64 realizations per NN as batched launches, with the layout bb_tune
+    picks for the geometry (timed on one of the fields); fields/hour batched
+    and one at a time on the same estimator."""
     res = {}
     for NN in (128, 192, 256):
         g = geometry(B, NN)
@@ -165,7 +166,9 @@ def batched_leg(B, torch, precision, nfields=64, distinct=8, reps=3):
         base = torch.stack([B.make_field_device(NN, NRi=k + 1) for k in range(distinct)])
         fields = base.repeat(nfields // distinct, 1, 1, 1).contiguous()
         del base
-        est = B.Estimator(plan, precision, max_batch=nfields)
+        torch.cuda.synchronize()
+        opts, _ = plan.tune(precision, d_field=fields.data_ptr())
+        est = B.Estimator(plan, precision, max_batch=nfields, **opts)
         out = torch.zeros(3, nfields, nb, dtype=torch.float64, device="cuda")
         torch.cuda.synchronize()   # the estimator runs on its own stream
         ptrs = (fields.data_ptr(), out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr())
@@ -184,7 +187,7 @@ def batched_leg(B, torch, precision, nfields=64, distinct=8, reps=3):
         t1 = time.perf_counter() - t0
         res[str(NN)] = {"fields_per_hour_batched": nfields * 3600.0 / tb,
                         "fields_per_hour_one_at_a_time": nfields * 3600.0 / t1,
-                        "batch": nfields, "ms_per_batch": tb * 1e3,
+                        "batch": nfields, "ms_per_batch": tb * 1e3, "layout": opts,
                         "ncontr": plan.stats["ncontr"],
                         "realizations": f"{distinct} seeded fields x {nfields // distinct} "
                                         "copies (cost does not depend on the values)"}
@@ -207,7 +210,8 @@ def mixed_leg(B, torch, counts=(22, 21, 21), reps=3):
     for NN, nf in zip(sizes, counts):
         g = geometry(B, NN)
         plan = B.Plan(g)
-        ests.append(B.Estimator(plan, "bf16x3", max_batch=nf, waves=4, occupancy=3))
+        ests.append(B.Estimator(plan, "bf16x3", max_batch=nf, waves=4, occupancy=3,
+                                grid_levels=4))
         f = torch.empty((nf, NN, NN, NN + 2), dtype=torch.float64, device="cuda")
         for k in range(nf):
             B.generate_field(NN, B.seed_for(k + 1), out=f[k].data_ptr())
@@ -238,7 +242,8 @@ def mixed_leg(B, torch, counts=(22, 21, 21), reps=3):
     same = all(bool(torch.equal(x[0], x[1])) for x in outs)
     for e in ests:
         e.close()
-    return {"fields": dict(zip([str(n) for n in sizes], counts)), "layout": "waves=4,occupancy=3",
+    return {"fields": dict(zip([str(n) for n in sizes], counts)),
+            "layout": "grid_levels=4,waves=4,occupancy=3 (one shape for all sizes)",
             "separate_per_NN_batches": res["separate"], "mixed_one_launch": res["mixed"],
             "speedup": res["separate"]["ms_per_call"] / res["mixed"]["ms_per_call"],
             "bins_identical": same}
