@@ -441,8 +441,17 @@ int pack_items(std::vector<PairBlock> &blocks, const Level &L, bool split_b, int
 // that units carry similar work (or into x_splits ranges when given), and
 // every tile gets one partial tile per x-range.  Rewrites the tile indices
 // of the windows and of pos into partial-tile positions; units are ordered
-// largest first.
+// largest first.  The units of one launch: about kTargetUnits workgroups per
+// batch of fields, so an estimator of F fields per batch splits each field's
+// work into kTargetUnits / F units (a batch fills the chip by its fields, and
+// every x-split costs a partial tile written by the contraction and read by
+// the binning).  The split depends on the estimator's batch size, never on
+// the size of the batch it runs: a field gets the same bits in every batch.
 constexpr long long kTargetUnits = 2048;
+long long target_units(int max_batch)
+{
+    return std::max<long long>(1, kTargetUnits / std::max(1, max_batch));
+}
 // Largest cost of one unit (tile windows of the busiest wave x chunks).  Long
 // workgroups delay the kernels of other streams that wait for a free CU
 // (the request server runs several estimates at once), so large estimates get
@@ -451,7 +460,7 @@ constexpr long long kMaxUnitCost = 1024;
 
 int make_units(WorkLists &W, int chunk, int x_splits, std::vector<WorkUnit> &units,
                std::vector<int32_t> &pos, std::vector<int16_t> &nsp, long long &part_tiles,
-               long long *total_cost)
+               long long *total_cost, long long target = kTargetUnits)
 {
     const size_t n = W.items.size();
     std::vector<int> nch(n), sp(n);
@@ -462,7 +471,7 @@ int make_units(WorkLists &W, int chunk, int x_splits, std::vector<WorkUnit> &uni
     }
     if (total_cost) *total_cost = total;
     const long long unit_cost =
-        std::max<long long>(1, std::min(total / kTargetUnits, kMaxUnitCost));
+        std::max<long long>(1, std::min(total / std::max<long long>(1, target), kMaxUnitCost));
     for (size_t i = 0; i < n; ++i) {
         const long long cost = (long long)W.item_work[i] * nch[i];
         int s = x_splits > 0 ? x_splits : (int)std::min<long long>(256, (cost + unit_cost - 1) / unit_cost);
@@ -887,7 +896,7 @@ int build_layout(const PlanData &d, CPrec prec, const bb_options &opt, int waves
     for (size_t c = 0; c < d.contr.size(); ++c) L.useful_pts += L.lv[lvl[c]].npts;
     std::vector<WorkUnit> units;
     int rc = make_units(W, L.chunk, opt.x_splits, units, L.pos, L.nsp, L.part_tiles,
-                        &L.total_cost);
+                        &L.total_cost, target_units(opt.max_batch));
     if (rc) return rc;
     for (const WorkUnit &u : units) (W.kind[u.item] ? L.units_f : L.units).push_back(u);
     L.items.swap(W.items);
@@ -949,7 +958,8 @@ int get_layout(const bb_plan *p, CPrec prec, const bb_options &opt,
                             "/" + std::to_string(opt.x_splits) + "/" + std::to_string(lev) + "/" +
                             std::to_string(waves) + "/" + std::to_string(tww) + "/" +
                             std::to_string(prec == CPrec::BF16X3 ? opt.mix_fp32 : 0) + "/" +
-                            std::to_string(layout_occ3(prec, opt));
+                            std::to_string(layout_occ3(prec, opt)) + "/" +
+                            std::to_string(target_units(opt.max_batch));
     bb_plan *mp = const_cast<bb_plan *>(p);
     {
         std::lock_guard<std::mutex> g(mp->mu);

@@ -73,7 +73,12 @@ typedef struct bb_options {
                        summed points over all contractions. */
     int max_batch;  /* fields one launch sequence processes together (0 or 1:
                        one).  Device buffers per field are held max_batch
-                       times; the work lists are shared. */
+                       times; the work lists are shared.  With x_splits 0 the
+                       contraction work of a field is cut into about
+                       2048 / max_batch x-ranges, so a field's bins are the
+                       same bits in every batch of one estimator, and agree
+                       with those of an estimator of another max_batch to
+                       rounding (FP64: ~1e-15; split paths: ~1e-7). */
     int waves;      /* waves per contraction workgroup: 4 or 8 (0: 8, or the
                        environment variable BARBIE_CONTRACT_WAVES) */
     int mix_fp32;   /* BF16X3 only: percent (0-90) of the contraction's cost given

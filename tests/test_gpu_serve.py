@@ -1,6 +1,8 @@
 """Request server on the GPU: a field's bins are the same bits alone and in a
-mixed, batched, concurrent workload; admission under a small HBM budget
-evicts idle lanes and refuses a request that cannot fit."""
+mixed, batched, concurrent workload (the server's lanes hold estimators of
+its max_batch, so the references are estimators of that batch size);
+admission under a small HBM budget evicts idle lanes and refuses a request
+that cannot fit."""
 import numpy as np
 import pytest
 
@@ -10,16 +12,16 @@ import barbie_amd as B
 pytestmark = pytest.mark.gpu
 
 
-def alone(gf, NRi, prec):
+def alone(gf, NRi, prec, max_batch=1):
     g = gf()
-    return B.Estimator(B.Plan(g), prec).estimate(B.make_field(g.NN, NRi=NRi))
+    return B.Estimator(B.Plan(g), prec, max_batch=max_batch).estimate(B.make_field(g.NN, NRi=NRi))
 
 
 @pytest.mark.parametrize("prec", ["bf16x3", "fp64"])
 def test_results_same_alone_and_in_the_mix(prec):
     cases = [(medium, 1), (config1, 2), (small, 3), (medium, 4), (config1, 5), (medium, 6),
              (small, 7), (config1, 8), (medium, 9)]
-    ref = {c: alone(c[0], c[1], prec) for c in cases}
+    ref = {c: alone(c[0], c[1], prec, max_batch=3) for c in cases}
     srv = B.Server(prec, max_inflight=3, max_batch=3)
     # a request alone
     t = srv.submit(medium(), B.make_field(medium().NN, NRi=1))
@@ -48,7 +50,7 @@ def test_device_resident_fields():
     torch = pytest.importorskip("torch")
     g = config1()
     f = B.make_field(g.NN, NRi=3)
-    ref = alone(config1, 3, "bf16x3")
+    ref = alone(config1, 3, "bf16x3", max_batch=4)   # the server's default batch
     d = torch.from_numpy(f).to("cuda")
     torch.cuda.synchronize()
     srv = B.Server("bf16x3")
