@@ -95,7 +95,8 @@ struct Lane {
     int slot = -1;
     double *d_in = nullptr;     // batch fields
     double *d_out = nullptr;    // bs | norm | biana, each batch * nb
-    double *h_out = nullptr;    // pinned copy of d_out
+    double *h_out = nullptr;    // pinned copy of d_out (host-coherent)
+    double *h_out_dev = nullptr;   // h_out's device address
     size_t bytes = 0;
     std::vector<Req *> reqs;    // in flight
     bool busy = false;
@@ -137,6 +138,36 @@ struct bb_server {
 };
 
 namespace {
+
+// Copies a batch's outputs (three arrays of n_nb values, `stride` apart) into
+// the lane's host-coherent pinned buffer with stores from the lane's own
+// stream.  A copy-engine request (hipMemcpyAsync to the host) would wait in
+// the engine's queue for this lane's kernels, and the requests of other lanes
+// queued behind it on the same engine would wait too: a short request's
+// output would come back only when a long one's does.
+__global__ void k_store_out(const double *__restrict__ d, double *__restrict__ h, size_t n_nb,
+                            size_t stride)
+{
+    // System-scope stores write through to the host memory; the kernel's
+    // completion, which the scheduler sees before it reads h, orders them.
+    // (A system-scope release fence instead would write back every dirty L2
+    // line, those of the kernels running beside this one included.)
+    const size_t step = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < 3 * n_nb; i += step) {
+        const size_t a = i / n_nb, r = i - a * n_nb;
+        __hip_atomic_store(h + a * stride + r, d[a * stride + r], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+hipError_t launch_store_out(const double *d, double *h, size_t n_nb, size_t stride,
+                            hipStream_t s)
+{
+    const size_t blocks = std::min<size_t>((3 * n_nb + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_store_out, dim3((unsigned)std::max<size_t>(blocks, 1)), dim3(256), 0, s,
+                       d, h, n_nb, stride);
+    return hipGetLastError();
+}
 
 int lane_bytes(Model *m, const bb_server_options &o, size_t *bytes)
 {
@@ -199,8 +230,10 @@ int bb_server::make_lane(Model *m, Lane **out)
         rc = set_error(BB_E_DEVMEM, "lane input buffer");
     if (!rc && hipMalloc(&L->d_out, 3 * B * m->nb * 8) != hipSuccess)
         rc = set_error(BB_E_DEVMEM, "lane output buffer");
-    if (!rc && hipHostMalloc(&L->h_out, 3 * B * m->nb * 8, hipHostMallocDefault) != hipSuccess)
+    if (!rc && hipHostMalloc(&L->h_out, 3 * B * m->nb * 8, hipHostMallocCoherent) != hipSuccess)
         rc = set_error(BB_E_NOMEM, "lane pinned output buffer");
+    if (!rc && hipHostGetDevicePointer((void **)&L->h_out_dev, L->h_out, 0) != hipSuccess)
+        rc = set_error(BB_E_DEVICE, "lane pinned output buffer: no device address");
     L->bytes = need;
     used += need;
     if (rc) {
@@ -370,8 +403,8 @@ bool bb_server::dispatch()
         if (!rc)
             rc = bb_estimate_batch_async(lane->est, (int)n, lane->d_in, lane->d_out,
                                          lane->d_out + B * nb, lane->d_out + 2 * B * nb);
-        if (!rc && hipMemcpyAsync(lane->h_out, lane->d_out, 3 * B * nb * 8,
-                                  hipMemcpyDeviceToHost, lane->stream) != hipSuccess)
+        if (!rc && launch_store_out(lane->d_out, lane->h_out_dev, n * nb, B * nb,
+                                    lane->stream) != hipSuccess)
             rc = set_error(BB_E_DEVICE, "output copy failed");
         if (rc) {
             if (lane->stream) (void)hipStreamSynchronize(lane->stream);
