@@ -46,15 +46,14 @@ struct FieldTask {
 struct LevelDesc {
     FftDesc fd;
     int tw_off;          // first twiddle of the level in the twiddle table
-    int rpb, cpb;        // transforms per block of the row and column passes (2^k)
-    int rpb_log2, cpb_log2;
+    int cpb, cpb_log2;   // columns per block of the ring pass (2^k)
     int max_rows;        // widest row band of the level's fields
     long long off;       // first point of the level in the cache
     long long stride;    // points between fields (multiple of 8)
 };
-// Transforms per block of the row and column passes for length N.
-int fft_rows_per_block(int N);
-int fft_cols_per_block(int N);
+// Columns per block of the ring pass for length N and values of `elem`
+// bytes (8: FP64 caches, 4: FP32 caches).
+int fft_cols_per_block(int N, int elem);
 
 // ---------------------------------------------------------------------------
 // Grid levels.  The sum over x of a triple product of fields band-limited to
@@ -166,17 +165,16 @@ struct BatchStrides {
     long long binpart = 0;  // per-segment bin sums (doubles)
 };
 
-// Blocks of the row and column passes: {task, batch index} pairs in device
-// memory, built by fft_blocks on the host.
+// Work units of the ring pass: {task, first | (end << 16)} column-block
+// ranges in device memory, built by fft_blocks on the host.
 struct FftBlocks {
-    const int2 *row = nullptr;
-    const int2 *col = nullptr;
-    int nrow = 0, ncol = 0;
+    const int2 *units = nullptr;
+    int n = 0;
 };
-void fft_blocks(const FieldTask *tasks, int ntasks, const LevelDesc *h_lv, std::vector<int2> &row,
-                std::vector<int2> &col);
-// Row and column passes of every task (fields of all levels), writing the
-// FP64 cache, or the FP32 cache plus (split != null) the bf16 operands.
+void fft_blocks(const FieldTask *tasks, int ntasks, const LevelDesc *h_lv,
+                std::vector<int2> &units);
+// The ring pass over every task (fields of all levels), writing the FP64
+// cache, or the FP32 cache plus (split != null) the bf16 operands.
 // planes: NP x NP per plane (signed modes wrapped), NP >= every level's N;
 // d_lv / h_lv: the same LevelDesc table in device and host memory.
 hipError_t launch_fft_f64(const double2 *planes, int NP, double2 *cache, const FieldTask *tasks,

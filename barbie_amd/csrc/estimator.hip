@@ -989,7 +989,7 @@ struct bb_estimator {
     std::vector<LevelDesc> lvd;       // FFT geometry per level (host copy)
     LevelDesc *d_lvd = nullptr;
     double2 *d_tw = nullptr;          // twiddles of every level
-    FftBlocks fb;                     // blocks of the row and column passes
+    FftBlocks fb;                     // work units of the ring pass
     std::vector<void *> owned;
     int batch = 1;                    // fields per launch held in device memory
     BatchStrides bst;                 // per-field buffer strides
@@ -1072,16 +1072,14 @@ std::vector<double2> twiddles(int NG)
 
 // FFT geometry of a level: radices, twiddle offset, batch sizes, cache
 // placement.
-int level_desc(const Level &V, int tw_off, LevelDesc &D)
+int level_desc(const Level &V, int tw_off, int elem, LevelDesc &D)
 {
     std::memset(&D, 0, sizeof D);
     if (!make_fft_desc(V.NG, D.fd)) return set_error(BB_E_INVALID, "unsupported FFT length");
     D.tw_off = tw_off;
-    D.rpb = fft_rows_per_block(V.NG);
-    D.cpb = fft_cols_per_block(V.NG);
+    D.cpb = fft_cols_per_block(V.NG, elem);
     while ((1 << D.cpb_log2) < D.cpb) ++D.cpb_log2;
-    while ((1 << D.rpb_log2) < D.rpb) ++D.rpb_log2;
-    if ((1 << D.cpb_log2) != D.cpb || (1 << D.rpb_log2) != D.rpb || (D.cpb * V.NG) % 8 != 0)
+    if ((1 << D.cpb_log2) != D.cpb || (D.cpb * V.NG) % 8 != 0)
         return set_error(BB_E_INVALID, "internal: FFT batch sizes");
     D.max_rows = V.max_rows;
     D.off = V.off;
@@ -1159,16 +1157,15 @@ int compute_num(bb_estimator *E)
     L.tws.swap(W.tws);
     if ((rc = check_lists(L, rcount))) return rc;
     LevelDesc ud;
-    if ((rc = level_desc(U, 0, ud))) return rc;
+    if ((rc = level_desc(U, 0, 8, ud))) return rc;
     const std::vector<double2> tw = twiddles(NG);
     const long long uslots = std::max<long long>(L.part_tiles, 1) * 256;
     std::vector<void *> tmp;
     FieldTask *d_ut;
     LevelDesc *d_ud;
     int2 *d_ublk;
-    std::vector<int2> urow, ucol;
-    fft_blocks(U.tasks.data(), Nr, &ud, urow, ucol);
-    urow.insert(urow.end(), ucol.begin(), ucol.end());
+    std::vector<int2> ublk;
+    fft_blocks(U.tasks.data(), Nr, &ud, ublk);
     double2 *d_unit, *d_utw;
     WorkItem *d_uitems;
     WorkUnit *d_uunits;
@@ -1176,8 +1173,8 @@ int compute_num(bb_estimator *E)
     double *d_upart;
     rc = dev_alloc(&d_ut, Nr, tmp);
     if (!rc) rc = dev_alloc(&d_ud, 1, tmp);
-    if (!rc) rc = dev_alloc(&d_ublk, urow.size(), tmp);
-    if (!rc) rc = upload(d_ublk, urow.data(), sizeof(int2) * urow.size(), E->stream);
+    if (!rc) rc = dev_alloc(&d_ublk, std::max<size_t>(ublk.size(), 1), tmp);
+    if (!rc && !ublk.empty()) rc = upload(d_ublk, ublk.data(), sizeof(int2) * ublk.size(), E->stream);
     if (!rc) rc = dev_alloc(&d_utw, NG, tmp);
     if (!rc) rc = dev_alloc(&d_unit, (size_t)L.cache_pts, tmp);
     if (!rc && hipMemsetAsync(d_unit, 0, sizeof(double2) * L.cache_pts, E->stream) != hipSuccess)
@@ -1197,10 +1194,8 @@ int compute_num(bb_estimator *E)
         rc = upload(d_utws, L.tws.data(), sizeof(TWin) * L.tws.size(), E->stream);
     if (!rc) {
         FftBlocks ub;
-        ub.row = d_ublk;
-        ub.nrow = (int)(urow.size() - ucol.size());
-        ub.col = d_ublk + ub.nrow;
-        ub.ncol = (int)ucol.size();
+        ub.units = d_ublk;
+        ub.n = (int)ublk.size();
         const BatchStrides one;
         hipError_t e = launch_fft_f64(nullptr, NG, d_unit, d_ut, ub, d_ud, &ud, 1, d_utw, nullptr,
                                       one, 1, E->stream);
@@ -1278,7 +1273,7 @@ int visit_nn_fields(bb_estimator *E, const double *field_r2c, int stride, Fn fn)
     V.npts = (long long)NN * NN;
     V.stride = (V.npts + 31) / 32 * 32;
     LevelDesc D;
-    if (int rc = level_desc(V, 0, D)) return rc;
+    if (int rc = level_desc(V, 0, 8, D)) return rc;
     std::vector<int> sel;
     for (int f = 0; f < Nmax * Nr; f += std::max(1, stride)) sel.push_back(f);
     const int nsel = (int)sel.size();
@@ -1290,7 +1285,7 @@ int visit_nn_fields(bb_estimator *E, const double *field_r2c, int stride, Fn fn)
     FieldTask *d_tasks;
     LevelDesc *d_ld;
     int2 *d_blk;
-    const size_t maxblk = (size_t)nb * (NN / std::max(1, std::min(D.rpb, D.cpb)) + 2) * 2;
+    const size_t maxblk = (size_t)nb * (NN / D.cpb + 2);
     int rc = dev_alloc(&d_field, fsz, tmp);
     if (!rc) rc = dev_alloc(&d_planes, (size_t)Nmax * NN * NN, tmp);
     if (!rc) rc = dev_alloc(&d_cache, (size_t)nb * V.stride, tmp);
@@ -1316,22 +1311,18 @@ int visit_nn_fields(bb_estimator *E, const double *field_r2c, int stride, Fn fn)
             const int band = d.nhi[r] > 0 ? std::min(NN / 2, K[r]) : -1;
             tasks.push_back(FieldTask{p, r, k, band, d.nlo[r], d.nhi[r], 0, 0, 3});
         }
-        std::vector<int2> brow, bcol;
-        fft_blocks(tasks.data(), n, &D, brow, bcol);
-        if (brow.size() + bcol.size() > maxblk) {
+        std::vector<int2> blk;
+        fft_blocks(tasks.data(), n, &D, blk);
+        if (blk.size() > maxblk) {
             rc = set_error(BB_E_INVALID, "internal: field block list");
             break;
         }
-        const size_t nrow = brow.size();
-        brow.insert(brow.end(), bcol.begin(), bcol.end());
         rc = upload(d_tasks, tasks.data(), sizeof(FieldTask) * n, s);
-        if (!rc && !brow.empty()) rc = upload(d_blk, brow.data(), sizeof(int2) * brow.size(), s);
+        if (!rc && !blk.empty()) rc = upload(d_blk, blk.data(), sizeof(int2) * blk.size(), s);
         if (!rc) {
             FftBlocks fb;
-            fb.row = d_blk;
-            fb.nrow = (int)nrow;
-            fb.col = d_blk + nrow;
-            fb.ncol = (int)bcol.size();
+            fb.units = d_blk;
+            fb.n = (int)blk.size();
             const BatchStrides one;
             hipError_t e = launch_fft_f64(d_planes, NN, d_cache, d_tasks, fb, d_ld, &D, 1, d_tw,
                                           nullptr, one, 1, s);
@@ -1515,9 +1506,8 @@ size_t layout_bytes(const PlanData &d, const Layout &L, size_t F)
     for (const Level &V : L.lv) {
         tw += V.NG;
         nt += V.tasks.size();
-        const int rpb = fft_rows_per_block(V.NG), cpb = fft_cols_per_block(V.NG);
-        for (const FieldTask &T : V.tasks)
-            nblk += (std::min(V.NG, 2 * T.rowband + 1) + rpb - 1) / rpb + (V.NG + cpb - 1) / cpb;
+        const int cpb = fft_cols_per_block(V.NG, L.prec == CPrec::F64 ? 8 : 4);
+        nblk += V.tasks.size() * (size_t)((V.NG + cpb - 1) / cpb);   // at most one unit per block
     }
     b += nblk * sizeof(int2);
     b += tw * 16 + nt * sizeof(FieldTask) + L.lv.size() * sizeof(LevelDesc);
@@ -1557,23 +1547,21 @@ int estimator_init(bb_estimator *E)
     std::vector<double2> tw_all;
     for (int l = 0; l < nl; ++l) {
         const Level &V = L.lv[l];
-        if ((rc = level_desc(V, (int)tw_all.size(), E->lvd[l]))) return rc;
+        if ((rc = level_desc(V, (int)tw_all.size(), E->cprec == CPrec::F64 ? 8 : 4, E->lvd[l])))
+            return rc;
         const std::vector<double2> tw = twiddles(V.NG);
         tw_all.insert(tw_all.end(), tw.begin(), tw.end());
         tasks.insert(tasks.end(), V.tasks.begin(), V.tasks.end());
     }
     {
-        std::vector<int2> row, col;
-        fft_blocks(tasks.data(), (int)tasks.size(), E->lvd.data(), row, col);
+        std::vector<int2> blk;
+        fft_blocks(tasks.data(), (int)tasks.size(), E->lvd.data(), blk);
         int2 *d;
-        row.insert(row.end(), col.begin(), col.end());
-        if ((rc = dev_alloc(&d, row.size(), E->owned))) return rc;
-        if (!row.empty() && (rc = upload(d, row.data(), sizeof(int2) * row.size(), E->stream)))
+        if ((rc = dev_alloc(&d, std::max<size_t>(blk.size(), 1), E->owned))) return rc;
+        if (!blk.empty() && (rc = upload(d, blk.data(), sizeof(int2) * blk.size(), E->stream)))
             return rc;
-        E->fb.row = d;
-        E->fb.nrow = (int)(row.size() - col.size());
-        E->fb.col = d + E->fb.nrow;
-        E->fb.ncol = (int)col.size();
+        E->fb.units = d;
+        E->fb.n = (int)blk.size();
     }
     if ((rc = dev_alloc(&E->d_tw, tw_all.size(), E->owned))) return rc;
     if ((rc = upload(E->d_tw, tw_all.data(), sizeof(double2) * tw_all.size(), E->stream))) return rc;

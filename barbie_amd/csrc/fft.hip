@@ -1,21 +1,18 @@
-// Plane gather and batched ring-masked 2-D inverse FFTs (the plane-ring
-// cache).  Replaces Df / wrapid_fast (funcs.c:88-109, 537-566) and the ring
-// IFFTs of calCR (funcs.c:367-382).
+// Plane gather and the ring pass: batched ring-masked 2-D inverse FFTs (the
+// plane-ring cache).  Replaces Df / wrapid_fast (funcs.c:88-109, 537-566)
+// and the ring IFFTs of calCR (funcs.c:367-382).
 //
 // D_{p,r}(x) = sum_{k in ring r} Delta(kx, ky, kz=p) exp(+2 pi i k.x / NN),
 // unnormalised (FFTW_BACKWARD), with the DC mode of plane 0 zeroed.
 //
-// Two passes per field, each a batch of 1-D Stockham FFTs in LDS (FP64 for
-// FP64 caches, FP32 for FP32 caches), one launch each for the fields of
-// every grid level:
-//  - row pass: for each row i that can hold masked modes, gather the plane
-//    row, apply the ring mask (integer radius bounds), transform along j,
-//    write the row into the field's cache slot;
-//  - column pass: transform along i in place (rows outside the ring's band
-//    are zero), write the field to the cache and, for FP32 caches, as bf16
-//    hi/lo operands, each in the forms the contraction reads it in; one
-//    round of workgroups loops over the column groups, loading the next
-//    group during the current one's transform.
+// One launch builds the fields of every grid level (FP64 for FP64 caches,
+// FP32 for the FP32 and BF16x3 caches).  A workgroup takes a range of one
+// field's column blocks: the transform along j is evaluated at the block's
+// columns as direct sums over the ring's few modes per row (the ring's
+// modes staged once in LDS as +-ja pairs), the transform along i is a batch
+// of LDS Stockham FFTs, and the block is written in the forms the
+// contraction reads it in (FP32 values and/or bf16 hi/lo operands).  No
+// intermediate goes through memory.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -96,10 +93,10 @@ hipError_t launch_gather(const double2 *fields, double2 *planes, int NN, int NG,
     return hipSuccess;
 }
 
-// Threads per block of both passes, and the most points a thread loads
-// (rpb * N and cpb * N stay within kPassThreads * kPassPer).
-constexpr int kPassThreads = 256;
-constexpr int kPassPer = 10;
+// Threads per workgroup of the ring pass; a column block holds at most
+// kPassMaxPoints points (cpb * N).
+
+constexpr int kPassMaxPoints = 4096;
 
 // The passes transform in the cache's precision: FP64 for FP64 caches, FP32
 // for FP32 caches (the FP32 paths' error budget is far above FP32 rounding).
@@ -111,92 +108,6 @@ template <>
 __device__ __forceinline__ float2 cvt2<float2>(double2 v)
 {
     return make_float2((float)v.x, (float)v.y);
-}
-
-// Fields are stored column-major: point (i, j) of an N x N field sits at
-// x = j * N + i.  (The contraction sums over all points, so the order is
-// free; this one makes the column pass read and write contiguous memory.)
-//
-// Row pass over the tasks of every level, one block per entry of `blk`:
-// block {t, x} transforms rows [x * rpb, x * rpb + rpb) of task t's row band
-// and writes them into the field's cache slot (cache + off + fid * stride),
-// column-major, in place of the final field.  The plane values come from the
-// NP x NP plane grid (NP >= N): a mode (ia, ja) inside the ring satisfies
-// |ia|, |ja| < N/2 and sits at (ia mod NP, ja mod NP) there.
-template <typename C2>
-__global__ __launch_bounds__(kPassThreads) void k_rowpass(const double2 *__restrict__ planes, int NP,
-                                                 C2 *__restrict__ cache,
-                                                 const FieldTask *__restrict__ tasks,
-                                                 const int2 *__restrict__ blk,
-                                                 const LevelDesc *__restrict__ lv,
-                                                 const double2 *__restrict__ tw_all,
-                                                 int *__restrict__ nzflag, BatchStrides bs)
-{
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    C2 *lds = reinterpret_cast<C2 *>(lds_raw);
-    const int2 B = blk[blockIdx.x];
-    const int fb = blockIdx.y;   // field of the batch
-    planes += fb * bs.planes;
-    cache += fb * bs.cache;
-    if (nzflag) nzflag += fb * bs.nz;
-    const FieldTask T = tasks[B.x];
-    const LevelDesc &L = lv[T.level];
-    const int N = L.fd.N, rpb = L.rpb;
-    const int nrows = min(N, 2 * T.rowband + 1);
-    const int q0 = B.y * rpb;
-    if (q0 >= nrows) return;
-    const int nb = min(rpb, nrows - q0);
-    C2 *twl = lds, *a = lds + N, *b = a + rpb * N;
-    for (int t = threadIdx.x; t < N; t += blockDim.x) twl[t] = cvt2<C2>(tw_all[L.tw_off + t]);
-    const unsigned long long mN = L.fd.mN;
-    // All of this thread's loads are issued before any LDS store, so they
-    // are in flight together (at most kPassPer per thread).
-    int nz = 0;
-    double2 vals[kPassPer];
-#pragma unroll
-    for (int e = 0; e < kPassPer; ++e) {
-        const int idx = threadIdx.x + e * kPassThreads;
-        double2 v = make_double2(0.0, 0.0);
-        if (idx < nb * N) {
-            const int bt = (int)udiv((unsigned)idx, mN), j = idx - bt * N;
-            const int q = q0 + bt;
-            const int i = (q <= T.rowband) ? q : N - nrows + q;
-            const int ia = (i > N / 2) ? (i - N) : i;
-            const int ja = (j > N / 2) ? (j - N) : j;
-            const int n = ia * ia + ja * ja;
-            if (n >= T.nlo && n < T.nhi) {
-                if (T.plane < 0) {
-                    v = make_double2(1.0, 0.0);
-                } else if (!(T.plane == 0 && i == 0 && j == 0)) {
-                    const int ip = ia < 0 ? ia + NP : ia, jp = ja < 0 ? ja + NP : ja;
-                    v = planes[((long long)T.plane * NP + ip) * NP + jp];
-                }
-            }
-        }
-        vals[e] = v;
-    }
-#pragma unroll
-    for (int e = 0; e < kPassPer; ++e) {
-        const int idx = threadIdx.x + e * kPassThreads;
-        if (idx < nb * N) {
-            nz |= (vals[e].x != 0.0 || vals[e].y != 0.0);
-            a[idx] = cvt2<C2>(vals[e]);   // a[bt * N + j] with idx = bt * N + j
-        }
-    }
-    if (nzflag && __any(nz) && (threadIdx.x & 63) == 0) atomicOr(&nzflag[T.zid], 1);
-    __syncthreads();
-    C2 *res = lds_fft(a, b, nb, L.fd, twl);
-    // Column-major store: consecutive threads take consecutive rows of one
-    // column (runs of nb points).
-    C2 *dst = cache + L.off + (long long)T.fid * L.stride;
-    const int rsh = L.rpb_log2;
-    for (int idx = threadIdx.x; idx < rpb * N; idx += blockDim.x) {
-        const int bt = idx & (rpb - 1), j = idx >> rsh;
-        if (bt >= nb) continue;
-        const int q = q0 + bt;
-        const int i = (q <= T.rowband) ? q : N - nrows + q;
-        dst[(long long)j * N + i] = res[bt * N + j];
-    }
 }
 
 // Two FP32 values as a bf16 pair, round to nearest even (one conversion
@@ -253,98 +164,264 @@ __device__ __forceinline__ void store8(float2 *dst, __bf16 *sp, const float2 *v)
     }
 }
 
-// Column pass over the tasks of every level, in place.  Entry {t, x} of
-// `blk` transforms columns [x * cpb, x * cpb + cpb) of task t's field along
-// i; those columns are the contiguous points [j0 * N, (j0 + cpb) * N) of the
-// field (column-major), j0 * N and cpb * N multiples of 8.  Rows outside the
-// ring's band were not written by the row pass and are taken as zero.
+// Fields are stored column-major: point (i, j) of an N x N field sits at
+// x = j * N + i.  (The contraction sums over all points, so the order is
+// free; this one makes every column block one contiguous range.)
+//
+// Ring pass: one entry {t, x} of `blk` builds columns [j0, j0 + cpb) of task
+// t's field, j0 = x * cpb, in three steps, all in LDS:
+//  1. rows: for every row i of the ring's band, the transform along j of
+//     the masked plane row, evaluated at the block's columns only, as a
+//     direct sum over the ring's modes on that row,
+//         X[i][j] = sum_{ja : nlo <= ia^2 + ja^2 < nhi} Delta(ia, ja) w^(ja j),
+//     w = exp(+2 pi i / N) (a ring holds few modes per row: the sum costs
+//     about as much as the row FFT would, and nothing is written to memory
+//     between the two directions); rows outside the band are zero;
+//  2. columns: the cpb transforms along i (Stockham, radices 4, 2, 3, 5);
+//  3. stores: the block's cpb * N contiguous points, in the cache's type
+//     and, for FP32 caches with `split`, as bf16 hi/lo operands, each in the
+//     forms the contraction reads the field in.
+// The modes of the band rows are staged in LDS first (in groups of rows
+// whose modes fit in the FFT's scratch buffer), converted to the cache's
+// type; the unit fields (plane < 0) take the value 1 on the ring, and the
+// DC mode of plane 0 is zero (funcs.c:101-103).  Modes (ia, ja) are taken
+// for ja in (-N/2, N/2], the range a length-N row transform sees.
 //
 // The grid is one round of resident workgroups; workgroup g takes entries
-// g, g + G, ...  While it transforms one entry in LDS and stores it, the
-// loads of its next entry are in flight in registers.  Every level's
-// twiddles (ntw values, the table of all levels) stay in LDS for the whole
-// launch.
-//
-// The result is written 8 points at a time; with `split` set (FP32 caches)
-// also as bf16 hi/lo operands at the same point offsets: for every 8
-// consecutive points, 32 bf16 values [Re hi x8][Re lo x8][Im hi x8]
-// [Im lo x8], with hi = bf16(v) and lo = bf16(v - hi) of the FP32 value v.
-// The task's `forms` selects which of the two are written (the contraction
-// reads a field in one form or both).  Past the field's N*N points (up to a
-// multiple of 8, inside the padded stride) zeros are written.
-template <typename C2>
-__device__ __forceinline__ void colpass_load(const C2 *__restrict__ cache,
-                                             const FieldTask *__restrict__ tasks,
-                                             const int2 *__restrict__ blk, int nblk,
-                                             const LevelDesc *__restrict__ lv, int bi,
-                                             C2 (&vals)[kPassPer])
+// g, g + G, ...  Every level's twiddles stay in LDS for the launch.
+
+// floor(sqrt(v)) for 0 <= v < 2^24.
+__device__ __forceinline__ int isqrt_dev(int v)
 {
-#pragma unroll
-    for (int e = 0; e < kPassPer; ++e) vals[e] = mk<C2>(0, 0);
-    if (bi >= nblk) return;
-    const int2 B = blk[bi];
-    const FieldTask T = tasks[B.x];
-    const LevelDesc &L = lv[T.level];
-    const int N = L.fd.N, cpb = L.cpb;
-    const int j0 = B.y * cpb;
-    if (j0 >= N) return;
-    const int nrows = min(N, 2 * T.rowband + 1);
-    const C2 *fld = cache + L.off + (long long)T.fid * L.stride + (long long)j0 * N;
-    const int npt = min(cpb, N - j0) * N;
-    const unsigned long long mN = L.fd.mN;
-#pragma unroll
-    for (int e = 0; e < kPassPer; ++e) {
-        const int idx = threadIdx.x + e * kPassThreads;
-        if (idx < npt) {
-            const int c = (int)udiv((unsigned)idx, mN), i = idx - c * N;
-            const int ia = (i > N / 2) ? (i - N) : i;
-            const bool inband = (ia <= T.rowband && -ia <= T.rowband) || nrows == N;
-            if (inband) vals[e] = fld[idx];
+    int r = (int)sqrtf((float)v);
+    while (r * r > v) --r;
+    while ((r + 1) * (r + 1) <= v) ++r;
+    return r;
+}
+
+// The ring's modes on row ia of a length-N transform (ja in (-N/2, N/2]),
+// as units of consecutive ja = ja0 .. ja0 + n - 1 >= 0: unit ja > 0 up to hn
+// holds both modes +ja and -ja (they share the twiddle's cosine and sine),
+// units ja = 0 and ja = N/2 hold one mode.
+struct RowUnits {
+    int ja0, n, hn;
+};
+__device__ __forceinline__ RowUnits row_units(int ia, int nlo, int nhi, int N)
+{
+    RowUnits u{0, 0, -1};
+    const int s = ia * ia, hi2 = nhi - 1 - s;
+    if (hi2 < 0) return u;
+    const int jhi = isqrt_dev(hi2);
+    const int lo2 = nlo - s;
+    const int jlo = lo2 <= 0 ? 0 : isqrt_dev(lo2 - 1) + 1;   // least j with j^2 >= lo2
+    const int hp = min(jhi, N / 2);   // positive modes: [jlo, hp]; negative: [-hn, -max(jlo, 1)]
+    if (jlo > hp) return u;
+    u.hn = min(jhi, N / 2 - 1);
+    u.ja0 = jlo;
+    u.n = hp - jlo + 1;
+    return u;
+}
+
+// Sum over a row's staged units md[2u], md[2u + 1] = (A, B) of ja = ja0 + u:
+// A w_r + i B w_i with w = w_N^(ja j): for a pair, A = v(+ja) + v(-ja) and
+// B = v(+ja) - v(-ja), so that v(+ja) w + v(-ja) conj(w) = A Re w + i B Im w;
+// for ja = 0 and ja = N/2, A = v(ja) and B = 0.
+template <typename C2>
+__device__ __forceinline__ C2 row_sum(const C2 *md, int n, int ja0, int j, int N,
+                                      unsigned long long mN, const C2 *tw)
+{
+    const unsigned u = (unsigned)(ja0 * j);
+    int t = (int)(u - udiv(u, mN) * (unsigned)N);
+    C2 acc = mk<C2>(0, 0);
+    for (int m = 0; m < n; ++m) {
+        const C2 A = md[2 * m], Bv = md[2 * m + 1], w = tw[t];
+        acc.x += A.x * w.x - Bv.y * w.y;
+        acc.y += A.y * w.x + Bv.x * w.y;
+        t += j;
+        if (t >= N) t -= N;
+    }
+    return acc;
+}
+
+// Units (pairs of modes) a workgroup stages once for all of its column
+// blocks; fields with more are staged again for every block, in row groups
+// that fit in the FFT's scratch buffer.
+// Shape of the ring pass per cache type (measured at the NN=384 grid
+// levels): FP32 workgroups of 512 threads with column blocks of up to 2048
+// points and 1024 staged units (three workgroups per CU by LDS); FP64
+// workgroups of 256 threads with blocks of up to 512 points and 640 units,
+// so that three FP64 workgroups also fit.  Fields with more units than that
+// are staged per block instead (a row holds at most N / 2 + 1 units).
+template <typename C2>
+struct PassShape;
+template <>
+struct PassShape<float2> {
+    static constexpr int threads = 512, unit_cap = 1024, block_points = 2048;
+};
+template <>
+struct PassShape<double2> {
+    static constexpr int threads = 256, unit_cap = 640, block_points = 512;
+};
+
+// Stages units [k0, k1) of a field (counted over its band rows in row
+// order) into dst[2 (k - k0)], dst[2 (k - k0) + 1]; returns whether one of
+// their modes is nonzero.
+template <typename C2>
+__device__ __forceinline__ int stage_units(const double2 *__restrict__ planes, int NP,
+                                           const FieldTask &T, int N, int nrows,
+                                           const int4 *rinfo, const int *roff, int g0, int g1,
+                                           int k0, int k1, C2 *dst)
+{
+    int nz = 0;
+    for (int k = k0 + (int)threadIdx.x; k < k1; k += blockDim.x) {
+        int ql = g0, qh = g1 - 1;   // the row q with roff[q] <= k < roff[q + 1]
+        while (ql < qh) {
+            const int mid = (ql + qh + 1) >> 1;
+            if (roff[mid] <= k) ql = mid;
+            else qh = mid - 1;
         }
+        const int q = ql;
+        const int4 ri = rinfo[q];
+        const int ja = ri.x + (k - roff[q]);
+        const bool pair = ja > 0 && ja <= ri.z;
+        const int i = q <= T.rowband ? q : N - nrows + q;
+        const int ia = i > N / 2 ? i - N : i;
+        double2 vp = make_double2(1.0, 0.0), vn = make_double2(pair ? 1.0 : 0.0, 0.0);
+        if (T.plane >= 0) {
+            const int ip = ia < 0 ? ia + NP : ia;
+            const double2 *prow = planes + ((long long)T.plane * NP + ip) * NP;
+            vp = (T.plane == 0 && ia == 0 && ja == 0) ? make_double2(0.0, 0.0) : prow[ja];
+            vn = pair ? prow[NP - ja] : make_double2(0.0, 0.0);
+        }
+        nz |= (vp.x != 0.0 || vp.y != 0.0 || vn.x != 0.0 || vn.y != 0.0);
+        dst[2 * (k - k0)] = cvt2<C2>(make_double2(vp.x + vn.x, vp.y + vn.y));
+        dst[2 * (k - k0) + 1] = cvt2<C2>(make_double2(vp.x - vn.x, vp.y - vn.y));
+    }
+    return nz;
+}
+
+// Direct row sums of rows [g0, g1) at columns [j0, j0 + cpb) into a; md
+// holds the units from unit k0 on.
+template <typename C2>
+__device__ __forceinline__ void row_sums(const int4 *rinfo, const int *roff, int g0, int g1,
+                                         int k0, const C2 *md, const FieldTask &T, int N,
+                                         int nrows, int j0, int cpb, int csh,
+                                         unsigned long long mN, const C2 *tw, C2 *a)
+{
+    for (int idx = threadIdx.x; idx < (g1 - g0) * cpb; idx += blockDim.x) {
+        const int q = g0 + (idx >> csh), c = idx & (cpb - 1);
+        const int4 ri = rinfo[q];
+        const int i = q <= T.rowband ? q : N - nrows + q;
+        a[c * N + i] = row_sum(md + 2 * (roff[q] - k0), ri.y, ri.x, j0 + c, N, mN, tw);
     }
 }
 
+// One unit of work per entry of `blk`: {task, first | (end << 16)} covers
+// column blocks [first, end) of the task's field (columns [x cpb, x cpb + cpb)).
 template <typename C2>
-__global__ __launch_bounds__(kPassThreads) void k_colpass(C2 *__restrict__ cache,
-                                                 __bf16 *__restrict__ split,
-                                                 const FieldTask *__restrict__ tasks,
-                                                 const int2 *__restrict__ blk, int nblk,
-                                                 const LevelDesc *__restrict__ lv,
-                                                 const double2 *__restrict__ tw_all, int ntw,
-                                                 BatchStrides bs)
+__global__ __launch_bounds__(PassShape<C2>::threads) void k_ringpass(
+                                                  const double2 *__restrict__ planes,
+                                                  int NP, C2 *__restrict__ cache,
+                                                  __bf16 *__restrict__ split,
+                                                  const FieldTask *__restrict__ tasks,
+                                                  const int2 *__restrict__ blk, int nblk,
+                                                  const LevelDesc *__restrict__ lv,
+                                                  const double2 *__restrict__ tw_all, int ntw,
+                                                  int maxN, int *__restrict__ nzflag,
+                                                  BatchStrides bs)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     C2 *twl_all = reinterpret_cast<C2 *>(lds_raw);
-    C2 *abuf = twl_all + ((ntw + 1) & ~1);   // 16-byte aligned
+    constexpr int kUnitCap = PassShape<C2>::unit_cap;
+    C2 *md = twl_all + ((ntw + 1) & ~1);                                 // kUnitCap units
+    int4 *rinfo = reinterpret_cast<int4 *>(md + 2 * kUnitCap);           // per band row
+    int *roff = reinterpret_cast<int *>(rinfo + maxN);                   // maxN + 1 offsets
+    C2 *abuf = reinterpret_cast<C2 *>(roff + ((maxN + 4) & ~3));
     const int fb = blockIdx.y;   // field of the batch
+    planes += fb * bs.planes;
     cache += fb * bs.cache;
     if (split) split += fb * bs.cache * 4;
-    for (int t = threadIdx.x; t < ntw; t += blockDim.x) twl_all[t] = cvt2<C2>(tw_all[t]);
-    C2 cur[kPassPer], nxt[kPassPer];
-    int bi = blockIdx.x;
-    colpass_load<C2>(cache, tasks, blk, nblk, lv, bi, cur);
-    for (; bi < nblk; bi += gridDim.x) {
-        colpass_load<C2>(cache, tasks, blk, nblk, lv, bi + gridDim.x, nxt);
+    if (nzflag) nzflag += fb * bs.nz;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int t = tid; t < ntw; t += blockDim.x) twl_all[t] = cvt2<C2>(tw_all[t]);
+    for (int bi = blockIdx.x; bi < nblk; bi += gridDim.x) {
         const int2 B = blk[bi];
         const FieldTask T = tasks[B.x];
         const LevelDesc &L = lv[T.level];
-        const int N = L.fd.N, cpb = L.cpb;
-        const int j0 = B.y * cpb;
-        if (j0 < N) {   // uniform over the workgroup
-            const long long base = L.off + (long long)T.fid * L.stride + (long long)j0 * N;
-            const int npt = min(cpb, N - j0) * N;
-            C2 *fld = cache + base;
-            C2 *a = abuf, *b = abuf + cpb * N;
+        const int N = L.fd.N, cpb = L.cpb, csh = L.cpb_log2;
+        const int xb = B.y & 0xffff, xe = min(B.y >> 16, (N + cpb - 1) / cpb);
+        const C2 *tw = twl_all + L.tw_off;
+        const unsigned long long mN = L.fd.mN;
+        const int nrows = max(0, min(N, 2 * T.rowband + 1));
+        C2 *a = abuf, *b = abuf + cpb * N;   // b: FFT scratch (and unit staging of big fields)
+        const int cap = cpb * N / 2;          // units b holds
+
+        // Band rows: the ring's units per row and their offsets (wave 0).
+        if (wave == 0) {
+            const int per = (nrows + 63) / 64, q0 = lane * per;
+            int cnt = 0;
+            for (int q = q0; q < min(q0 + per, nrows); ++q) {
+                const int i = q <= T.rowband ? q : N - nrows + q;
+                const int ia = i > N / 2 ? i - N : i;
+                const RowUnits ru = row_units(ia, T.nlo, T.nhi, N);
+                rinfo[q] = make_int4(ru.ja0, ru.n, ru.hn, 0);
+                cnt += ru.n;
+            }
+            int inc = cnt;   // inclusive prefix over the lanes
 #pragma unroll
-            for (int e = 0; e < kPassPer; ++e) {
-                const int idx = threadIdx.x + e * kPassThreads;
-                if (idx < cpb * N) a[idx] = cur[e];
+            for (int d = 1; d < 64; d <<= 1) {
+                const int v = __shfl_up(inc, d, 64);
+                if (lane >= d) inc += v;
+            }
+            int off = inc - cnt;
+            for (int q = q0; q < min(q0 + per, nrows); ++q) {
+                roff[q] = off;
+                off += rinfo[q].y;
+            }
+            if (lane == 63) roff[nrows] = inc;
+        }
+        __syncthreads();
+        const int total = roff[nrows];
+        const bool once = total <= kUnitCap;   // uniform
+        int nz = 0;
+        if (once) nz |= stage_units(planes, NP, T, N, nrows, rinfo, roff, 0, max(nrows, 1), 0,
+                                    total, md);
+        for (int x = xb; x < xe; ++x) {
+            const int j0 = x * cpb;
+            // Rows outside the band are zero.
+            for (int idx = tid; idx < (N - nrows) * cpb; idx += blockDim.x)
+                a[(idx & (cpb - 1)) * N + T.rowband + 1 + (idx >> csh)] = mk<C2>(0, 0);
+            if (once) {
+                __syncthreads();   // md staged (first block); b free (later blocks)
+                row_sums(rinfo, roff, 0, nrows, 0, md, T, N, nrows, j0, cpb, csh, mN, tw, a);
+            } else {
+                // Groups of band rows whose units fit in b (a row holds at most
+                // N / 2 + 1 <= cap units).
+                for (int g0 = 0; g0 < nrows;) {
+                    int lo = g0 + 1, hi = nrows;   // largest g1 with roff[g1] - roff[g0] <= cap
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (roff[mid] - roff[g0] <= cap) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    const int g1 = lo;
+                    __syncthreads();   // b free
+                    nz |= stage_units(planes, NP, T, N, nrows, rinfo, roff, g0, g1, roff[g0],
+                                      roff[g1], b);
+                    __syncthreads();
+                    row_sums(rinfo, roff, g0, g1, roff[g0], b, T, N, nrows, j0, cpb, csh, mN, tw,
+                             a);
+                    g0 = g1;
+                }
             }
             __syncthreads();
-            C2 *res = lds_fft(a, b, cpb, L.fd, twl_all + L.tw_off);
+            C2 *res = lds_fft(a, b, cpb, L.fd, tw);
+            const long long fbase = L.off + (long long)T.fid * L.stride + (long long)j0 * N;
+            C2 *fld = cache + fbase;
+            const int npt = min(cpb, N - j0) * N;
             const int ngroups = (npt + 7) >> 3;
             const bool own = T.forms & 1, ops = split && (T.forms & 2);
-            for (int g = threadIdx.x; g < ngroups; g += blockDim.x) {
+            for (int g = tid; g < ngroups; g += blockDim.x) {
                 C2 v[8];
                 if (8 * g + 8 <= npt) {   // a whole group: no per-point test
 #pragma unroll
@@ -352,26 +429,30 @@ __global__ __launch_bounds__(kPassThreads) void k_colpass(C2 *__restrict__ cache
                 } else {
 #pragma unroll
                     for (int t = 0; t < 8; ++t) {
-                        const int x = 8 * g + t;
-                        v[t] = x < npt ? res[x] : mk<C2>(0, 0);
+                        const int xx = 8 * g + t;
+                        v[t] = xx < npt ? res[xx] : mk<C2>(0, 0);
                     }
                 }
-                store8(own ? fld + 8 * g : nullptr, ops ? split + (base + 8 * g) * 4 : nullptr, v);
+                store8(own ? fld + 8 * g : nullptr, ops ? split + (fbase + 8 * g) * 4 : nullptr,
+                       v);
             }
-            __syncthreads();   // the next entry overwrites a and b
+            __syncthreads();   // the next block overwrites a and b
         }
-#pragma unroll
-        for (int e = 0; e < kPassPer; ++e) cur[e] = nxt[e];
+        if (nzflag && xb == 0 && __any(nz) && lane == 0) atomicOr(&nzflag[T.zid], 1);
     }
 }
 
-// Transforms per block: about 1024 points of work for 256 threads, within
-// the 160 KiB of LDS up to N = 600 ((2 b + 1) N values of the cache's
-// type).  A column
-// block's points start and end on whole 8-point groups: N is even and the
-// column batch at least 4.
-int fft_rows_per_block(int N) { return N >= 256 ? 4 : N >= 128 ? 8 : N >= 64 ? 16 : 32; }
-int fft_cols_per_block(int N) { return N >= 320 ? 4 : N >= 128 ? 8 : N >= 64 ? 16 : 32; }
+// Columns per block: the largest power of two with cpb N within the block
+// size of the cache type (at least 4, so that blocks of even N start on
+// 8-point groups).  Larger blocks leave fewer threads idle in the FFT stages
+// and cost fewer barriers per point; smaller ones fit more workgroups.
+int fft_cols_per_block(int N, int elem)
+{
+    const int pts = elem == 8 ? PassShape<double2>::block_points : PassShape<float2>::block_points;
+    int c = 4;
+    while (2 * c * N <= pts) c *= 2;
+    return c;
+}
 
 template <typename C2>
 static hipError_t launch_fft(const double2 *planes, int NP, C2 *cache, __bf16 *split,
@@ -380,59 +461,65 @@ static hipError_t launch_fft(const double2 *planes, int NP, C2 *cache, __bf16 *s
                              const BatchStrides &bs, int nf, hipStream_t s)
 {
     if (nf < 1 || nf > 65535) return hipErrorInvalidValue;
-    size_t lds_r = 0, lds_c = 0;
-    int ntw = 0;   // twiddles of all levels (the table is the levels' back to back)
+    size_t lds_a = 0;
+    int ntw = 0, maxN = 0;   // twiddles of all levels (the table is the levels' back to back)
     for (int l = 0; l < nlev; ++l) {
         const LevelDesc &L = h_lv[l];
         const int N = L.fd.N;
-        if (L.stride < (long long)N * N || L.stride % 8 != 0 || L.rpb < 1 || L.cpb < 1)
+        if (L.stride < (long long)N * N || L.stride % 8 != 0 || L.cpb < 4)
             return hipErrorInvalidValue;
-        if (L.cpb != (1 << L.cpb_log2) || L.rpb != (1 << L.rpb_log2) || (L.cpb * N) % 8 != 0 ||
-            N >= 4096 || L.rpb * N > kPassThreads * kPassPer || L.cpb * N > kPassThreads * kPassPer)
+        // N < 1024: the twiddle index products (ja mod N) * j stay below 2^20 (udiv).
+        if (L.cpb != (1 << L.cpb_log2) || (L.cpb * N) % 8 != 0 || N >= 1024 ||
+            L.cpb * N > kPassMaxPoints)
             return hipErrorInvalidValue;
-        lds_r = std::max(lds_r, (2 * (size_t)L.rpb + 1) * N * sizeof(C2));
-        lds_c = std::max(lds_c, 2 * (size_t)L.cpb * N * sizeof(C2));
+        lds_a = std::max(lds_a, 2 * (size_t)L.cpb * N * sizeof(C2));
         ntw = std::max(ntw, L.tw_off + N);
+        maxN = std::max(maxN, N);
     }
-    lds_c += (size_t)((ntw + 1) & ~1) * sizeof(C2);
-    if (fb.nrow > 0) {
-        (void)hipFuncSetAttribute((const void *)k_rowpass<C2>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_r);
-        hipLaunchKernelGGL(k_rowpass<C2>, dim3(fb.nrow, nf), dim3(kPassThreads), lds_r, s, planes, NP,
-                           cache, tasks, fb.row, d_lv, tw, nzflag, bs);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    if (fb.ncol > 0) {
-        (void)hipFuncSetAttribute((const void *)k_colpass<C2>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_c);
-        // One round of resident workgroups, shared by the fields of the batch.
-        int dev = 0, ncu = 0, per_cu = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_colpass<C2>, kPassThreads,
-                                                         lds_c) != hipSuccess)
-            return hipErrorInvalidValue;
-        const long long round = (long long)std::max(1, ncu) * std::max(1, per_cu);
-        const int gx = (int)std::max(1LL, std::min<long long>(fb.ncol, round / nf));
-        hipLaunchKernelGGL(k_colpass<C2>, dim3(gx, nf), dim3(kPassThreads), lds_c, s, cache, split,
-                           tasks, fb.col, fb.ncol, d_lv, tw, ntw, bs);
-    }
+    if (fb.n <= 0) return hipSuccess;
+    const size_t lds = (size_t)(((ntw + 1) & ~1) + 2 * PassShape<C2>::unit_cap) * sizeof(C2) +
+                       (size_t)maxN * sizeof(int4) +
+                       (size_t)((maxN + 4) & ~3) * sizeof(int) + lds_a;
+    auto k = k_ringpass<C2>;
+    (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    // One round of resident workgroups, shared by the fields of the batch.
+    int dev = 0, ncu = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, PassShape<C2>::threads, lds) !=
+            hipSuccess)
+        return hipErrorInvalidValue;
+    const long long round = (long long)std::max(1, ncu) * std::max(1, per_cu);
+    const int gx = (int)std::max(1LL, std::min<long long>(fb.n, round / nf));
+    hipLaunchKernelGGL(k, dim3(gx, nf), dim3(PassShape<C2>::threads), lds, s, planes, NP, cache, split,
+                       tasks, fb.units, fb.n, d_lv, tw, ntw, maxN, nzflag, bs);
     return hipGetLastError();
 }
 
-void fft_blocks(const FieldTask *tasks, int ntasks, const LevelDesc *h_lv, std::vector<int2> &row,
+// Units of the ring pass: each task's column blocks in runs of about
+// kUnitPoints points (a unit stages its field's modes once), largest first.
+constexpr int kUnitPoints = 8192;
+
+void fft_blocks(const FieldTask *tasks, int ntasks, const LevelDesc *h_lv,
                 std::vector<int2> &col)
 {
-    row.clear();
     col.clear();
+    std::vector<std::pair<long long, int2>> units;
     for (int t = 0; t < ntasks; ++t) {
         const LevelDesc &L = h_lv[tasks[t].level];
-        const int N = L.fd.N;
-        const int nrows = std::min(N, 2 * tasks[t].rowband + 1);
-        for (int x = 0; x * L.rpb < nrows; ++x) row.push_back(make_int2(t, x));
-        for (int x = 0; x * L.cpb < N; ++x) col.push_back(make_int2(t, x));
+        const int N = L.fd.N, nblocks = (N + L.cpb - 1) / L.cpb;
+        const int per = std::max(1, kUnitPoints / (L.cpb * N));
+        for (int x = 0; x < nblocks; x += per) {
+            const int e = std::min(nblocks, x + per);
+            units.push_back({(long long)(e - x) * L.cpb * N, make_int2(t, x | (e << 16))});
+        }
     }
+    std::stable_sort(units.begin(), units.end(),
+                     [](const std::pair<long long, int2> &u, const std::pair<long long, int2> &v) {
+                         return u.first > v.first;
+                     });
+    for (const auto &u : units) col.push_back(u.second);
 }
 
 hipError_t launch_fft_f64(const double2 *planes, int NP, double2 *cache, const FieldTask *tasks,
