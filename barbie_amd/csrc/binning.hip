@@ -159,18 +159,42 @@ __device__ __forceinline__ void bin_reduce_block(const double *abs_, const doubl
         binpart[tid] = v;
     }
 }
+// The same for the 16 sums of S bs[ii] alone (the same operations, so the
+// same bits as the first 16 of bin_reduce_block).
+__device__ __forceinline__ void bin_reduce_block16(const double *abs_,
+                                                   double *__restrict__ binpart)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __shared__ double part[kBinThreads / 64][16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const double v0 = wave_sum(abs_[k]);
+        if (lane == 0) part[wave][k] = v0;
+    }
+    __syncthreads();
+    if (tid < 16) {
+        double v = part[0][tid];
+        for (int w = 1; w < kBinThreads / 64; ++w) v += part[w][tid];
+        binpart[tid] = v;
+    }
+}
 
 
+// All 48 sums of each segment.  With `skip`, a field whose skip flag is set
+// (its norm and analytic sums are taken from the field-independent ones)
+// is left out.
 __global__ __launch_bounds__(kBinThreads) void k_binning(const int32_t *__restrict__ sg,
                                                 const BinRec *__restrict__ recs,
                                                 const double *__restrict__ S,
                                                 const double *__restrict__ num,
                                                 const int *__restrict__ nzflag,
                                                 const double *__restrict__ avg, BinParams bp,
-                                                double *__restrict__ binpart, BatchStrides st)
+                                                double *__restrict__ binpart, BatchStrides st,
+                                                const int *__restrict__ skip)
 {
     const int sgi = blockIdx.x;   // segment
     const int fb = blockIdx.y;    // field of the batch
+    if (skip && skip[fb]) return;   // uniform over the workgroup
     S += fb * st.part;
     nzflag += fb * st.nz;
     binpart += fb * st.binpart;
@@ -202,6 +226,65 @@ __global__ __launch_bounds__(kBinThreads) void k_binning(const int32_t *__restri
         rec_accumulate(R, Sv, num[R.rp], biana, bs, nm, nzflag, bp.Nr, abs_, anm, abi);
     }
     bin_reduce_block(abs_, anm, abi, binpart + (long long)sgi * 48);
+}
+
+// The 16 sums of S bs[ii] of each segment: k_binning's record loop without
+// the norm and analytic sums (which do not depend on the field's values),
+// in the same order, so the same bits.  Fewer registers than k_binning:
+// more waves per SIMD to hide the scattered partial-sum loads.
+__global__ __launch_bounds__(kBinThreads) void k_binning_s(const int32_t *__restrict__ sg,
+                                                  const BinRec *__restrict__ recs,
+                                                  const double *__restrict__ S,
+                                                  const double *__restrict__ avg, BinParams bp,
+                                                  double *__restrict__ binpart_s,
+                                                  BatchStrides st)
+{
+#pragma clang fp contract(off)
+    const int sgi = blockIdx.x;   // segment
+    const int fb = blockIdx.y;    // field of the batch
+    S += fb * st.part;
+    binpart_s += fb * st.binpart_s;
+    const int tid = threadIdx.x;
+    double abs_[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) abs_[k] = 0.0;
+    const int end = sg[sgi + 1];
+    int c = sg[sgi] + tid;
+    BinRec Rn{};
+    double Sn = 0.0;
+    if (c < end) {
+        Rn = recs[c];
+        for (int sp = 0; sp < Rn.nsp; ++sp) Sn += S[(long long)Rn.pos + 256 * sp];
+    }
+    for (; c < end; c += kBinThreads) {
+        const BinRec R = Rn;
+        const double Sv = Sn;   // sum of the x-split partials, fixed order
+        if (c + kBinThreads < end) {
+            Rn = recs[c + kBinThreads];
+            Sn = 0.0;
+            for (int sp = 0; sp < Rn.nsp; ++sp) Sn += S[(long long)Rn.pos + 256 * sp];
+        }
+        double bs[16], nm[16], biana;
+        rec_coeffs(R, avg, bp, bs, nm, biana);   // the compiler drops nm and biana
+#pragma unroll
+        for (int k = 0; k < 16; ++k) abs_[k] = fma(Sv, bs[k], abs_[k]);
+    }
+    bin_reduce_block16(abs_, binpart_s + (long long)sgi * 16);
+}
+
+// match[f] = 1 when field f's nonzero flags equal `ref` (then its norm and
+// analytic sums are the field-independent ones), else 0.  One workgroup per
+// field of the batch.
+__global__ __launch_bounds__(256) void k_flags_match(const int *__restrict__ nzflag,
+                                                     const int *__restrict__ ref, int n,
+                                                     int *__restrict__ match, BatchStrides st)
+{
+    const int fb = blockIdx.x;
+    nzflag += fb * st.nz;
+    int diff = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) diff |= (nzflag[i] != 0) != (ref[i] != 0);
+    diff = __syncthreads_or(diff);
+    if (threadIdx.x == 0) match[fb] = diff ? 0 : 1;
 }
 
 // The field-independent coefficients of every record, computed once (the
@@ -262,12 +345,16 @@ __global__ __launch_bounds__(kBinThreads) void k_binning_w(const int32_t *__rest
 
 // Bin sums: thread (bin, t < 48) adds the bin's segment sums in order;
 // bins without records get zeros.
+// With a BinStatic (bs.binpart_s set), the S sums come from binpart_s and
+// the norm and analytic sums of a field whose flags match the reference
+// from the field-independent bins (bs.sums: [3][nbins * 16], the outputs of
+// a full pass with the reference flags).
 __global__ __launch_bounds__(256) void k_bin_reduce(const int32_t *__restrict__ bin_sg, int nbins,
                                                     const double *__restrict__ binpart,
                                                     double *__restrict__ out_bs,
                                                     double *__restrict__ out_norm,
                                                     double *__restrict__ out_biana,
-                                                    BatchStrides st)
+                                                    BatchStrides st, BinStatic bst)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nbins * 48) return;
@@ -275,7 +362,14 @@ __global__ __launch_bounds__(256) void k_bin_reduce(const int32_t *__restrict__ 
     binpart += fb * st.binpart;
     const int b = i / 48, t = i - b * 48;
     double v = 0.0;
-    for (int sgi = bin_sg[b]; sgi < bin_sg[b + 1]; ++sgi) v += binpart[(long long)sgi * 48 + t];
+    if (t < 16 && bst.binpart_s) {
+        const double *bp = bst.binpart_s + fb * st.binpart_s;
+        for (int sgi = bin_sg[b]; sgi < bin_sg[b + 1]; ++sgi) v += bp[(long long)sgi * 16 + t];
+    } else if (t >= 16 && bst.binpart_s && bst.match[fb]) {
+        v = bst.sums[(long long)(t >> 4) * nbins * 16 + (long long)b * 16 + (t & 15)];
+    } else {
+        for (int sgi = bin_sg[b]; sgi < bin_sg[b + 1]; ++sgi) v += binpart[(long long)sgi * 48 + t];
+    }
     double *o = t < 16 ? out_bs : t < 32 ? out_norm : out_biana;
     if (o) o[fb * st.bins + (long long)b * 16 + (t & 15)] = v;
 }
@@ -293,10 +387,26 @@ hipError_t launch_binning(const int32_t *sg, int nseg, const int32_t *bin_sg, co
                           const double *S, const double *num, const int *nzflag, const double *avg,
                           const BinParams &bp, double *binpart, double *bins_bs,
                           double *bins_norm, double *bins_biana, const BatchStrides &st, int nf,
-                          hipStream_t s, const double *W, int nrec)
+                          hipStream_t s, const double *W, int nrec, const BinStatic &bst)
 {
     if (bp.nbins <= 0) return hipSuccess;
     if (nf < 1 || nf > 65535) return hipErrorInvalidValue;
+    if (nseg > 0 && !W && bst.binpart_s) {
+        // S sums for every field; the full pass only for fields whose flags
+        // differ from the reference.
+        hipLaunchKernelGGL(k_flags_match, dim3(nf), dim3(256), 0, s, nzflag, bst.ref, bst.nflags,
+                           bst.match, st);
+        hipLaunchKernelGGL(k_binning_s, dim3(nseg, nf), dim3(kBinThreads), 0, s, sg, recs, S, avg,
+                           bp, bst.binpart_s, st);
+        hipLaunchKernelGGL(k_binning, dim3(nseg, nf), dim3(kBinThreads), 0, s, sg, recs, S, num,
+                           nzflag, avg, bp, binpart, st, (const int *)bst.match);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        const int n = bp.nbins * 48;
+        hipLaunchKernelGGL(k_bin_reduce, dim3((n + 255) / 256, nf), dim3(256), 0, s, bin_sg,
+                           bp.nbins, binpart, bins_bs, bins_norm, bins_biana, st, bst);
+        return hipGetLastError();
+    }
     if (nseg > 0 && W) {
         if (nseg > 65535) return hipErrorInvalidValue;
         hipLaunchKernelGGL(k_binning_w, dim3(nf, nseg), dim3(kBinThreads), 0, s, sg, recs, nrec, W,
@@ -305,13 +415,13 @@ hipError_t launch_binning(const int32_t *sg, int nseg, const int32_t *bin_sg, co
         if (e != hipSuccess) return e;
     } else if (nseg > 0) {
         hipLaunchKernelGGL(k_binning, dim3(nseg, nf), dim3(kBinThreads), 0, s, sg, recs, S, num,
-                           nzflag, avg, bp, binpart, st);
+                           nzflag, avg, bp, binpart, st, (const int *)nullptr);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     const int n = bp.nbins * 48;
     hipLaunchKernelGGL(k_bin_reduce, dim3((n + 255) / 256, nf), dim3(256), 0, s, bin_sg, bp.nbins,
-                       binpart, bins_bs, bins_norm, bins_biana, st);
+                       binpart, bins_bs, bins_norm, bins_biana, st, BinStatic{});
     return hipGetLastError();
 }
 

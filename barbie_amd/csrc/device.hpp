@@ -163,6 +163,7 @@ struct BatchStrides {
     long long part = 0;     // partial sums (doubles)
     long long bins = 0;     // bin outputs (doubles)
     long long binpart = 0;  // per-segment bin sums (doubles)
+    long long binpart_s = 0;  // per-segment S sums (doubles)
 };
 
 // Work units of the ring pass: {task, first | (end << 16)} column-block
@@ -223,11 +224,26 @@ hipError_t launch_contract(CPrec p, int waves, int tww, const void *cache, const
 // within one bin: sg[s]..sg[s+1] are segment s's records, bin_sg[b]..
 // bin_sg[b+1] the segments of bin b.  binpart holds 48 sums per segment.
 constexpr int kBinSeg = 2048;
+// Field-independent binning: the norm and analytic sums of each bin depend
+// on the field only through its nonzero flags (the zero rule).  `sums` holds
+// them ([3][nbins * 16], bins as launch_binning writes them; the first third
+// unused) for the reference flags `ref`; launch_binning then accumulates
+// the 16 S sums per bin alone (binpart_s, [segment][16] per field) and runs
+// the full 48-sum pass only for fields whose flags differ (match[f] = 0,
+// decided on the device).  The bits are the same either way.
+struct BinStatic {
+    const int *ref = nullptr;
+    int nflags = 0;
+    const double *sums = nullptr;
+    int *match = nullptr;          // [batch]
+    double *binpart_s = nullptr;   // null: no field-independent sums
+};
 hipError_t launch_binning(const int32_t *sg, int nseg, const int32_t *bin_sg, const BinRec *recs,
                           const double *S, const double *num, const int *nzflag, const double *avg,
                           const BinParams &bp, double *binpart, double *bins_bs,
                           double *bins_norm, double *bins_biana, const BatchStrides &bs, int nf,
-                          hipStream_t s, const double *W = nullptr, int nrec = 0);
+                          hipStream_t s, const double *W = nullptr, int nrec = 0,
+                          const BinStatic &bst = BinStatic{});
 // The records' field-independent binning coefficients, [33][nrec] doubles
 // (k_binning's own computation): with W, launch_binning reads them instead
 // of computing them for every field, with the same bits.

@@ -1020,6 +1020,7 @@ struct bb_estimator {
     int32_t *d_sg = nullptr, *d_bin_sg = nullptr;
     double *d_binpart = nullptr;      // [segment][48] per field
     double *d_bincoef = nullptr;      // [33][records] binning coefficients (batches), or null
+    BinStatic bstat;                  // field-independent norm and analytic sums (single fields)
     double *d_num = nullptr;
     std::vector<double> num_host;
     int *d_nz = nullptr;              // [Nmax * Nr]
@@ -1516,8 +1517,53 @@ size_t layout_bytes(const PlanData &d, const Layout &L, size_t F)
     b += L.recs.size() * sizeof(BinRec) + (L.sg.size() + L.bin_sg.size()) * 4;
     b += (L.sg.size() - 1) * 48 * 8 * (size_t)F;
     b += bin_table_bytes(L, F);
+    if (bin_table_bytes(L, F) == 0)   // field-independent bins (static_bins)
+        b += (L.sg.size() - 1) * 16 * 8 * (size_t)F + (size_t)3 * d.nbins * BB_NPOLES * 8 +
+             (size_t)Nmax * Nr * 4 + (size_t)F * 4;
     b += d.tri.size() / 3 * 8 + Nr * 8;
     return b;
+}
+
+// The field-independent bins (BinStatic): the reference nonzero flags are
+// those of any field whose ring modes are not all zero, i.e. every field a
+// task builds except the rings with no mode (plane 0: no mode but the DC
+// mode, which the reference zeroes).  One full binning pass with those flags
+// (and zero contraction sums) gives the norm and analytic sums.
+int static_bins(bb_estimator *E)
+{
+    const PlanData &d = E->plan->d;
+    const Layout &L = *E->L;
+    const int Nmax = d.g.Nmax, Nr = d.g.Nr, nseg = (int)L.sg.size() - 1;
+    std::vector<int> ref((size_t)Nmax * Nr, 0);
+    for (const Level &V : L.lv)
+        for (const FieldTask &T : V.tasks) {
+            bool any = false;
+            for (int ia = 0; ia * ia < T.nhi && !any; ++ia)
+                for (int ja = 0; ia * ia + ja * ja < T.nhi && !any; ++ja) {
+                    const int n = ia * ia + ja * ja;
+                    any = n >= T.nlo && (T.plane != 0 || n > 0);
+                }
+            if (any) ref[T.zid] = 1;
+        }
+    int rc;
+    int *d_ref, *d_match;
+    double *d_sums, *d_bps;
+    if ((rc = dev_alloc(&d_ref, ref.size(), E->owned))) return rc;
+    if ((rc = dev_alloc(&d_match, (size_t)E->batch, E->owned))) return rc;
+    if ((rc = dev_alloc(&d_sums, (size_t)3 * d.nbins * BB_NPOLES, E->owned))) return rc;
+    if ((rc = dev_alloc(&d_bps, (size_t)nseg * 16 * E->batch, E->owned))) return rc;
+    if ((rc = upload(d_ref, ref.data(), sizeof(int) * ref.size(), E->stream))) return rc;
+    const size_t nb = (size_t)d.nbins * BB_NPOLES;
+    HIPCHK(launch_binning(E->d_sg, nseg, E->d_bin_sg, E->d_recs, E->d_part, E->d_num, d_ref,
+                          E->d_avg, L.bp, E->d_binpart, d_sums, d_sums + nb, d_sums + 2 * nb,
+                          E->bst, 1, E->stream));
+    E->bst.binpart_s = (long long)nseg * 16;
+    E->bstat.ref = d_ref;
+    E->bstat.nflags = Nmax * Nr;
+    E->bstat.sums = d_sums;
+    E->bstat.match = d_match;
+    E->bstat.binpart_s = d_bps;
+    return BB_OK;
 }
 
 int estimator_init(bb_estimator *E)
@@ -1637,6 +1683,7 @@ int estimator_init(bb_estimator *E)
     // written by the transforms: zero them (and everything else) once.
     HIPCHK(hipMemsetAsync(E->d_cache, 0, cpts * (E->cprec == CPrec::F64 ? 16 : 8), E->stream));
     if (E->d_split) HIPCHK(hipMemsetAsync(E->d_split, 0, cpts * 8, E->stream));
+    if (!E->d_bincoef && L.sg.size() > 1 && (rc = static_bins(E))) return rc;
     HIPCHK(hipStreamSynchronize(E->stream));
     return BB_OK;
 }
@@ -1673,7 +1720,7 @@ int enqueue_kernels(bb_estimator *E, int nf, const double *d_fields, double *d_b
     if (timed) HIPCHK(hipEventRecord(E->ev[3], s));
     HIPCHK(launch_binning(E->d_sg, (int)L.sg.size() - 1, E->d_bin_sg, E->d_recs, E->d_part,
                           E->d_num, E->d_nz, E->d_avg, L.bp, E->d_binpart, d_bs, d_norm, d_biana,
-                          E->bst, nf, s, E->d_bincoef, (int)L.recs.size()));
+                          E->bst, nf, s, E->d_bincoef, (int)L.recs.size(), E->bstat));
     return BB_OK;
 }
 
@@ -1694,7 +1741,7 @@ int enqueue_post(bb_estimator *E, int nf, double *d_bs, double *d_norm, double *
     const Layout &L = *E->L;
     HIPCHK(launch_binning(E->d_sg, (int)L.sg.size() - 1, E->d_bin_sg, E->d_recs, E->d_part,
                           E->d_num, E->d_nz, E->d_avg, L.bp, E->d_binpart, d_bs, d_norm, d_biana,
-                          E->bst, nf, E->stream, E->d_bincoef, (int)L.recs.size()));
+                          E->bst, nf, E->stream, E->d_bincoef, (int)L.recs.size(), E->bstat));
     return BB_OK;
 }
 
