@@ -232,19 +232,30 @@ __global__ __launch_bounds__(kBinThreads) void k_binning(const int32_t *__restri
 // the norm and analytic sums (which do not depend on the field's values),
 // in the same order, so the same bits.  Fewer registers than k_binning:
 // more waves per SIMD to hide the scattered partial-sum loads.
+//
+// Each workgroup also compares its slice of the field's nonzero flags with
+// the reference flags and clears match[f] on a difference (match[f] is set
+// to 1 by the plane gather, before the flags are raised).
 __global__ __launch_bounds__(kBinThreads) void k_binning_s(const int32_t *__restrict__ sg,
-                                                  const BinRec *__restrict__ recs,
+                                                  int nseg, const BinRec *__restrict__ recs,
                                                   const double *__restrict__ S,
                                                   const double *__restrict__ avg, BinParams bp,
-                                                  double *__restrict__ binpart_s,
-                                                  BatchStrides st)
+                                                  const int *__restrict__ nzflag,
+                                                  BinStatic bst, BatchStrides st)
 {
 #pragma clang fp contract(off)
     const int sgi = blockIdx.x;   // segment
     const int fb = blockIdx.y;    // field of the batch
     S += fb * st.part;
-    binpart_s += fb * st.binpart_s;
+    double *binpart_s = bst.binpart_s + fb * st.binpart_s;
     const int tid = threadIdx.x;
+    {
+        const int per = (bst.nflags + nseg - 1) / nseg, f0 = sgi * per;
+        int diff = 0;
+        for (int k = tid; k < per && f0 + k < bst.nflags; k += kBinThreads)
+            diff |= (nzflag[fb * st.nz + f0 + k] != 0) != (bst.ref[f0 + k] != 0);
+        if (__any(diff) && (tid & 63) == 0) atomicAnd(&bst.match[fb], 0);
+    }
     double abs_[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) abs_[k] = 0.0;
@@ -270,21 +281,6 @@ __global__ __launch_bounds__(kBinThreads) void k_binning_s(const int32_t *__rest
         for (int k = 0; k < 16; ++k) abs_[k] = fma(Sv, bs[k], abs_[k]);
     }
     bin_reduce_block16(abs_, binpart_s + (long long)sgi * 16);
-}
-
-// match[f] = 1 when field f's nonzero flags equal `ref` (then its norm and
-// analytic sums are the field-independent ones), else 0.  One workgroup per
-// field of the batch.
-__global__ __launch_bounds__(256) void k_flags_match(const int *__restrict__ nzflag,
-                                                     const int *__restrict__ ref, int n,
-                                                     int *__restrict__ match, BatchStrides st)
-{
-    const int fb = blockIdx.x;
-    nzflag += fb * st.nz;
-    int diff = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) diff |= (nzflag[i] != 0) != (ref[i] != 0);
-    diff = __syncthreads_or(diff);
-    if (threadIdx.x == 0) match[fb] = diff ? 0 : 1;
 }
 
 // The field-independent coefficients of every record, computed once (the
@@ -394,10 +390,8 @@ hipError_t launch_binning(const int32_t *sg, int nseg, const int32_t *bin_sg, co
     if (nseg > 0 && !W && bst.binpart_s) {
         // S sums for every field; the full pass only for fields whose flags
         // differ from the reference.
-        hipLaunchKernelGGL(k_flags_match, dim3(nf), dim3(256), 0, s, nzflag, bst.ref, bst.nflags,
-                           bst.match, st);
-        hipLaunchKernelGGL(k_binning_s, dim3(nseg, nf), dim3(kBinThreads), 0, s, sg, recs, S, avg,
-                           bp, bst.binpart_s, st);
+        hipLaunchKernelGGL(k_binning_s, dim3(nseg, nf), dim3(kBinThreads), 0, s, sg, nseg, recs, S,
+                           avg, bp, nzflag, bst, st);
         hipLaunchKernelGGL(k_binning, dim3(nseg, nf), dim3(kBinThreads), 0, s, sg, recs, S, num,
                            nzflag, avg, bp, binpart, st, (const int *)bst.match);
         hipError_t e = hipGetLastError();

@@ -151,9 +151,11 @@ struct BinParams {
 // Launchers (fft.hip, contract.hip, binning.hip).
 // nf fields of NN * NN * (NN/2+1) complex values each, back to back.
 // zero (may be null): nzero ints the gather also sets to 0 (the fields'
-// nonzero flags, which the row pass then raises).
+// nonzero flags, which the ring pass then raises); one (may be null): nf
+// ints it sets to 1 (the fields' flag-match results, which binning clears).
 hipError_t launch_gather(const double2 *fields, double2 *planes, int NN, int NG, int kmax,
-                         int Nmax, int nf, hipStream_t s, int *zero = nullptr, int nzero = 0);
+                         int Nmax, int nf, hipStream_t s, int *zero = nullptr, int nzero = 0,
+                         int *one = nullptr);
 // Per-field strides of a batched launch (blockIdx.y, or blockIdx.z for the
 // gather, selects the field): elements between consecutive fields' buffers.
 struct BatchStrides {
@@ -230,7 +232,9 @@ constexpr int kBinSeg = 2048;
 // unused) for the reference flags `ref`; launch_binning then accumulates
 // the 16 S sums per bin alone (binpart_s, [segment][16] per field) and runs
 // the full 48-sum pass only for fields whose flags differ (match[f] = 0,
-// decided on the device).  The bits are the same either way.
+// decided on the device: launch_gather sets match[f] = 1 when it clears
+// the flags, the S pass clears it on a difference).  The bits are the same
+// either way.
 struct BinStatic {
     const int *ref = nullptr;
     int nflags = 0;

@@ -1698,9 +1698,10 @@ int enqueue_kernels(bb_estimator *E, int nf, const double *d_fields, double *d_b
     const Layout &L = *E->L;
     hipStream_t s = E->stream;
     const int nz = d.g.Nmax * d.g.Nr * nf;
-    // The gather also clears the nonzero flags that the row pass raises.
+    // The gather also clears the nonzero flags that the ring pass raises and
+    // sets the flag-match results that the binning clears.
     HIPCHK(launch_gather((const double2 *)d_fields, E->d_planes, L.NN, L.NGtop, L.kmax,
-                         d.g.Nmax, nf, s, E->d_nz, nz));
+                         d.g.Nmax, nf, s, E->d_nz, nz, E->bstat.match));
     if (timed) HIPCHK(hipEventRecord(E->ev[1], s));
     int rc = run_fields(E, E->d_planes, L.NGtop, E->d_nz, nf);
     if (rc) return rc;
@@ -1733,7 +1734,7 @@ int enqueue_pre(bb_estimator *E, int nf, const double *d_fields)
     const Layout &L = *E->L;
     const int nz = d.g.Nmax * d.g.Nr * nf;
     HIPCHK(launch_gather((const double2 *)d_fields, E->d_planes, L.NN, L.NGtop, L.kmax,
-                         d.g.Nmax, nf, E->stream, E->d_nz, nz));
+                         d.g.Nmax, nf, E->stream, E->d_nz, nz, E->bstat.match));
     return run_fields(E, E->d_planes, L.NGtop, E->d_nz, nf);
 }
 int enqueue_post(bb_estimator *E, int nf, double *d_bs, double *d_norm, double *d_biana)

@@ -34,14 +34,15 @@ namespace bb {
 __global__ __launch_bounds__(256) void k_gather(const double2 *__restrict__ fields,
                                                 double2 *__restrict__ planes_all, int NN, int NG,
                                                 int kmax, int Nmax, int *__restrict__ zero,
-                                                int nzero)
+                                                int nzero, int *__restrict__ one, int none)
 {
     __shared__ double2 tile[32][33];
-    if (zero) {   // the flags to clear, one thread each over the whole grid
+    if (zero) {   // the flags to clear (and set), one thread each over the whole grid
         const long long gid =
             ((long long)(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * 256 +
             threadIdx.x;
         if (gid < nzero) zero[gid] = 0;
+        if (one && gid < none) one[gid] = 1;
     }
     const int H = NN / 2 + 1;
     const int f = blockIdx.z / NG, i = blockIdx.z - f * NG;   // field of the batch, row
@@ -74,7 +75,7 @@ __global__ __launch_bounds__(256) void k_gather(const double2 *__restrict__ fiel
 // than fit are gathered by several launches, the first of which clears the
 // flags.
 hipError_t launch_gather(const double2 *fields, double2 *planes, int NN, int NG, int kmax,
-                         int Nmax, int nf, hipStream_t s, int *zero, int nzero)
+                         int Nmax, int nf, hipStream_t s, int *zero, int nzero, int *one)
 {
     if (NG < 1 || NG > 65535 || nf < 1) return hipErrorInvalidValue;
     const int per = 65535 / NG;
@@ -84,9 +85,10 @@ hipError_t launch_gather(const double2 *fields, double2 *planes, int NN, int NG,
         dim3 grid((NG + 31) / 32, (Nmax + 31) / 32, NG * n);
         int *z = f0 == 0 ? zero : nullptr;
         const int nz = f0 == 0 ? nzero : 0;
-        if ((long long)grid.x * grid.y * grid.z * 256 < nz) return hipErrorInvalidValue;
+        if ((long long)grid.x * grid.y * grid.z * 256 < std::max(nz, nf)) return hipErrorInvalidValue;
         hipLaunchKernelGGL(k_gather, grid, dim3(256), 0, s, fields + f0 * fstride,
-                           planes + f0 * pstride, NN, NG, kmax, Nmax, z, nz);
+                           planes + f0 * pstride, NN, NG, kmax, Nmax, z, nz, f0 == 0 ? one : nullptr,
+                           nf);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
