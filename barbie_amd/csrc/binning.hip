@@ -311,10 +311,12 @@ __global__ __launch_bounds__(kBinThreads) void k_binning_w(const int32_t *__rest
                                                   const double *__restrict__ S,
                                                   const double *__restrict__ num,
                                                   const int *__restrict__ nzflag, BinParams bp,
-                                                  double *__restrict__ binpart, BatchStrides st)
+                                                  double *__restrict__ binpart, BatchStrides st,
+                                                  const int *__restrict__ skip)
 {
     const int fb = blockIdx.x;    // field of the batch
     const int sgi = blockIdx.y;   // segment
+    if (skip && skip[fb]) return;   // uniform over the workgroup
     S += fb * st.part;
     nzflag += fb * st.nz;
     binpart += fb * st.binpart;
@@ -337,6 +339,42 @@ __global__ __launch_bounds__(kBinThreads) void k_binning_w(const int32_t *__rest
                        abi);
     }
     bin_reduce_block(abs_, anm, abi, binpart + (long long)sgi * 48);
+}
+
+// k_binning_s with the weights bs[] read from the table (rows 0..15): the
+// same sums in the same order, so the same bits.  Workgroups of one segment
+// for consecutive fields are adjacent (blockIdx.x the field).
+__global__ __launch_bounds__(kBinThreads) void k_binning_ws(const int32_t *__restrict__ sg,
+                                                   int nseg, const BinRec *__restrict__ recs,
+                                                   int nrec, const double *__restrict__ W,
+                                                   const double *__restrict__ S,
+                                                   const int *__restrict__ nzflag, BinStatic bst,
+                                                   BatchStrides st)
+{
+    const int fb = blockIdx.x;    // field of the batch
+    const int sgi = blockIdx.y;   // segment
+    S += fb * st.part;
+    double *binpart_s = bst.binpart_s + fb * st.binpart_s;
+    const int tid = threadIdx.x;
+    {
+        const int per = (bst.nflags + nseg - 1) / nseg, f0 = sgi * per;
+        int diff = 0;
+        for (int k = tid; k < per && f0 + k < bst.nflags; k += kBinThreads)
+            diff |= (nzflag[fb * st.nz + f0 + k] != 0) != (bst.ref[f0 + k] != 0);
+        if (__any(diff) && (tid & 63) == 0) atomicAnd(&bst.match[fb], 0);
+    }
+    double abs_[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) abs_[k] = 0.0;
+    const int end = sg[sgi + 1];
+    for (int c = sg[sgi] + tid; c < end; c += kBinThreads) {
+        const BinRec R = recs[c];
+        double Sv = 0.0;
+        for (int sp = 0; sp < R.nsp; ++sp) Sv += S[(long long)R.pos + 256 * sp];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) abs_[k] = fma(Sv, W[(long long)k * nrec + c], abs_[k]);
+    }
+    bin_reduce_block16(abs_, binpart_s + (long long)sgi * 16);
 }
 
 // Bin sums: thread (bin, t < 48) adds the bin's segment sums in order;
@@ -387,13 +425,21 @@ hipError_t launch_binning(const int32_t *sg, int nseg, const int32_t *bin_sg, co
 {
     if (bp.nbins <= 0) return hipSuccess;
     if (nf < 1 || nf > 65535) return hipErrorInvalidValue;
-    if (nseg > 0 && !W && bst.binpart_s) {
+    if (nseg > 0 && bst.binpart_s) {
         // S sums for every field; the full pass only for fields whose flags
         // differ from the reference.
-        hipLaunchKernelGGL(k_binning_s, dim3(nseg, nf), dim3(kBinThreads), 0, s, sg, nseg, recs, S,
-                           avg, bp, nzflag, bst, st);
-        hipLaunchKernelGGL(k_binning, dim3(nseg, nf), dim3(kBinThreads), 0, s, sg, recs, S, num,
-                           nzflag, avg, bp, binpart, st, (const int *)bst.match);
+        if (nseg > 65535) return hipErrorInvalidValue;
+        if (W) {
+            hipLaunchKernelGGL(k_binning_ws, dim3(nf, nseg), dim3(kBinThreads), 0, s, sg, nseg, recs,
+                               nrec, W, S, nzflag, bst, st);
+            hipLaunchKernelGGL(k_binning_w, dim3(nf, nseg), dim3(kBinThreads), 0, s, sg, recs, nrec,
+                               W, S, num, nzflag, bp, binpart, st, (const int *)bst.match);
+        } else {
+            hipLaunchKernelGGL(k_binning_s, dim3(nseg, nf), dim3(kBinThreads), 0, s, sg, nseg, recs,
+                               S, avg, bp, nzflag, bst, st);
+            hipLaunchKernelGGL(k_binning, dim3(nseg, nf), dim3(kBinThreads), 0, s, sg, recs, S, num,
+                               nzflag, avg, bp, binpart, st, (const int *)bst.match);
+        }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         const int n = bp.nbins * 48;
@@ -404,7 +450,7 @@ hipError_t launch_binning(const int32_t *sg, int nseg, const int32_t *bin_sg, co
     if (nseg > 0 && W) {
         if (nseg > 65535) return hipErrorInvalidValue;
         hipLaunchKernelGGL(k_binning_w, dim3(nf, nseg), dim3(kBinThreads), 0, s, sg, recs, nrec, W,
-                           S, num, nzflag, bp, binpart, st);
+                           S, num, nzflag, bp, binpart, st, (const int *)nullptr);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     } else if (nseg > 0) {

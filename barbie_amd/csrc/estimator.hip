@@ -1020,7 +1020,7 @@ struct bb_estimator {
     int32_t *d_sg = nullptr, *d_bin_sg = nullptr;
     double *d_binpart = nullptr;      // [segment][48] per field
     double *d_bincoef = nullptr;      // [33][records] binning coefficients (batches), or null
-    BinStatic bstat;                  // field-independent norm and analytic sums (single fields)
+    BinStatic bstat;                  // field-independent norm and analytic sums
     double *d_num = nullptr;
     std::vector<double> num_host;
     int *d_nz = nullptr;              // [Nmax * Nr]
@@ -1517,9 +1517,8 @@ size_t layout_bytes(const PlanData &d, const Layout &L, size_t F)
     b += L.recs.size() * sizeof(BinRec) + (L.sg.size() + L.bin_sg.size()) * 4;
     b += (L.sg.size() - 1) * 48 * 8 * (size_t)F;
     b += bin_table_bytes(L, F);
-    if (bin_table_bytes(L, F) == 0)   // field-independent bins (static_bins)
-        b += (L.sg.size() - 1) * 16 * 8 * (size_t)F + (size_t)3 * d.nbins * BB_NPOLES * 8 +
-             (size_t)Nmax * Nr * 4 + (size_t)F * 4;
+    b += (L.sg.size() - 1) * 16 * 8 * (size_t)F + (size_t)3 * d.nbins * BB_NPOLES * 8 +
+         (size_t)Nmax * Nr * 4 + (size_t)F * 4;   // field-independent bins (static_bins)
     b += d.tri.size() / 3 * 8 + Nr * 8;
     return b;
 }
@@ -1683,7 +1682,7 @@ int estimator_init(bb_estimator *E)
     // written by the transforms: zero them (and everything else) once.
     HIPCHK(hipMemsetAsync(E->d_cache, 0, cpts * (E->cprec == CPrec::F64 ? 16 : 8), E->stream));
     if (E->d_split) HIPCHK(hipMemsetAsync(E->d_split, 0, cpts * 8, E->stream));
-    if (!E->d_bincoef && L.sg.size() > 1 && (rc = static_bins(E))) return rc;
+    if (L.sg.size() > 1 && (rc = static_bins(E))) return rc;
     HIPCHK(hipStreamSynchronize(E->stream));
     return BB_OK;
 }
