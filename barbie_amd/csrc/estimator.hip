@@ -51,6 +51,30 @@ bool make_fft_desc(int N, FftDesc &fd)
     return true;
 }
 
+// The ring pass's FFT plan: the radix-2..5 stages, adjacent pairs merged
+// into one composite stage in registers where the product is 6, 8, 9, 10,
+// 12, 15 or 16 (160 = 16 x 10, 108 = 12 x 9, 80 = 16 x 5, 54 = 6 x 9).
+bool make_ring_fft_desc(int N, FftDesc &fd)
+{
+    FftDesc base;
+    if (!make_fft_desc(N, base)) return false;
+    fd = base;
+    fd.nst = 0;
+    for (int s = 0; s < base.nst;) {
+        const int m = s + 1 < base.nst ? base.radix[s] * base.radix[s + 1] : 0;
+        const bool merge = m == 6 || m == 8 || m == 9 || m == 10 || m == 12 || m == 15 || m == 16;
+        fd.radix[fd.nst++] = merge ? m : base.radix[s];
+        s += merge ? 2 : 1;
+    }
+    int p = 1;
+    for (int s = 0; s < fd.nst; ++s) {
+        fd.mNR[s] = udiv_magic((unsigned)(N / fd.radix[s]));
+        fd.mP[s] = udiv_magic((unsigned)p);
+        p *= fd.radix[s];
+    }
+    return true;
+}
+
 bool smooth235(int n)
 {
     if (n < 1) return false;
@@ -1076,7 +1100,7 @@ std::vector<double2> twiddles(int NG)
 int level_desc(const Level &V, int tw_off, int elem, LevelDesc &D)
 {
     std::memset(&D, 0, sizeof D);
-    if (!make_fft_desc(V.NG, D.fd)) return set_error(BB_E_INVALID, "unsupported FFT length");
+    if (!make_ring_fft_desc(V.NG, D.fd)) return set_error(BB_E_INVALID, "unsupported FFT length");
     D.tw_off = tw_off;
     D.cpb = fft_cols_per_block(V.NG, elem);
     while ((1 << D.cpb_log2) < D.cpb) ++D.cpb_log2;

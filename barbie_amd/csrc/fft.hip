@@ -417,7 +417,7 @@ __global__ __launch_bounds__(PassShape<C2>::threads) void k_ringpass(
                 }
             }
             __syncthreads();
-            C2 *res = lds_fft(a, b, cpb, L.fd, tw);
+            C2 *res = lds_fft<C2, true>(a, b, cpb, L.fd, tw);
             const long long fbase = L.off + (long long)T.fid * L.stride + (long long)j0 * N;
             C2 *fld = cache + fbase;
             const int npt = min(cpb, N - j0) * N;
