@@ -53,7 +53,12 @@ bool make_fft_desc(int N, FftDesc &fd)
 
 // The ring pass's FFT plan: the radix-2..5 stages, adjacent pairs merged
 // into one composite stage in registers where the product is 6, 8, 9, 10,
-// 12, 15 or 16 (160 = 16 x 10, 108 = 12 x 9, 80 = 16 x 5, 54 = 6 x 9).
+// 12, 15 or 16, in the order largest odd factor first, then ascending
+// (160 = 10 x 16, 108 = 9 x 12, 80 = 5 x 16, 54 = 9 x 6, 384 = 3 x 8 x 16).
+// A Stockham stage with p = 1 writes its R outputs at a stride of R
+// elements across the lanes; a power of two R puts many lanes on one LDS
+// bank there, so the powers of two go last, where p is large and the
+// writes are contiguous runs.
 bool make_ring_fft_desc(int N, FftDesc &fd)
 {
     FftDesc base;
@@ -66,6 +71,13 @@ bool make_ring_fft_desc(int N, FftDesc &fd)
         fd.radix[fd.nst++] = merge ? m : base.radix[s];
         s += merge ? 2 : 1;
     }
+    auto odd = [](int r) {
+        while (r % 2 == 0) r /= 2;
+        return r;
+    };
+    std::stable_sort(fd.radix, fd.radix + fd.nst, [&](int x, int y) {
+        return odd(x) != odd(y) ? odd(x) > odd(y) : x < y;
+    });
     int p = 1;
     for (int s = 0; s < fd.nst; ++s) {
         fd.mNR[s] = udiv_magic((unsigned)(N / fd.radix[s]));
