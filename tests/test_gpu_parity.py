@@ -363,3 +363,39 @@ def test_bf16x3_three_workgroups_per_cu_within_tolerance(gf):
     S = est.contraction_sums()
     So = o["contr"][:, 6]
     assert np.array_equal(S == 0.0, So == 0.0)
+
+
+def wide_rings():   # NN=96, 6 rings of width 8: about 2200 modes per outer ring
+    return B.Geometry(NN=96, UL_k1=49, LL_k1=1, BW_k1=4, Nmax=6, Nr=6,
+                      ring_rule=B.RINGS_UNIFORM, ring_width=8.0, last_ring_to_nmax=0)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("bf16x3", 1e-5)])
+def test_wide_rings_stage_by_block(prec, tol):
+    """Rings of width 8: the outer rings' fields hold more mode pairs than
+    the ring pass keeps in LDS for a whole field (640 in FP64, 1024 in FP32),
+    so their modes are staged again for every column block, in groups of
+    rows.  Fields within 1e-12 (FP64) of the oracle's Df; bins within 1e-10
+    (FP64) or 1e-5 per multipole (BF16x3) of the oracle."""
+    o = oracle_run(wide_rings, NRi=1)
+    g = o["g"]
+    lo, hi, av = o["edges"]
+    ia = np.fft.fftfreq(g.NN, 1.0 / g.NN)
+    rad = np.sqrt(ia[:, None] ** 2 + ia[None, :] ** 2)
+    assert np.count_nonzero((rad >= lo[5]) & (rad < hi[5])) > 2 * 1024
+    est = B.Estimator(B.Plan(g), prec)
+    bs, nm, bi = est.estimate(o["field"])
+    obs, onm, obi = (o[k].reshape(bs.shape) for k in ("bs", "nm", "bi"))
+    assert rel_err(nm, onm) <= 1e-12
+    assert rel_err(bi, obi) <= 1e-10
+    for ii in range(16):
+        if np.linalg.norm(obs[..., ii]) > 0:
+            assert (np.linalg.norm(bs[..., ii] - obs[..., ii]) / np.linalg.norm(obs[..., ii])
+                    <= tol), ii
+    if prec == "fp64":
+        fields = B.Estimator(B.Plan(g), "fp64", full_grid=True)
+        fields.estimate(o["field"])
+        for p, r in ((1, 5), (0, 4), (3, 5), (0, 0)):
+            ref = O.df(o["field"], g.NN, p, lo[r], hi[r])
+            got = fields.field(0, p, r)
+            assert np.max(np.abs(got - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref))), (p, r)
