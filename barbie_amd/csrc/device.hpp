@@ -38,7 +38,7 @@ struct FieldTask {
     int32_t nlo, nhi; // ring holds ia^2 + ja^2 in [nlo, nhi)
     int32_t zid;      // nonzero flag of (plane, ring): plane * Nr + ring
     int32_t level;    // grid level (index into the LevelDesc table)
-    int32_t forms;    // results the column pass writes: bit 0 the cache's own values, bit 1
+    int32_t forms;    // results the ring pass writes: bit 0 the cache's own values, bit 1
                       // the bf16 hi/lo operands (when the launch has a split buffer)
 };
 

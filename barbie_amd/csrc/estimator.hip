@@ -900,7 +900,7 @@ int build_layout(const PlanData &d, CPrec prec, const bb_options &opt, int waves
         // What each field of the level is read as: the BF16x3 kernel reads
         // pair fields as FP32 values and third fields as bf16 operands; the
         // FP32 kernel of mixed layouts reads all three as FP32 values.  The
-        // column pass writes only those forms.
+        // ring pass writes only those forms.
         if (prec == CPrec::BF16X3) {
             std::vector<uint8_t> role((size_t)Nmax * Nr, 0);
             for (int f = 0; f < 2; ++f)

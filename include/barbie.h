@@ -261,7 +261,8 @@ int bb_estimator_field(bb_estimator *e, int level, int plane, int ring, double *
  * their level (points summed), [13] tile-window slots (items x waves x
  * windows per wave), [14] cost of the busiest waves (windows x chunks, summed
  * over items), [15] points of all plane-ring fields of all levels (the
- * cache), [16] band rows the row pass transforms (summed over fields),
+ * cache), [16] band rows of the fields (the rows whose transforms along j the
+ * ring pass sums, over all fields),
  * [17] pair-product points formed (16 rows per pair block, times the points
  * of its grid), [18] the part of [17] in rows that hold a pair, [19] tile
  * windows times the points of their grid.  The environment variables
