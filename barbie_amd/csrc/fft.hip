@@ -95,13 +95,13 @@ hipError_t launch_gather(const double2 *fields, double2 *planes, int NN, int NG,
     return hipSuccess;
 }
 
-// Threads per workgroup of the ring pass; a column block holds at most
-// kPassMaxPoints points (cpb * N).
-
+// A column block of the ring pass holds at most kPassMaxPoints points
+// (cpb * N).
 constexpr int kPassMaxPoints = 4096;
 
-// The passes transform in the cache's precision: FP64 for FP64 caches, FP32
-// for FP32 caches (the FP32 paths' error budget is far above FP32 rounding).
+// The ring pass transforms in the cache's precision: FP64 for FP64 caches,
+// FP32 for FP32 caches (the FP32 paths' error budget is far above FP32
+// rounding).
 template <typename V2>
 __device__ __forceinline__ V2 cvt2(double2 v);
 template <>
@@ -245,15 +245,14 @@ __device__ __forceinline__ C2 row_sum(const C2 *md, int n, int ja0, int j, int N
     return acc;
 }
 
-// Units (pairs of modes) a workgroup stages once for all of its column
-// blocks; fields with more are staged again for every block, in row groups
-// that fit in the FFT's scratch buffer.
 // Shape of the ring pass per cache type (measured at the NN=384 grid
 // levels): FP32 workgroups of 512 threads with column blocks of up to 2048
-// points and 1024 staged units (three workgroups per CU by LDS); FP64
+// points and room for 1024 units (mode pairs) staged once for all of a
+// workgroup's column blocks (three workgroups per CU by LDS); FP64
 // workgroups of 256 threads with blocks of up to 512 points and 640 units,
 // so that three FP64 workgroups also fit.  Fields with more units than that
-// are staged per block instead (a row holds at most N / 2 + 1 units).
+// are staged again for every block, in groups of rows that fit in the FFT's
+// scratch buffer (a row holds at most N / 2 + 1 units).
 template <typename C2>
 struct PassShape;
 template <>
