@@ -399,3 +399,31 @@ def test_wide_rings_stage_by_block(prec, tol):
             ref = O.df(o["field"], g.NN, p, lo[r], hi[r])
             got = fields.field(0, p, r)
             assert np.max(np.abs(got - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref))), (p, r)
+
+
+# Grid sizes whose ring-pass FFT plans cover every stage radix (largest odd
+# factor first): 30 = 5 x 6, 60 = 5 x 12, 72 = 9 x 8, 90 = 15 x 6,
+# 100 = 5 x 5 x 4, 160 = 10 x 16, 96 = 6 x 16.
+@pytest.mark.parametrize("NN", [30, 60, 72, 90, 96, 100, 160])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-6)])
+def test_ring_pass_fields_every_radix(NN, prec, tol):
+    """Plane-ring fields of the ring pass on the NN grid against the
+    oracle's Df (a direct masked inverse 2-D FFT), relative to the largest
+    magnitude: FP64 within 1e-12, FP32 within 2e-6."""
+    g = B.Geometry(NN=NN, UL_k1=NN // 2, LL_k1=1, BW_k1=1, Nmax=4, Nr=4,
+                   ring_rule=B.RINGS_UNIFORM, ring_width=NN / 10, last_ring_to_nmax=0)
+    lo, hi, av = O.edges_uniform(g.Nr, g.ring_width, g.Nmax, g.last_ring_to_nmax)
+    field = B.make_field(NN, NRi=3)
+    plan = B.Plan(g)
+    est = B.Estimator(plan, prec, full_grid=True)
+    assert est.levels == [NN]
+    est.estimate(field)
+    pc = plan.contractions()
+    used = {(int(p), int(r)) for p, r in zip(pc[:, 0], pc[:, 1])}
+    used |= {(abs(int(p)), int(r)) for p, r in zip(pc[:, 2], pc[:, 3])}
+    used |= {(abs(int(p)), int(r)) for p, r in zip(pc[:, 4], pc[:, 5])}
+    assert len(used) >= 8
+    for p, r in sorted(used):
+        ref = O.df(field, NN, p, lo[r], hi[r])
+        got = est.field(0, p, r)
+        assert np.max(np.abs(got - ref)) <= tol * max(1.0, np.max(np.abs(ref))), (p, r)
