@@ -271,8 +271,8 @@ namespace {
 // rings take the first rows as contiguous ranges, and the pair rings the
 // remaining rows, assigned greedily so that the distinct D1 rings of each
 // block, and its distinct D2 rings, fall into distinct groups where
-// possible (empty A rows read the zero ring, row nrows, which counts as a
-// member of their block).  Otherwise every range is contiguous, in key order.
+// possible (empty A rows read zeros in bank group nrows mod 16, which
+// counts as a member of their block).  Otherwise every range is contiguous, in key order.
 template <class KeyMap, class Key>
 void assign_rows(const std::vector<PairBlock> &blocks, const std::vector<size_t> &item_blocks,
                  const KeyMap &keys, bool split_b, int nrows, std::map<Key, int> &row0,
