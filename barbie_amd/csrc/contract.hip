@@ -168,8 +168,8 @@ __host__ __device__ constexpr int dma_lds_bytes(int nw, int tww)
 static_assert((rowcap_for(4) + 15) / 16 <= kDmaGroups * 4 &&
                   (rowcap_for(8) + 15) / 16 <= kDmaGroups * 8,
               "every 16-ring group of an item has a wave");
-static_assert(3 * dma_lds_bytes(4, kTWWOcc3) <= 160 * 1024, "3 workgroups' LDS per CU");
-static_assert(2 * dma_lds_bytes(4, kTWW32) <= 160 * 1024 && dma_lds_bytes(8, kTWW32) <= 160 * 1024,
+static_assert(lds_resident(dma_lds_bytes(4, kTWWOcc3)) >= 3, "3 workgroups' LDS per CU");
+static_assert(lds_resident(dma_lds_bytes(4, kTWW32)) >= 2 && lds_resident(dma_lds_bytes(8, kTWW32)) >= 1,
               "2 (4 waves) or 1 (8 waves) workgroups' LDS per CU");
 
 __device__ __forceinline__ int dma_off(int r, int k)

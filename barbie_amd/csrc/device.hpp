@@ -90,6 +90,16 @@ __host__ __device__ constexpr int rowcap_for(int waves) { return 30 * waves; }
 __host__ __device__ constexpr int rowcap_occ3(int waves) { return 24 * waves - 1; }
 constexpr int kTWWOcc3 = 6;   // tile windows per wave, BF16x3 at 3 workgroups per CU
 
+// Workgroups of `lds` bytes of LDS that fit one CU (160 KiB).  The LDS is
+// allocated in granules of 1280 bytes: workgroups of 53 920 bytes ran 2 per
+// CU although the occupancy query reports 3 (up to 54 613 bytes), and 53 536
+// bytes ran 3.
+constexpr int kLdsPerCU = 160 * 1024, kLdsGranule = 1280;
+__host__ __device__ constexpr int lds_resident(long long lds)
+{
+    return lds <= 0 ? 1 << 20 : (int)(kLdsPerCU / ((lds + kLdsGranule - 1) / kLdsGranule * kLdsGranule));
+}
+
 enum : int16_t { TW_CONJ2 = 1, TW_CONJ3 = 2, TW_NEWA = 4 };
 
 struct TWin {

@@ -491,6 +491,7 @@ static hipError_t launch_fft(const double2 *planes, int NP, C2 *cache, __bf16 *s
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, PassShape<C2>::threads, lds) !=
             hipSuccess)
         return hipErrorInvalidValue;
+    per_cu = std::min(per_cu, lds_resident((long long)lds));
     const long long round = (long long)std::max(1, ncu) * std::max(1, per_cu);
     const int gx = (int)std::max(1LL, std::min<long long>(fb.n, round / nf));
     hipLaunchKernelGGL(k, dim3(gx, nf), dim3(PassShape<C2>::threads), lds, s, planes, NP, cache, split,
