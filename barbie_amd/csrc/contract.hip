@@ -144,7 +144,7 @@ struct Stager {
 // the tile's rings.
 constexpr int kDmaGroups = 2;   // 16-ring groups per wave (rowcap / 16 / waves, rounded up)
 
-// Staged rings per item: 120 (4 waves), 240 (8 waves), 95 (4 waves sized
+// Staged rings per item: 120 (4 waves), 240 (8 waves), 96 (4 waves sized
 // for 3 workgroups per CU, kTWWOcc3 tile windows).
 __host__ __device__ constexpr int dma_rowcap(int nw, int tww)
 {

@@ -85,9 +85,9 @@ constexpr int kTWWF32 = 6;      // tile windows per wave, FP32 form
 constexpr int kMaxBlk = 48;     // ring blocks per item
 constexpr int kRowCap = 240;    // staged rings per item (8 waves; 30 per wave)
 __host__ __device__ constexpr int rowcap_for(int waves) { return 30 * waves; }
-// Staged rings per item of the BF16x3 kernel sized for 3 workgroups per CU.
-// (One ring slot of the 96 in the LDS buffer is kept for the zero ring.)
-__host__ __device__ constexpr int rowcap_occ3(int waves) { return 24 * waves - 1; }
+// Staged rings per item of the BF16x3 kernel sized for 3 workgroups per CU
+// (six groups of 16 in each LDS chunk buffer).
+__host__ __device__ constexpr int rowcap_occ3(int waves) { return 24 * waves; }
 constexpr int kTWWOcc3 = 6;   // tile windows per wave, BF16x3 at 3 workgroups per CU
 
 // Workgroups of `lds` bytes of LDS that fit one CU (160 KiB).  The LDS is

@@ -92,7 +92,7 @@ typedef struct bb_options {
                        estimator's own buffers, so they always replay) */
     int occupancy;  /* BF16X3 with 4 waves only: contraction workgroups per
                        compute unit the kernel is sized for, 2 (0: default) or
-                       3 (6 instead of 8 tile windows per wave and at most 95
+                       3 (6 instead of 8 tile windows per wave and at most 96
                        instead of 120 staged rings per work item, so that three
                        workgroups' registers and LDS fit on one CU) */
 } bb_options;

@@ -347,14 +347,14 @@ def test_mixed_bf16_fp32_layout_within_tolerance(gf):
 @pytest.mark.parametrize("gf", [medium, config1])
 def test_bf16x3_three_workgroups_per_cu_within_tolerance(gf):
     """The BF16x3 kernel sized for 3 workgroups per CU (6 tile windows per
-    wave, at most 95 staged rings per item): within the BF16x3 bar of the FP64
+    wave, at most 96 staged rings per item): within the BF16x3 bar of the FP64
     oracle, and the same zero pattern of the sums."""
     o = oracle_run(gf, NRi=1)
     g = o["g"]
     plan = B.Plan(g)
     est = B.Estimator(plan, "bf16x3", waves=4, occupancy=3)
     wl = est.worklist()
-    assert wl["max_rings"] <= 95 and wl["waves"] == 4
+    assert wl["max_rings"] <= 96 and wl["waves"] == 4
     bs, nm, bi = est.estimate(o["field"])
     obs, onm = (o[k].reshape(bs.shape) for k in ("bs", "nm"))
     errs = [np.linalg.norm(bs[..., i] - obs[..., i]) / np.linalg.norm(obs[..., i]) for i in range(16)]

@@ -129,7 +129,7 @@ def test_occupancy_option_is_validated():
         assert B.lib().bb_plan_worklist(plan.handle, B.native.ctypes.byref(o), out) == -1
     w3 = plan.worklist("bf16x3", waves=4, occupancy=3)
     w2 = plan.worklist("bf16x3", waves=4)
-    assert w3["max_rings"] <= 95
+    assert w3["max_rings"] <= 96
     assert w3["tiles"] == w2["tiles"]
 
 
