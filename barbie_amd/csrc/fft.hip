@@ -179,7 +179,8 @@ __device__ __forceinline__ void store8(float2 *dst, __bf16 *sp, const float2 *v)
 //     w = exp(+2 pi i / N) (a ring holds few modes per row: the sum costs
 //     about as much as the row FFT would, and nothing is written to memory
 //     between the two directions); rows outside the band are zero;
-//  2. columns: the cpb transforms along i (Stockham, radices 4, 2, 3, 5);
+//  2. columns: the cpb transforms along i (Stockham stages in LDS, radices
+//     2 to 5 and their two-stage composites 6 to 16, lds_fft);
 //  3. stores: the block's cpb * N contiguous points, in the cache's type
 //     and, for FP32 caches with `split`, as bf16 hi/lo operands, each in the
 //     forms the contraction reads the field in.
