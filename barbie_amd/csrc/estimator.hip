@@ -348,6 +348,63 @@ void assign_rows(const std::vector<PairBlock> &blocks, const std::vector<size_t>
     }
 }
 
+// Tile rows of a pair block's pairs in the BF16x3 kernel.  ds_read_b128
+// serves the 16 rows of an A tile in phases that hold rows {0-3, 12-15} of
+// one quarter of the chunk and rows {4-11} of the next (contract.hip), so a
+// ring read by rows of both halves is two addresses in one bank group.
+// Rows that share a D1 ring or a D2 ring (transitively) are placed in one
+// half where they fit: such groups largest first, each into the half with
+// more room (the other if only that one holds it; a group larger than both
+// is split); empty rows take the slots left.  Returns row m -> tile row.
+std::array<int, 16> tile_rows(const PairBlock &B)
+{
+    static const int half[2][8] = {{0, 1, 2, 3, 12, 13, 14, 15}, {4, 5, 6, 7, 8, 9, 10, 11}};
+    int parent[16];
+    for (int m = 0; m < 16; ++m) parent[m] = m;
+    auto root = [&](int x) {
+        while (parent[x] != x) x = parent[x] = parent[parent[x]];
+        return x;
+    };
+    for (int a = 0; a < 16; ++a)
+        for (int b = a + 1; b < 16; ++b)
+            if (B.r1[a] >= 0 && B.r1[b] >= 0 &&
+                ((B.pa[a] == B.pa[b] && B.r1[a] == B.r1[b]) ||
+                 (B.pb[a] == B.pb[b] && B.r2[a] == B.r2[b])))
+                parent[root(a)] = root(b);
+    std::vector<std::vector<int>> groups;
+    int gid[16];
+    for (int m = 0; m < 16; ++m) gid[m] = -1;
+    for (int m = 0; m < 16; ++m) {
+        if (B.r1[m] < 0) continue;
+        const int r = root(m);
+        if (gid[r] < 0) {
+            gid[r] = (int)groups.size();
+            groups.emplace_back();
+        }
+        groups[gid[r]].push_back(m);
+    }
+    std::stable_sort(groups.begin(), groups.end(),
+                     [](const std::vector<int> &x, const std::vector<int> &y) { return x.size() > y.size(); });
+    std::array<int, 16> to;
+    to.fill(-1);
+    int used[2] = {0, 0};
+    for (const std::vector<int> &g : groups) {
+        const int n = (int)g.size();
+        int h = used[0] <= used[1] ? 0 : 1;
+        if (used[h] + n > 8 && used[1 - h] + n <= 8) h = 1 - h;
+        for (int m : g) {
+            if (used[h] == 8) h = 1 - h;
+            to[m] = half[h][used[h]++];
+        }
+    }
+    for (int m = 0; m < 16; ++m)
+        if (to[m] < 0) {
+            const int h = used[0] < 8 ? 0 : 1;
+            to[m] = half[h][used[h]++];
+        }
+    return to;
+}
+
 // Packs pair blocks of one level (already grouped so that neighbours share
 // planes) into work items.  split_b: stage the r3 rings as bf16 hi/lo
 // operands (BF16x3) instead of raw values.  pos[cid] receives tile * 256 +
@@ -441,12 +498,15 @@ int pack_items(std::vector<PairBlock> &blocks, const Level &L, bool split_b, int
                 const PairBlock &B = blocks[bi];
                 const int rc = row0.at({B.pc, split_b ? 1 : 0});
                 const int tile0 = W.ntw;
+                std::array<int, 16> to;
+                for (int m = 0; m < 16; ++m) to[m] = m;
+                if (split_b) to = tile_rows(B);
                 for (int v = 0; v < B.nwin; ++v) {
                     TWin &T = W.tws[it.tw0 + w * kTWW + j];
                     std::memset(&T, 0, sizeof T);
                     for (int m = 0; m < 16; ++m) {
-                        T.a1[m] = (int16_t)(B.r1[m] >= 0 ? prow.at({B.pa[m], B.r1[m]}) : -1);
-                        T.a2[m] = (int16_t)(B.r2[m] >= 0 ? prow.at({B.pb[m], B.r2[m]}) : 0);
+                        T.a1[to[m]] = (int16_t)(B.r1[m] >= 0 ? prow.at({B.pa[m], B.r1[m]}) : -1);
+                        T.a2[to[m]] = (int16_t)(B.r2[m] >= 0 ? prow.at({B.pb[m], B.r2[m]}) : 0);
                     }
                     const int ws = B.lo + 16 * v;
                     T.b0 = (int16_t)(rc + ws);
@@ -457,7 +517,7 @@ int pack_items(std::vector<PairBlock> &blocks, const Level &L, bool split_b, int
                 }
                 for (const auto &c : B.cs) {
                     const int d = c[1] - B.lo;
-                    pos[c[2]] = (tile0 + d / 16) * 256 + c[0] * 16 + d % 16;
+                    pos[c[2]] = (tile0 + d / 16) * 256 + to[c[0]] * 16 + d % 16;
                 }
             }
             it.cnt[w] = (int8_t)j;
