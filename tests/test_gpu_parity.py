@@ -344,17 +344,24 @@ def test_mixed_bf16_fp32_layout_within_tolerance(gf):
     assert wl["units"] > B.Estimator(plan, "bf16x3").worklist()["units"] // 4
 
 
+# (waves, occupancy) -> most staged rings per item of that BF16x3 kernel shape
+BF16X3_SHAPES = {(4, 3): 96, (4, 0): 120, (8, 0): 240}
+
+
+@pytest.mark.parametrize("shape", sorted(BF16X3_SHAPES))
 @pytest.mark.parametrize("gf", [medium, config1])
-def test_bf16x3_three_workgroups_per_cu_within_tolerance(gf):
-    """The BF16x3 kernel sized for 3 workgroups per CU (6 tile windows per
-    wave, at most 96 staged rings per item): within the BF16x3 bar of the FP64
+def test_bf16x3_kernel_shapes_within_tolerance(gf, shape):
+    """Each BF16x3 kernel shape (4 waves and 6 tile windows per wave sized for
+    3 workgroups per CU, at most 96 staged rings per item; 4 waves and 8
+    windows, 120 rings; 8 waves, 240 rings): within the BF16x3 bar of the FP64
     oracle, and the same zero pattern of the sums."""
     o = oracle_run(gf, NRi=1)
     g = o["g"]
     plan = B.Plan(g)
-    est = B.Estimator(plan, "bf16x3", waves=4, occupancy=3)
+    waves, occ = shape
+    est = B.Estimator(plan, "bf16x3", waves=waves, occupancy=occ)
     wl = est.worklist()
-    assert wl["max_rings"] <= 96 and wl["waves"] == 4
+    assert wl["max_rings"] <= BF16X3_SHAPES[shape] and wl["waves"] == waves
     bs, nm, bi = est.estimate(o["field"])
     obs, onm = (o[k].reshape(bs.shape) for k in ("bs", "nm"))
     errs = [np.linalg.norm(bs[..., i] - obs[..., i]) / np.linalg.norm(obs[..., i]) for i in range(16)]
